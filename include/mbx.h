@@ -1,0 +1,186 @@
+/*
+ * mbx.h -- C ABI of libmbx, the MI355X (gfx950) kernel library behind the
+ * mbirjax_amd TomographyModel API.
+ *
+ * Every entry point is stream-ordered on the caller's hipStream_t (passed as
+ * void*; NULL means the default stream), allocates no device memory unless it
+ * says so, validates its scalar arguments before any HIP call, and returns an
+ * int status: 0 = OK, negative = error (see MBX_E*).  mbx_last_error() returns
+ * a thread-local message describing the most recent failure.
+ *
+ * Array conventions (all device pointers):
+ *   - pixel indices: int32, flat index p = i*num_recon_cols + j into the 2D
+ *     (rows, cols) recon grid (C order), every entry in [0, rows*cols).
+ *   - voxel cylinders: float32 (npix, num_recon_slices), row-major.
+ *   - sinograms: views x channels x rows ("channel-major", MBX_LAYOUT_VCR) or
+ *     the user-facing views x rows x channels (MBX_LAYOUT_VRC); float32 or
+ *     bfloat16 (MBX_DTYPE_*).  The projectors read/write MBX_LAYOUT_VCR;
+ *     mbx_sino_transpose converts.
+ *   - angles: float32 (num_views,), radians.
+ *
+ * Reference interfaces replaced (mbirjax 0.6.7, paths relative to its root):
+ *   mbx_forward_project  <- Projectors.sparse_forward_project
+ *                           (mbirjax/projectors.py:61-129) over the geometry
+ *                           plugins ConeBeamModel.forward_project_pixel_batch_to_one_view
+ *                           (mbirjax/cone_beam.py:189-377) and
+ *                           ParallelBeamModel.forward_project_pixel_batch_to_one_view
+ *                           (mbirjax/parallel_beam.py:138-178)
+ *   mbx_back_project     <- Projectors.sparse_back_project
+ *                           (mbirjax/projectors.py:131-209) over
+ *                           ConeBeamModel.back_project_one_view_to_pixel_batch
+ *                           (mbirjax/cone_beam.py:379-541) and
+ *                           ParallelBeamModel.back_project_one_view_to_pixel_batch
+ *                           (mbirjax/parallel_beam.py:180-221); coeff_power=2 is
+ *                           TomographyModel.compute_hessian_diagonal
+ *                           (mbirjax/tomography_model.py:679-708)
+ *   mbx_qggmrf_grad_hess <- qggmrf_gradient_and_hessian_at_indices
+ *                           (mbirjax/qggmrf.py:118-279)
+ *   mbx_vcd_subset_update<- TomographyModel.create_vcd_subset_updater /
+ *                           vcd_subset_updater (mbirjax/tomography_model.py:1310-1544)
+ *                           with get_forward_lin_quad (1546-1631) and
+ *                           update_recon (1820-1823)
+ *   mbx_dot              <- the jnp.sum reductions of vcd_recon / get_forward_model_loss
+ *                           (mbirjax/tomography_model.py:1139-1152, 1633-1658)
+ *
+ * Python binds these through ctypes (mbirjax_amd/_lib.py).
+ */
+#ifndef MBX_H
+#define MBX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MBX_OK 0
+#define MBX_EINVAL (-1)       /* invalid argument (checked before any HIP call) */
+#define MBX_ELAUNCH (-2)      /* a HIP launch or runtime call failed */
+#define MBX_EUNSUPPORTED (-3) /* valid but unsupported combination */
+#define MBX_EWORKSPACE (-4)   /* workspace missing or too small */
+
+#define MBX_GEOM_PARALLEL 0
+#define MBX_GEOM_CONE 1
+
+#define MBX_DTYPE_F32 0
+#define MBX_DTYPE_BF16 1
+
+#define MBX_LAYOUT_VCR 0 /* (views, channels, rows): projector-native */
+#define MBX_LAYOUT_VRC 1 /* (views, rows, channels): mbirjax user layout */
+
+/* Geometry of one TomographyModel, mirroring the reference's projector_params
+ * = (sinogram_shape, recon_shape, geometry_params): ConeBeamModel.get_geometry_parameters
+ * (mbirjax/cone_beam.py:110-136) and ParallelBeamModel.get_geometry_parameters
+ * (mbirjax/parallel_beam.py:94-113).  Doubles are rounded to float32 on the
+ * device, as the reference's jitted float32 arithmetic does. */
+typedef struct mbx_geom {
+    int32_t kind;             /* MBX_GEOM_PARALLEL or MBX_GEOM_CONE */
+    int32_t num_views;        /* length of the angles array */
+    int32_t num_det_rows;     /* R */
+    int32_t num_det_channels; /* C */
+    int32_t num_recon_rows;
+    int32_t num_recon_cols;
+    int32_t num_recon_slices; /* Nz */
+    int32_t psf_radius;       /* channel (and cone back-projection row) tap radius */
+    int32_t bp_psf_radius;    /* cone forward-projection slice tap radius */
+    int32_t reserved;
+    double delta_det_row;
+    double delta_det_channel;
+    double det_row_offset;
+    double det_channel_offset;
+    double source_detector_dist; /* may be +inf (cone) ; unused for parallel */
+    double magnification;        /* source_detector_dist / source_iso_dist; 1 for parallel */
+    double delta_voxel;
+    double recon_slice_offset;
+} mbx_geom;
+
+/* qGGMRF prior parameters: b = get_b_from_nbr_wts(qggmrf_nbr_wts) in the order
+ * [row+1, row-1, col+1, col-1, slice+1, slice-1] (mbirjax/qggmrf.py:388-395). */
+typedef struct mbx_qggmrf {
+    double b[6];
+    double sigma_x;
+    double p;
+    double q;
+    double T;
+} mbx_qggmrf;
+
+int mbx_version(void);
+const char *mbx_last_error(void);
+
+/* Number of device bytes mbx_back_project needs as workspace (0 = none). */
+int64_t mbx_back_workspace_bytes(const mbx_geom *g, int32_t npix, int32_t nviews);
+
+/* Sparse forward projection (mbirjax/projectors.py:61-129).
+ * vox: (npix, Nz) f32; pix: (npix,) int32.  views: NULL = all num_views views,
+ * else nviews int32 indices into angles; output view slot i uses angle views[i].
+ * sino: (nviews_out, C, R) in sino_dtype, layout sino_layout.
+ * accumulate = 0 writes the projection, 1 adds it to sino.
+ * sumsq (nullable, device double[1]): receives sum over the written
+ * projection values of value^2 (times weights[...] when weights != NULL; weights
+ * then has sino's shape, f32, layout sino_layout); this is forward_quadratic
+ * of get_forward_lin_quad before the 1/sigma_y^2 factor.  sumsq needs a
+ * workspace of mbx_forward_workspace_bytes() bytes (deterministic per-block
+ * partials); with sumsq == NULL the workspace may be NULL. */
+int64_t mbx_forward_workspace_bytes(const mbx_geom *g, int32_t npix, int32_t nviews);
+int mbx_forward_project(const mbx_geom *g, const float *angles, const float *vox, const int32_t *pix,
+                        int32_t npix, const int32_t *views, int32_t nviews, void *sino, int32_t sino_dtype,
+                        int32_t sino_layout, int32_t accumulate, const float *weights, double *sumsq,
+                        void *workspace, int64_t workspace_bytes, void *stream);
+
+/* Sparse back projection (mbirjax/projectors.py:131-209, tomography_model.py:624-677).
+ * sino: (num_sino_views, C, R) in sino_dtype, MBX_LAYOUT_VCR.  views: NULL =
+ * views 0..num_sino_views-1 (then num_sino_views must equal g->num_views), else
+ * nviews indices that select both the sinogram view and its angle (the
+ * reference's view_indices semantics, tomography_model.py:661-663).
+ * weights (nullable) multiplies the sinogram element-wise (same shape, f32).
+ * coeff_power 1 or 2 (2 = Hessian diagonal).  kahan = 1 uses compensated
+ * summation across views.  out (npix, Nz) f32 = (accumulate ? out : 0) +
+ * scale * A^T(weights*sino).  workspace: mbx_back_workspace_bytes() bytes. */
+int mbx_back_project(const mbx_geom *g, const float *angles, const void *sino, int32_t sino_dtype,
+                     int32_t num_sino_views, const float *weights, const int32_t *pix, int32_t npix,
+                     const int32_t *views, int32_t nviews, int32_t coeff_power, int32_t kahan, float scale,
+                     int32_t accumulate, float *out, void *workspace, int64_t workspace_bytes, void *stream);
+
+/* Convert a sinogram between layouts (and dtypes).  src and dst must not alias. */
+int mbx_sino_transpose(const void *src, int32_t src_dtype, int32_t src_layout, void *dst, int32_t dst_dtype,
+                       int32_t dst_layout, int32_t num_views, int32_t num_rows, int32_t num_channels,
+                       void *stream);
+
+/* qGGMRF surrogate gradient and Hessian diagonal at the cylinders pix
+ * (mbirjax/qggmrf.py:118-279): grad, hess (npix, Nz) f32 are overwritten. */
+int mbx_qggmrf_grad_hess(const float *flat_recon, int32_t num_rows, int32_t num_cols, int32_t num_slices,
+                         const int32_t *pix, int32_t npix, const mbx_qggmrf *params, float *grad, float *hess,
+                         void *stream);
+
+/* Deterministic FP64 reduction: out[0] = sum_i a[i]*b[i]*(w ? w[i] : 1),
+ * with b == NULL meaning b = a.  a, b in their dtypes, w f32.  n elements.
+ * The workspace holds per-block partials (mbx_dot_workspace_bytes()). */
+int64_t mbx_dot_workspace_bytes(void);
+int mbx_dot(const void *a, int32_t a_dtype, const void *b, int32_t b_dtype, const float *w, int64_t n,
+            double *out, void *workspace, int64_t workspace_bytes, void *stream);
+
+/* e = e - alpha[0] * d  (alpha a device double), elementwise over n entries of
+ * the error sinogram e (e_dtype) and the delta sinogram d (d_dtype). */
+int mbx_sino_axpy(void *e, int32_t e_dtype, const void *d, int32_t d_dtype, const double *alpha, int64_t n,
+                  void *stream);
+
+/* One VCD subset update: vcd_subset_updater (mbirjax/tomography_model.py:1343-1542).
+ * State: flat_recon (rows*cols, Nz) f32 and the error sinogram e (V, C, R) in
+ * e_dtype (MBX_LAYOUT_VCR), e = y - A x.  weights: NULL = constant 1, else
+ * (V, C, R) f32.  fm_hessian: (rows*cols, Nz) f32 = diag(A^T W A).
+ * prox_input: NULL = qGGMRF prior, else (rows*cols, Nz) f32 proximal-map input
+ * with prior Hessian sigma_prox^2.  delta_dtype: storage of the delta sinogram.
+ * stats (device double[4]) accumulates {sum |alpha*delta|, alpha, 1, 0} for
+ * per-iteration statistics.  workspace: mbx_vcd_workspace_bytes() bytes. */
+int64_t mbx_vcd_workspace_bytes(const mbx_geom *g, int32_t npix, int32_t delta_dtype);
+int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_recon, void *e, int32_t e_dtype,
+                          const float *weights, const float *fm_hessian, const int32_t *pix, int32_t npix,
+                          double fm_constant, const mbx_qggmrf *qparams, const float *prox_input,
+                          double sigma_prox, int32_t positivity, int32_t delta_dtype, int32_t kahan,
+                          double *stats, void *workspace, int64_t workspace_bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MBX_H */

@@ -1,0 +1,33 @@
+"""In-tree build of libmbx.so for gfx950 (hipcc)."""
+import os
+import subprocess
+import sys
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+SOURCES = ["csrc/mbx_project.hip", "csrc/mbx_vcd.hip"]
+OUTPUT = os.path.join(_HERE, "libmbx.so")
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+    # packed FP32 instructions stay disabled in every HIP translation unit
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def build(verbose=False):
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    tmp = OUTPUT + ".tmp"
+    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp] + [os.path.join(_HERE, s) for s in SOURCES]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    errs = "\n".join(l for l in res.stderr.splitlines() if "packed-fp32-ops" not in l)
+    if res.returncode != 0:
+        sys.stderr.write(errs + "\n")
+        raise RuntimeError("hipcc failed building libmbx.so")
+    os.replace(tmp, OUTPUT)
+    return OUTPUT
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

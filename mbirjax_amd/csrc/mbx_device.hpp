@@ -1,0 +1,174 @@
+// Device-side building blocks shared by the libmbx kernels (gfx950 / CDNA4).
+//
+// The projector geometry restates ConeBeamModel.compute_horizontal_data,
+// compute_vertical_data_single_pixel and recon_ijk_to_xyz / geometry_xyz_to_uv_mag
+// (mbirjax/cone_beam.py:543-681) and ParallelBeamModel.compute_proj_data
+// (mbirjax/parallel_beam.py:223-284) as one per-(pixel, view) record that both the
+// forward and the back projector evaluate with identical float32 operations, so
+// the two kernels apply exactly transposed operators.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mbx.h"
+
+namespace mbx {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- bfloat16
+// Storage type for the mixed-precision residual sinogram: the upper 16 bits of
+// an IEEE float32, rounded to nearest-even on store.
+struct bf16 {
+    uint16_t bits;
+};
+
+__device__ __forceinline__ float to_float(float x) { return x; }
+__device__ __forceinline__ float to_float(bf16 x) { return __uint_as_float(((uint32_t)x.bits) << 16); }
+
+template <typename T>
+__device__ __forceinline__ T from_float(float x);
+template <>
+__device__ __forceinline__ float from_float<float>(float x) {
+    return x;
+}
+template <>
+__device__ __forceinline__ bf16 from_float<bf16>(float x) {
+    uint32_t u = __float_as_uint(x);
+    bf16 r;
+    if ((u & 0x7fffffffu) > 0x7f800000u) {  // NaN: keep it quiet
+        r.bits = (uint16_t)((u >> 16) | 0x40u);
+        return r;
+    }
+    u += 0x7fffu + ((u >> 16) & 1u);
+    r.bits = (uint16_t)(u >> 16);
+    return r;
+}
+
+// ---------------------------------------------------------------- wave sync
+// LDS hand-off between lanes of one wavefront (rocPRIM's wave_barrier idiom).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+
+// ---------------------------------------------------------------- geometry
+// Float32 copy of mbx_geom with the derived centres precomputed on the host.
+struct DevGeom {
+    int kind, V, R, C, rows, cols, Nz, rh, rb;
+    float dr, dc, row_off, ch_off, inv_sdd, inv_M, dv, slice_off;
+    float row_center;    // (R-1)/2
+    float ch_center;     // (C-1)/2
+    float img_row_center;  // (rows-1)/2
+    float img_col_center;  // (cols-1)/2
+    float slice_center;  // (Nz-1)/2
+    float dv_over_dc, dv_over_dr;
+};
+
+// The view-dependent part of one (pixel, view) pair that both projectors
+// store or recompute: the pixel magnification, the fractional channel n_p,
+// cos(alpha_xy) and cos(theta_p).
+struct PVBase {
+    float mag, np, cosa, ctp;
+};
+
+// Pixel p at a view with cos/sin (cs, sn):  recon_ijk_to_xyz, the pixel
+// magnification 1/(1/M - y/sdd), u = mag*x, n_p = (u + det_channel_offset)/dc +
+// (C-1)/2 and, for the cone, theta_p = atan2(u, sdd) expressed through
+// cos/sin so that cos(angle - theta_p) needs no atan2.
+template <int GEOM>
+__device__ __forceinline__ PVBase pv_base(const DevGeom &g, int p, float cs, float sn) {
+    const int i = p / g.cols;
+    const int j = p - i * g.cols;
+    const float yt = g.dv * ((float)i - g.img_row_center);
+    const float xt = g.dv * ((float)j - g.img_col_center);
+    const float x = cs * xt - sn * yt;
+    PVBase b;
+    if (GEOM == MBX_GEOM_CONE) {
+        const float y = sn * xt + cs * yt;
+        const float mag = 1.0f / (g.inv_M - y * g.inv_sdd);
+        const float u = mag * x;
+        b.mag = mag;
+        b.np = (u + g.ch_off) / g.dc + g.ch_center;
+        const float us = u * g.inv_sdd;
+        const float ctp = 1.0f / sqrtf(1.0f + us * us);  // cos(theta_p)
+        const float stp = us * ctp;                      // sin(theta_p)
+        const float cd = cs * ctp + sn * stp;            // cos(angle - theta_p)
+        const float sd = sn * ctp - cs * stp;            // sin(angle - theta_p)
+        b.cosa = fmaxf(fabsf(cd), fabsf(sd));
+        b.ctp = ctp;
+    } else {
+        b.mag = 1.0f;
+        b.np = (x + g.ch_off) / g.dc + g.ch_center;
+        b.cosa = fmaxf(fabsf(cs), fabsf(sn));
+        b.ctp = 1.0f;
+    }
+    return b;
+}
+
+// Everything the taps need, derived from PVBase.
+struct PV {
+    float np;    // fractional channel
+    float hc;    // (W_c + 1)/2
+    float Lc;    // min(1, W_c)
+    float sc;    // delta_voxel / cos(alpha_xy)
+    // cone vertical fan: row position of slice k is t_k = a + W*k
+    float a, W, hv, Lv, invW;
+    // v_k / sdd = c0 + c1*k, so 1/cos(phi_k) = sqrt(1 + (c0 + c1*k)^2)
+    float c0, c1;
+};
+
+template <int GEOM>
+__device__ __forceinline__ PV pv_full(const DevGeom &g, const PVBase &b) {
+    PV q;
+    q.np = b.np;
+    float Wc;
+    if (GEOM == MBX_GEOM_CONE) {
+        Wc = b.mag * g.dv_over_dc * (b.cosa / b.ctp);
+    } else {
+        Wc = g.dv_over_dc * b.cosa;
+    }
+    q.hc = 0.5f * (Wc + 1.0f);
+    q.Lc = fminf(1.0f, Wc);
+    q.sc = g.dv / b.cosa;
+    if (GEOM == MBX_GEOM_CONE) {
+        const float W = b.mag * g.dv_over_dr;
+        const float z0 = g.slice_off - g.dv * g.slice_center;  // z of slice 0
+        q.W = W;
+        q.invW = 1.0f / W;
+        q.a = (b.mag * z0 + g.row_off) / g.dr + g.row_center;
+        q.hv = 0.5f * (W + 1.0f);
+        q.Lv = fminf(1.0f, W);
+        q.c0 = b.mag * z0 * g.inv_sdd;
+        q.c1 = b.mag * g.dv * g.inv_sdd;
+    } else {
+        q.W = 1.0f;
+        q.invW = 1.0f;
+        q.a = 0.0f;
+        q.hv = 1.0f;
+        q.Lv = 1.0f;
+        q.c0 = 0.0f;
+        q.c1 = 0.0f;
+    }
+    return q;
+}
+
+// Horizontal-fan weight of channel n (ConeBeamModel.forward_horizontal_fan_pixel_batch_to_one_view,
+// mbirjax/cone_beam.py:276-282): delta_voxel * clip((W_c+1)/2 - |n_p - n|, 0, min(1,W_c)) / cos(alpha),
+// zero outside [0, C).
+__device__ __forceinline__ float chan_weight(const PV &q, int n, int C) {
+    const float L = clampf(q.hc - fabsf(q.np - (float)n), 0.0f, q.Lc);
+    return (n >= 0 && n < C) ? q.sc * L : 0.0f;
+}
+
+// Vertical-fan weight between slice position t and detector row m
+// (mbirjax/cone_beam.py:367-368, 529-531) without the 1/cos(phi) factor.
+__device__ __forceinline__ float row_weight(const PV &q, float t_minus_m) {
+    return clampf(q.hv - fabsf(t_minus_m), 0.0f, q.Lv);
+}
+
+}  // namespace mbx

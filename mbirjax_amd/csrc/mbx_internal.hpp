@@ -1,0 +1,72 @@
+// Host-side helpers shared by the libmbx translation units: error reporting,
+// argument validation and the launch functions that the C ABI wraps.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <string>
+
+#include "../../include/mbx.h"
+#include "mbx_device.hpp"
+
+namespace mbx {
+
+// Thread-local message behind mbx_last_error().
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define MBX_HIP_CHECK(expr)                                                                         \
+    do {                                                                                            \
+        hipError_t _e = (expr);                                                                     \
+        if (_e != hipSuccess)                                                                       \
+            return ::mbx::fail(MBX_ELAUNCH, std::string(#expr " failed: ") + hipGetErrorString(_e)); \
+    } while (0)
+
+#define MBX_LAUNCH_CHECK()                                                                              \
+    do {                                                                                                \
+        hipError_t _e = hipGetLastError();                                                              \
+        if (_e != hipSuccess)                                                                           \
+            return ::mbx::fail(MBX_ELAUNCH, std::string("kernel launch failed: ") + hipGetErrorString(_e)); \
+    } while (0)
+
+// Validates g and fills the float32 device copy.  Returns MBX_OK or an error.
+int make_dev_geom(const mbx_geom *g, DevGeom *out);
+
+inline int dtype_size(int dt) { return dt == MBX_DTYPE_BF16 ? 2 : 4; }
+inline bool dtype_ok(int dt) { return dt == MBX_DTYPE_F32 || dt == MBX_DTYPE_BF16; }
+
+// Back projection planning: number of view splits (partials) and slice chunking.
+struct BackPlan {
+    int nsplit;  // view groups reduced at the end (1 = write directly)
+    int kpl;     // slices per lane (chunk = 64*kpl slices)
+    int nchunk;  // slice chunks
+};
+BackPlan plan_back(const DevGeom &g, int npix, int nviews);
+int64_t back_workspace_bytes(const DevGeom &g, int npix, int nviews);
+
+struct FwdPlan {
+    int cb;      // channels per block
+    int ncb;     // channel blocks
+    int rows_pb; // detector rows per block
+    int nrb;     // row blocks
+    int list;    // candidate list capacity
+    size_t lds;  // dynamic LDS bytes
+    int64_t nblocks;
+};
+FwdPlan plan_fwd(const DevGeom &g, int nviews);
+int64_t fwd_workspace_bytes(const DevGeom &g, int nviews);
+
+// Launchers used by the C ABI and by the VCD subset update.  All arguments are
+// already validated.
+int launch_back(const DevGeom &g, const float *angles, const void *sino, int sino_dtype, const float *weights,
+                const int32_t *pix, int npix, const int32_t *views, int nviews, int coeff_power, int kahan,
+                float scale, int accumulate, float *out, void *ws, hipStream_t st);
+int launch_fwd(const DevGeom &g, const float *angles, const float *vox, const int32_t *pix, int npix,
+               const int32_t *views, int nviews, void *sino, int sino_dtype, int layout, int accumulate,
+               const float *weights, double *sumsq, void *ws, hipStream_t st);
+// Sum `n` doubles in fixed order into out[0] (one block).
+int launch_sum_partials(const double *partials, int64_t n, double *out, int accumulate, hipStream_t st);
+
+}  // namespace mbx

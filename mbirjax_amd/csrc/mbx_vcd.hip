@@ -1,0 +1,508 @@
+// qGGMRF prior, FP64 reductions and the VCD subset update on gfx950.
+//
+// mbx_vcd_subset_update restates vcd_subset_updater
+// (mbirjax/tomography_model.py:1343-1542) as a fixed kernel chain on one
+// stream with no host synchronisation:
+//   back projection of W*e            -> forward gradient (before -1/sigma_y^2)
+//   qGGMRF (or prox) gradient/Hessian -> prior terms
+//   delta = -(fg + pg)/(fh + ph)      -> with FP64 partials of prior_linear,
+//                                        prior_quadratic and forward_linear
+//   forward projection of delta       -> delta sinogram, FP64 forward_quadratic
+//   alpha = clip((fl - pl)/(fq + pq + eps32), eps32, 1.5)   (on the device)
+//   [positivity clamp + re-projection]
+//   x[S] += alpha*delta ; e -= alpha*delta_sino
+// forward_linear uses the adjoint identity
+//   sum(W e . A delta) = <A^T (W e), delta>,
+// so it is a by-product of the back projection instead of an extra pass over
+// the sinogram (get_forward_lin_quad, mbirjax/tomography_model.py:1546-1631).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "mbx_internal.hpp"
+
+namespace mbx {
+
+constexpr float kEps32 = 1.1920928955078125e-07f;  // jnp.finfo(jnp.float32).eps
+constexpr int kRedBlocks = 1024;
+
+// ======================================================================
+// qGGMRF gradient and Hessian (mbirjax/qggmrf.py:118-279)
+// ======================================================================
+
+struct QP {
+    float b[6];
+    float tsig;      // T*sigma_x
+    float scale;     // T^(p-2) / sigma_x^2
+    float qm2;       // q - 2
+    float qmp;       // q - p
+    float q_over_p;  // q / p
+};
+
+// get_2_b_tilde (mbirjax/qggmrf.py:250-279): b * rho'(delta)/delta.
+__device__ __forceinline__ float two_b_tilde(float delta, float bval, const QP &q) {
+    const float sd = fabsf(delta) / q.tsig + kEps32;
+    const float l2 = log2f(sd);
+    const float d_q2 = exp2f(q.qm2 * l2);  // sd^(q-2)
+    const float d_qp = exp2f(q.qmp * l2);  // sd^(q-p)
+    const float num = d_q2 * (q.q_over_p + d_qp);
+    const float den1 = 1.0f + d_qp;
+    const float den = den1 * den1;
+    return bval * (q.scale * num / den);
+}
+
+__global__ __launch_bounds__(256) void qggmrf_kernel(const float *__restrict__ x, int rows, int cols, int Nz,
+                                                     const int32_t *__restrict__ pix, int npix, QP q,
+                                                     float *__restrict__ grad, float *__restrict__ hess) {
+    const int s = blockIdx.x;
+    if (s >= npix) return;
+    const int p = pix[s];
+    const int i = p / cols, j = p - i * cols;
+    // clipped in-plane neighbours: ravel_multi_index(..., mode='clip') (qggmrf.py:235-236)
+    const int nb[4] = {min(i + 1, rows - 1) * cols + j, max(i - 1, 0) * cols + j, i * cols + min(j + 1, cols - 1),
+                       i * cols + max(j - 1, 0)};
+    const float *xc = x + (size_t)p * Nz;
+    for (int k = threadIdx.x; k < Nz; k += blockDim.x) {
+        const float v = xc[k];
+        // cylinder neighbours with reflected (zero-difference) ends (qggmrf.py:151-188)
+        const float dplus = (k + 1 < Nz) ? xc[k + 1] - v : 0.0f;
+        const float dminus = (k > 0) ? v - xc[k - 1] : 0.0f;
+        const float tp = two_b_tilde(dplus, 1.0f, q);
+        const float tm = two_b_tilde(dminus, 1.0f, q);
+        float g = -q.b[4] * (tp * dplus);
+        float h = q.b[4] * tp;
+        g += q.b[5] * (tm * dminus);
+        h += q.b[5] * tm;
+        // in-plane neighbours [row+1, row-1, col+1, col-1] (qggmrf.py:191-247)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const float d = v - x[(size_t)nb[t] * Nz + k];
+            const float bt = two_b_tilde(d, q.b[t], q);
+            g += bt * d;
+            h += bt;
+        }
+        grad[(size_t)s * Nz + k] = g;
+        hess[(size_t)s * Nz + k] = h;
+    }
+}
+
+static QP make_qp(const mbx_qggmrf *P) {
+    QP q;
+    for (int t = 0; t < 6; ++t) q.b[t] = (float)P->b[t];
+    q.tsig = (float)(P->T * P->sigma_x);
+    q.scale = (float)(std::pow(P->T, P->p - 2.0) / (P->sigma_x * P->sigma_x));
+    q.qm2 = (float)(P->q - 2.0);
+    q.qmp = (float)(P->q - P->p);
+    q.q_over_p = (float)(P->q / P->p);
+    return q;
+}
+
+static int check_qparams(const mbx_qggmrf *P) {
+    if (!P) return fail(MBX_EINVAL, "qggmrf params pointer is NULL");
+    if (!(P->sigma_x > 0) || !(P->T > 0) || !(P->p > 0) || !(P->q > 0))
+        return fail(MBX_EINVAL, "qggmrf sigma_x, T, p and q must be positive");
+    return MBX_OK;
+}
+
+// prox_gradient_at_indices (mbirjax/qggmrf.py:8-28)
+__global__ __launch_bounds__(256) void prox_grad_kernel(const float *__restrict__ x, const float *__restrict__ prox,
+                                                        const int32_t *__restrict__ pix, int npix, int Nz,
+                                                        float inv_sp2, float *__restrict__ grad) {
+    const int64_t n = (int64_t)npix * Nz;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int s = (int)(t / Nz), k = (int)(t - (int64_t)s * Nz);
+        const size_t o = (size_t)pix[s] * Nz + k;
+        grad[t] = inv_sp2 * (x[o] - prox[o]);
+    }
+}
+
+// ======================================================================
+// Reductions
+// ======================================================================
+
+template <typename T>
+__device__ __forceinline__ T block_reduce(T v, T *scratch) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    T r = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += scratch[i];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(256) void sum_partials_kernel(const double *__restrict__ part, int64_t n,
+                                                           double *__restrict__ out, int accumulate) {
+    __shared__ double scratch[8];
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+    const double r = block_reduce<double>(s, scratch);
+    if (threadIdx.x == 0) out[0] = accumulate ? out[0] + r : r;
+}
+
+int launch_sum_partials(const double *partials, int64_t n, double *out, int accumulate, hipStream_t st) {
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, partials, n, out, accumulate);
+    MBX_LAUNCH_CHECK();
+    return MBX_OK;
+}
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void dot_kernel(const TA *__restrict__ a, const TB *__restrict__ b,
+                                                  const float *__restrict__ w, int64_t n, double *__restrict__ part) {
+    __shared__ double scratch[8];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double av = (double)to_float(a[i]);
+        const double bv = b ? (double)to_float(b[i]) : av;
+        s += w ? av * bv * (double)w[i] : av * bv;
+    }
+    const double r = block_reduce<double>(s, scratch);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// e = e - float(alpha) * d
+template <typename TE, typename TD>
+__global__ __launch_bounds__(256) void axpy_kernel(TE *__restrict__ e, const TD *__restrict__ d,
+                                                   const double *__restrict__ alpha, int64_t n) {
+    const float a = (float)alpha[0];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float ad = a * to_float(d[i]);
+        e[i] = from_float<TE>(to_float(e[i]) - ad);
+    }
+}
+
+static int launch_axpy(void *e, int edt, const void *d, int ddt, const double *alpha, int64_t n, hipStream_t st) {
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 16384);
+    if (n == 0) return MBX_OK;
+    if (edt == MBX_DTYPE_F32 && ddt == MBX_DTYPE_F32)
+        hipLaunchKernelGGL((axpy_kernel<float, float>), dim3((unsigned)blocks), dim3(256), 0, st, (float *)e,
+                           (const float *)d, alpha, n);
+    else if (edt == MBX_DTYPE_F32)
+        hipLaunchKernelGGL((axpy_kernel<float, bf16>), dim3((unsigned)blocks), dim3(256), 0, st, (float *)e,
+                           (const bf16 *)d, alpha, n);
+    else if (ddt == MBX_DTYPE_F32)
+        hipLaunchKernelGGL((axpy_kernel<bf16, float>), dim3((unsigned)blocks), dim3(256), 0, st, (bf16 *)e,
+                           (const float *)d, alpha, n);
+    else
+        hipLaunchKernelGGL((axpy_kernel<bf16, bf16>), dim3((unsigned)blocks), dim3(256), 0, st, (bf16 *)e,
+                           (const bf16 *)d, alpha, n);
+    MBX_LAUNCH_CHECK();
+    return MBX_OK;
+}
+
+// ======================================================================
+// VCD subset update pieces
+// ======================================================================
+
+// delta = -((fg + pg) / (fh + ph)) with fg = -fmc*bp, fh = fmc*H[p];
+// partials: [0] prior_linear = sum pg*delta, [1] sum ph*delta^2, [2] sum bp*delta
+__global__ __launch_bounds__(256) void delta_kernel(const float *__restrict__ bp, const float *__restrict__ pg,
+                                                    const float *__restrict__ ph, float ph_const,
+                                                    const float *__restrict__ fm_hess, const int32_t *__restrict__ pix,
+                                                    int npix, int Nz, float fmc, float *__restrict__ delta,
+                                                    double *__restrict__ part) {
+    __shared__ double scratch[8];
+    const int64_t n = (int64_t)npix * Nz;
+    double pl = 0.0, pq = 0.0, fl = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int s = (int)(t / Nz), k = (int)(t - (int64_t)s * Nz);
+        const float fg = -(fmc * bp[t]);
+        const float fh = fmc * fm_hess[(size_t)pix[s] * Nz + k];
+        const float h = ph ? ph[t] : ph_const;
+        const float g = pg[t];
+        const float dl = -((fg + g) / (fh + h));
+        delta[t] = dl;
+        pl += (double)g * dl;
+        pq += (double)h * dl * dl;
+        fl += (double)bp[t] * dl;
+    }
+    const double r0 = block_reduce<double>(pl, scratch);
+    const double r1 = block_reduce<double>(pq, scratch);
+    const double r2 = block_reduce<double>(fl, scratch);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = r0;
+        part[kRedBlocks + blockIdx.x] = r1;
+        part[2 * kRedBlocks + blockIdx.x] = r2;
+    }
+}
+
+// scal[0] = fq_raw (sum of delta_sino^2 * w), written by the forward projection.
+// alpha = clip((fl - pl) / (fq + pq + eps32), eps32, 1.5)   (tomography_model.py:1467-1471)
+__global__ __launch_bounds__(256) void alpha_kernel(const double *__restrict__ part, int nblk, double fmc,
+                                                    double *__restrict__ scal, double *__restrict__ stats) {
+    __shared__ double scratch[8];
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+        a += part[i];
+        b += part[kRedBlocks + i];
+        c += part[2 * kRedBlocks + i];
+    }
+    const double pl = block_reduce<double>(a, scratch);
+    const double pq = 0.5 * block_reduce<double>(b, scratch);
+    const double fl = fmc * block_reduce<double>(c, scratch);
+    if (threadIdx.x == 0) {
+        const double fq = fmc * scal[0];
+        const double eps = (double)kEps32;
+        double alpha = (fl - pl) / (fq + pq + eps);
+        alpha = alpha < eps ? eps : (alpha > 1.5 ? 1.5 : alpha);
+        if (alpha != alpha) alpha = eps;  // only if every term is NaN
+        scal[1] = alpha;
+        scal[2] = fl;
+        scal[3] = fq;
+        scal[4] = pl;
+        scal[5] = pq;
+        if (stats) {
+            stats[1] += alpha;
+            stats[2] += 1.0;
+        }
+    }
+}
+
+// delta = max(-x[S]/(alpha + eps32), delta)   (tomography_model.py:1497-1504)
+__global__ __launch_bounds__(256) void positivity_kernel(const float *__restrict__ x, const int32_t *__restrict__ pix,
+                                                         int npix, int Nz, const double *__restrict__ scal,
+                                                         float *__restrict__ delta) {
+    const float alpha = (float)scal[1];
+    const float pc = 1.0f / (alpha + kEps32);
+    const int64_t n = (int64_t)npix * Nz;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int s = (int)(t / Nz), k = (int)(t - (int64_t)s * Nz);
+        delta[t] = fmaxf(-pc * x[(size_t)pix[s] * Nz + k], delta[t]);
+    }
+}
+
+// x[S] += alpha*delta ; per-block sum |alpha*delta|   (tomography_model.py:1512-1536, 1820-1823)
+__global__ __launch_bounds__(256) void update_recon_kernel(float *__restrict__ x, const int32_t *__restrict__ pix,
+                                                           int npix, int Nz, const double *__restrict__ scal,
+                                                           const float *__restrict__ delta, double *__restrict__ part) {
+    __shared__ double scratch[8];
+    const float alpha = (float)scal[1];
+    const int64_t n = (int64_t)npix * Nz;
+    double l1 = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int s = (int)(t / Nz), k = (int)(t - (int64_t)s * Nz);
+        const float ad = alpha * delta[t];
+        // a subset can list a pixel twice when the padding outnumbers a subset
+        // (vcd_utils.py:212-217); .at[].add then adds both copies
+        atomicAdd(&x[(size_t)pix[s] * Nz + k], ad);
+        l1 += fabs((double)ad);
+    }
+    const double r = block_reduce<double>(l1, scratch);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// e -= float(alpha) * d, reading alpha from scal[1]
+template <typename TE, typename TD>
+__global__ __launch_bounds__(256) void axpy_scal_kernel(TE *__restrict__ e, const TD *__restrict__ d,
+                                                        const double *__restrict__ scal, int64_t n) {
+    const float a = (float)scal[1];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float ad = a * to_float(d[i]);
+        e[i] = from_float<TE>(to_float(e[i]) - ad);
+    }
+}
+
+struct VcdLayout {
+    size_t back, bp, pg, ph, delta, dsino, fwd, dpart, l1part, scal, total;
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static VcdLayout vcd_layout(const DevGeom &g, int npix, int ddt) {
+    VcdLayout L;
+    size_t off = 0;
+    const size_t cyl = (size_t)npix * g.Nz * sizeof(float);
+    L.back = off;
+    off += align256((size_t)back_workspace_bytes(g, npix, g.V));
+    L.bp = off;
+    off += align256(cyl);
+    L.pg = off;
+    off += align256(cyl);
+    L.ph = off;
+    off += align256(cyl);
+    L.delta = off;
+    off += align256(cyl);
+    L.dsino = off;
+    off += align256((size_t)g.V * g.C * g.R * dtype_size(ddt));
+    L.fwd = off;
+    off += align256((size_t)fwd_workspace_bytes(g, g.V));
+    L.dpart = off;
+    off += align256(3 * kRedBlocks * sizeof(double));
+    L.l1part = off;
+    off += align256(kRedBlocks * sizeof(double));
+    L.scal = off;
+    off += align256(16 * sizeof(double));
+    L.total = off;
+    return L;
+}
+
+static int red_blocks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, kRedBlocks)); }
+
+}  // namespace mbx
+
+using namespace mbx;
+
+extern "C" {
+
+int mbx_qggmrf_grad_hess(const float *flat_recon, int32_t num_rows, int32_t num_cols, int32_t num_slices,
+                         const int32_t *pix, int32_t npix, const mbx_qggmrf *params, float *grad, float *hess,
+                         void *stream) {
+    if (num_rows <= 0 || num_cols <= 0 || num_slices <= 0 || npix < 0)
+        return fail(MBX_EINVAL, "bad recon shape or npix");
+    int rc = check_qparams(params);
+    if (rc != MBX_OK) return rc;
+    if (npix == 0) return MBX_OK;
+    if (!flat_recon || !pix || !grad || !hess) return fail(MBX_EINVAL, "NULL device pointer");
+    const QP q = make_qp(params);
+    const int threads = num_slices >= 256 ? 256 : (num_slices > 64 ? 128 : 64);
+    hipLaunchKernelGGL(qggmrf_kernel, dim3(npix), dim3(threads), 0, (hipStream_t)stream, flat_recon, num_rows,
+                       num_cols, num_slices, pix, npix, q, grad, hess);
+    MBX_LAUNCH_CHECK();
+    return MBX_OK;
+}
+
+int64_t mbx_dot_workspace_bytes(void) { return (int64_t)kRedBlocks * sizeof(double); }
+
+int mbx_dot(const void *a, int32_t a_dtype, const void *b, int32_t b_dtype, const float *w, int64_t n, double *out,
+            void *workspace, int64_t workspace_bytes, void *stream) {
+    if (n < 0) return fail(MBX_EINVAL, "n must be >= 0");
+    if (!dtype_ok(a_dtype) || (b && !dtype_ok(b_dtype))) return fail(MBX_EINVAL, "unknown dtype");
+    if (!out || (n > 0 && !a)) return fail(MBX_EINVAL, "NULL device pointer");
+    if (!workspace || workspace_bytes < mbx_dot_workspace_bytes())
+        return fail(MBX_EWORKSPACE, "mbx_dot workspace too small");
+    hipStream_t st = (hipStream_t)stream;
+    const int nb = red_blocks(n);
+    double *part = (double *)workspace;
+    if (!b) b_dtype = a_dtype;
+    if (a_dtype == MBX_DTYPE_F32 && b_dtype == MBX_DTYPE_F32)
+        hipLaunchKernelGGL((dot_kernel<float, float>), dim3(nb), dim3(256), 0, st, (const float *)a,
+                           (const float *)b, w, n, part);
+    else if (a_dtype == MBX_DTYPE_F32)
+        hipLaunchKernelGGL((dot_kernel<float, bf16>), dim3(nb), dim3(256), 0, st, (const float *)a, (const bf16 *)b,
+                           w, n, part);
+    else if (b_dtype == MBX_DTYPE_F32)
+        hipLaunchKernelGGL((dot_kernel<bf16, float>), dim3(nb), dim3(256), 0, st, (const bf16 *)a, (const float *)b,
+                           w, n, part);
+    else
+        hipLaunchKernelGGL((dot_kernel<bf16, bf16>), dim3(nb), dim3(256), 0, st, (const bf16 *)a, (const bf16 *)b, w,
+                           n, part);
+    MBX_LAUNCH_CHECK();
+    return launch_sum_partials(part, nb, out, 0, st);
+}
+
+int mbx_sino_axpy(void *e, int32_t e_dtype, const void *d, int32_t d_dtype, const double *alpha, int64_t n,
+                  void *stream) {
+    if (n < 0) return fail(MBX_EINVAL, "n must be >= 0");
+    if (!dtype_ok(e_dtype) || !dtype_ok(d_dtype)) return fail(MBX_EINVAL, "unknown dtype");
+    if (n > 0 && (!e || !d || !alpha)) return fail(MBX_EINVAL, "NULL device pointer");
+    return launch_axpy(e, e_dtype, d, d_dtype, alpha, n, (hipStream_t)stream);
+}
+
+int64_t mbx_vcd_workspace_bytes(const mbx_geom *g, int32_t npix, int32_t delta_dtype) {
+    DevGeom d;
+    if (make_dev_geom(g, &d) != MBX_OK || npix < 0 || !dtype_ok(delta_dtype)) return -1;
+    return (int64_t)vcd_layout(d, npix, delta_dtype).total;
+}
+
+int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_recon, void *e, int32_t e_dtype,
+                          const float *weights, const float *fm_hessian, const int32_t *pix, int32_t npix,
+                          double fm_constant, const mbx_qggmrf *qparams, const float *prox_input, double sigma_prox,
+                          int32_t positivity, int32_t delta_dtype, int32_t kahan, double *stats, void *workspace,
+                          int64_t workspace_bytes, void *stream) {
+    DevGeom d;
+    int rc = make_dev_geom(g, &d);
+    if (rc != MBX_OK) return rc;
+    if (npix <= 0) return fail(MBX_EINVAL, "npix must be positive");
+    if (!dtype_ok(e_dtype) || !dtype_ok(delta_dtype)) return fail(MBX_EINVAL, "unknown dtype");
+    if (!angles || !flat_recon || !e || !fm_hessian || !pix) return fail(MBX_EINVAL, "NULL device pointer");
+    if (!prox_input) {
+        rc = check_qparams(qparams);
+        if (rc != MBX_OK) return rc;
+    } else if (!(sigma_prox > 0)) {
+        return fail(MBX_EINVAL, "sigma_prox must be positive");
+    }
+    if (!(fm_constant > 0)) return fail(MBX_EINVAL, "fm_constant must be positive");
+    const VcdLayout L = vcd_layout(d, npix, delta_dtype);
+    if (!workspace || workspace_bytes < (int64_t)L.total)
+        return fail(MBX_EWORKSPACE, "VCD workspace too small: need " + std::to_string(L.total) + " bytes");
+    hipStream_t st = (hipStream_t)stream;
+    char *ws = (char *)workspace;
+    float *bp = (float *)(ws + L.bp);
+    float *pg = (float *)(ws + L.pg);
+    float *ph = (float *)(ws + L.ph);
+    float *delta = (float *)(ws + L.delta);
+    void *dsino = (void *)(ws + L.dsino);
+    double *dpart = (double *)(ws + L.dpart);
+    double *l1part = (double *)(ws + L.l1part);
+    double *scal = (double *)(ws + L.scal);
+    const int64_t ncyl = (int64_t)npix * d.Nz;
+    const int64_t nsino = (int64_t)d.V * d.C * d.R;
+    const int nb = red_blocks(ncyl);
+
+    // (c) back projection of the weighted error sinogram
+    rc = launch_back(d, angles, e, e_dtype, weights, pix, npix, nullptr, d.V, 1, kahan, 1.0f, 0, bp, ws + L.back, st);
+    if (rc != MBX_OK) return rc;
+    // (a) prior gradient and Hessian
+    float ph_const = 0.0f;
+    const float *php = ph;
+    if (prox_input) {
+        hipLaunchKernelGGL(prox_grad_kernel, dim3(nb), dim3(256), 0, st, flat_recon, prox_input, pix, npix, d.Nz,
+                           (float)(1.0 / (sigma_prox * sigma_prox)), pg);
+        MBX_LAUNCH_CHECK();
+        ph_const = (float)(sigma_prox * sigma_prox);
+        php = nullptr;
+    } else {
+        rc = mbx_qggmrf_grad_hess(flat_recon, d.rows, d.cols, d.Nz, pix, npix, qparams, pg, ph, stream);
+        if (rc != MBX_OK) return rc;
+    }
+    // (d,e,f) delta and FP64 partials
+    hipLaunchKernelGGL(delta_kernel, dim3(nb), dim3(256), 0, st, bp, pg, php, ph_const, fm_hessian, pix, npix, d.Nz,
+                       (float)fm_constant, delta, dpart);
+    MBX_LAUNCH_CHECK();
+    // (g,h) delta sinogram and forward_quadratic
+    rc = launch_fwd(d, angles, delta, pix, npix, nullptr, d.V, dsino, delta_dtype, MBX_LAYOUT_VCR, 0, weights,
+                    scal + 0, ws + L.fwd, st);
+    if (rc != MBX_OK) return rc;
+    // (i) step size on the device
+    hipLaunchKernelGGL(alpha_kernel, dim3(1), dim3(256), 0, st, dpart, nb, fm_constant, scal, stats);
+    MBX_LAUNCH_CHECK();
+    // (j) positivity
+    if (positivity) {
+        hipLaunchKernelGGL(positivity_kernel, dim3(nb), dim3(256), 0, st, flat_recon, pix, npix, d.Nz, scal, delta);
+        MBX_LAUNCH_CHECK();
+        rc = launch_fwd(d, angles, delta, pix, npix, nullptr, d.V, dsino, delta_dtype, MBX_LAYOUT_VCR, 0, nullptr,
+                        nullptr, nullptr, st);
+        if (rc != MBX_OK) return rc;
+    }
+    // (k,m) recon update and ell1
+    hipLaunchKernelGGL(update_recon_kernel, dim3(nb), dim3(256), 0, st, flat_recon, pix, npix, d.Nz, scal, delta,
+                       l1part);
+    MBX_LAUNCH_CHECK();
+    if (stats) {
+        rc = launch_sum_partials(l1part, nb, stats + 0, 1, st);
+        if (rc != MBX_OK) return rc;
+    }
+    // (l) error sinogram update
+    const int64_t blocks = std::min<int64_t>((nsino + 255) / 256, 16384);
+    if (e_dtype == MBX_DTYPE_F32 && delta_dtype == MBX_DTYPE_F32)
+        hipLaunchKernelGGL((axpy_scal_kernel<float, float>), dim3((unsigned)blocks), dim3(256), 0, st, (float *)e,
+                           (const float *)dsino, scal, nsino);
+    else if (e_dtype == MBX_DTYPE_F32)
+        hipLaunchKernelGGL((axpy_scal_kernel<float, bf16>), dim3((unsigned)blocks), dim3(256), 0, st, (float *)e,
+                           (const bf16 *)dsino, scal, nsino);
+    else if (delta_dtype == MBX_DTYPE_F32)
+        hipLaunchKernelGGL((axpy_scal_kernel<bf16, float>), dim3((unsigned)blocks), dim3(256), 0, st, (bf16 *)e,
+                           (const float *)dsino, scal, nsino);
+    else
+        hipLaunchKernelGGL((axpy_scal_kernel<bf16, bf16>), dim3((unsigned)blocks), dim3(256), 0, st, (bf16 *)e,
+                           (const bf16 *)dsino, scal, nsino);
+    MBX_LAUNCH_CHECK();
+    return MBX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,599 @@
+"""TomographyModel: the mbirjax model API on MI355X kernels.
+
+Public methods keep the signatures and semantics of mbirjax/tomography_model.py
+(forward_project 531, back_project 553, sparse_forward_project 576,
+sparse_back_project 624, compute_hessian_diagonal 679, set_params 710, recon 966,
+vcd_recon 1086, prox_map 1660, gen_weights 1727, get_forward_model_loss 1633).
+Arrays come back as torch tensors on the GPU; inputs may be NumPy arrays or
+torch tensors on any device.
+
+Compute goes through libmbx (include/mbx.h).  Internally the error sinogram
+lives in the channel-major (views, channels, rows) layout that the projectors
+stream; user-facing sinograms are (views, rows, channels) and are converted by
+mbx_sino_transpose at the boundary.
+"""
+import datetime
+import inspect
+import math
+import os
+import warnings
+from collections import namedtuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import qggmrf as _qggmrf
+from . import vcd_utils
+from .parameter_handler import ParameterHandler
+
+recon_param_names = ['num_iterations', 'granularity', 'partition_sequence', 'fm_rmse', 'prior_loss',
+                     'regularization_params', 'stop_threshold_change_pct', 'alpha_values']
+ReconParams = namedtuple('ReconParams', recon_param_names)
+
+_F32 = _lib.DTYPE_F32
+_BF16 = _lib.DTYPE_BF16
+
+
+def _torch_dtype(dt):
+    return torch.float32 if dt == _F32 else torch.bfloat16
+
+
+def _as_host_f32(a):
+    if isinstance(a, torch.Tensor):
+        return a.detach().to('cpu', torch.float32).numpy()
+    return np.asarray(a, dtype=np.float32)
+
+
+class TomographyModel(ParameterHandler):
+    """Base model; subclasses provide the geometry (ConeBeamModel, ParallelBeamModel)."""
+
+    DIRECT_RECON_VIEW_BATCH_SIZE = 100
+    GEOM_KIND = None
+
+    def __init__(self, sinogram_shape, **kwargs):
+        super().__init__()
+        self.set_params(no_compile=True, no_warning=True, sinogram_shape=tuple(int(v) for v in sinogram_shape),
+                        **kwargs)
+        if self.get_params('delta_voxel') is None:
+            delta_voxel = self.get_params('delta_det_channel') / self.get_magnification()
+            self.set_params(no_compile=True, no_warning=True, delta_voxel=delta_voxel)
+        self.use_ror_mask = True
+        self.auto_set_recon_size(sinogram_shape, no_compile=True, no_warning=True)
+        self.set_params(geometry_type=str(type(self)))
+        self.verify_valid_params()
+        self.view_batch_size_for_vmap = 512
+        self.pixel_batch_size_for_vmap = 2048
+        self.transfer_pixel_batch_size = 100 * self.pixel_batch_size_for_vmap
+        self.use_gpu = 'full'
+        self.main_device = self.sinogram_device = self.worker = None
+        self._geom = None
+        self._angles_dev = None
+        self.create_projectors()
+
+    @classmethod
+    def get_required_param_names(cls):
+        names = [p.name for p in inspect.signature(cls.__init__).parameters.values()
+                 if p.kind not in (inspect.Parameter.VAR_POSITIONAL, inspect.Parameter.VAR_KEYWORD)]
+        return names[1:] if names and names[0] == 'self' else names
+
+    # ------------------------------------------------------------------ setup
+    def set_devices_and_batch_sizes(self):
+        """Everything runs on the current HIP device (no CPU placement tiers)."""
+        if torch.cuda.is_available():
+            dev = torch.device('cuda', torch.cuda.current_device())
+            self.main_device = self.sinogram_device = self.worker = dev
+
+    def create_projectors(self):
+        """Re-plan the projectors after a geometry change (mbirjax/tomography_model.py:475-484)."""
+        self.set_devices_and_batch_sizes()
+        gp = self.get_geometry_parameters()
+        sinogram_shape, recon_shape = self.get_params(['sinogram_shape', 'recon_shape'])
+        g = gp._asdict()
+        self._geom = _lib.np_geom_struct(
+            self.GEOM_KIND, sinogram_shape, recon_shape, g['psf_radius'], g.get('bp_psf_radius', 1),
+            g.get('delta_det_row', 1.0), g['delta_det_channel'], g.get('det_row_offset', 0.0),
+            g['det_channel_offset'], g.get('source_detector_dist', math.inf), g.get('magnification', 1.0),
+            g['delta_voxel'], g.get('recon_slice_offset', 0.0))
+        self._angles_dev = None
+
+    def set_params(self, no_warning=False, no_compile=False, **kwargs):
+        recompile = super().set_params(no_warning=no_warning, no_compile=no_compile, **kwargs)
+        if recompile:
+            self.create_projectors()
+
+    def verify_valid_params(self):
+        super().verify_valid_params()
+        if self.get_params('use_gpu') not in ['automatic', 'full', 'sinograms', 'projections', 'none']:
+            raise ValueError("use_gpu must be one of 'automatic', 'full', 'sinograms', 'projections', 'none'")
+
+    def get_geometry_parameters(self):
+        raise NotImplementedError
+
+    def auto_set_recon_size(self, sinogram_shape, no_compile=True, no_warning=False):
+        raise NotImplementedError('auto_set_recon_size must be implemented by each specific geometry model.')
+
+    # ------------------------------------------------------------------ device helpers
+    def _device(self):
+        _lib.require_gpu()
+        if self.main_device is None:
+            self.set_devices_and_batch_sizes()
+        return self.main_device
+
+    def _angles(self):
+        dev = self._device()
+        if self._angles_dev is None:
+            name = self.get_params('view_params_name')
+            ang = self.get_params(name)
+            self._angles_dev = torch.as_tensor(_as_host_f32(ang).reshape(-1), device=dev).contiguous()
+        return self._angles_dev
+
+    def _tensor(self, a, dtype=torch.float32):
+        dev = self._device()
+        if isinstance(a, torch.Tensor):
+            return a.detach().to(device=dev, dtype=dtype).contiguous()
+        return torch.as_tensor(np.asarray(a), device=dev).to(dtype).contiguous()
+
+    def _indices(self, idx, bound=None, what='pixel_indices'):
+        """int32 device copy of an index array; with bound, every entry must lie in [0, bound)."""
+        dev = self._device()
+        if isinstance(idx, torch.Tensor):
+            t = idx.detach().to(device=dev, dtype=torch.int32).reshape(-1).contiguous()
+        else:
+            t = torch.as_tensor(_lib.as_i32(idx), device=dev)
+        if bound is None:
+            rows, cols = self.get_params('recon_shape')[:2]
+            bound = rows * cols
+        if t.numel() > 0:
+            lo, hi = int(t.min().item()), int(t.max().item())
+            if lo < 0 or hi >= bound:
+                raise ValueError('{} must lie in [0, {}); got range [{}, {}]'.format(what, bound, lo, hi))
+        return t
+
+    @staticmethod
+    def _output(t, output_device):
+        if output_device is None:
+            return t
+        return t.to(output_device)
+
+    # ------------------------------------------------------------------ layout conversion
+    def _transpose(self, src, src_layout, dst_layout, dst_dtype=_F32):
+        V = src.shape[0]
+        R, C = self.get_params('sinogram_shape')[1:]
+        shape = (V, C, R) if dst_layout == _lib.LAYOUT_VCR else (V, R, C)
+        dst = torch.empty(shape, dtype=_torch_dtype(dst_dtype), device=src.device)
+        src_dt = _F32 if src.dtype == torch.float32 else _BF16
+        _lib.check(_lib.lib().mbx_sino_transpose(_lib.ptr(src), src_dt, src_layout, _lib.ptr(dst), dst_dtype,
+                                                 dst_layout, V, R, C, _lib.stream()))
+        return dst
+
+    def to_channel_major(self, sinogram, dtype=_F32):
+        """(V, R, C) user sinogram -> (V, C, R) device tensor."""
+        s = self._tensor(sinogram)
+        if s.dim() != 3 or tuple(s.shape[1:]) != tuple(self.get_params('sinogram_shape')[1:]):
+            raise ValueError('sinogram must have shape (views, {}, {}), got {}'.format(
+                *self.get_params('sinogram_shape')[1:], tuple(s.shape)))
+        return self._transpose(s, _lib.LAYOUT_VRC, _lib.LAYOUT_VCR, dtype)
+
+    def from_channel_major(self, sino_vcr):
+        return self._transpose(sino_vcr, _lib.LAYOUT_VCR, _lib.LAYOUT_VRC, _F32)
+
+    # ------------------------------------------------------------------ projector cores
+    def _fwd(self, vox, pix, views=None, layout=_lib.LAYOUT_VRC, out_dtype=_F32, weights=None, want_sumsq=False,
+             out=None, accumulate=False):
+        """Forward project cylinders `vox` (npix, Nz) at pixels `pix` (device tensors)."""
+        V, R, C = self.get_params('sinogram_shape')
+        nv = V if views is None else int(views.numel())
+        if out is None:
+            shape = (nv, C, R) if layout == _lib.LAYOUT_VCR else (nv, R, C)
+            out = torch.empty(shape, dtype=_torch_dtype(out_dtype), device=vox.device)
+        sumsq = None
+        ws = None
+        wsb = 0
+        if want_sumsq:
+            sumsq = torch.zeros(1, dtype=torch.float64, device=vox.device)
+            wsb = int(_lib.lib().mbx_forward_workspace_bytes(self._geom, int(pix.numel()), nv))
+            ws = torch.empty(max(wsb, 8), dtype=torch.uint8, device=vox.device)
+        _lib.check(_lib.lib().mbx_forward_project(
+            self._geom, _lib.ptr(self._angles()), _lib.ptr(vox), _lib.ptr(pix), int(pix.numel()), _lib.ptr(views),
+            nv, _lib.ptr(out), out_dtype, layout, int(accumulate), _lib.ptr(weights), _lib.ptr(sumsq),
+            _lib.ptr(ws), wsb, _lib.stream()))
+        return (out, sumsq) if want_sumsq else out
+
+    def _back(self, sino_vcr, pix, views=None, coeff_power=1, weights=None, kahan=False, scale=1.0, out=None,
+              accumulate=False):
+        """Back project a channel-major sinogram (f32 or bf16) to cylinders (npix, Nz)."""
+        nz = self.get_params('recon_shape')[2]
+        npix = int(pix.numel())
+        nv = sino_vcr.shape[0] if views is None else int(views.numel())
+        if out is None:
+            out = torch.empty((npix, nz), dtype=torch.float32, device=sino_vcr.device)
+        wsb = int(_lib.lib().mbx_back_workspace_bytes(self._geom, npix, nv))
+        ws = torch.empty(wsb, dtype=torch.uint8, device=sino_vcr.device) if wsb > 0 else None
+        dt = _F32 if sino_vcr.dtype == torch.float32 else _BF16
+        _lib.check(_lib.lib().mbx_back_project(
+            self._geom, _lib.ptr(self._angles()), _lib.ptr(sino_vcr), dt, int(sino_vcr.shape[0]), _lib.ptr(weights),
+            _lib.ptr(pix), npix, _lib.ptr(views), nv, int(coeff_power), int(kahan), float(scale), int(accumulate),
+            _lib.ptr(out), _lib.ptr(ws), wsb, _lib.stream()))
+        return out
+
+    def _dot(self, a, b=None, w=None):
+        """FP64 sum(a*b*w) on the device (deterministic); returns a Python float."""
+        out = torch.zeros(1, dtype=torch.float64, device=a.device)
+        ws = torch.empty(int(_lib.lib().mbx_dot_workspace_bytes()), dtype=torch.uint8, device=a.device)
+        adt = _F32 if a.dtype == torch.float32 else _BF16
+        bdt = adt if b is None else (_F32 if b.dtype == torch.float32 else _BF16)
+        _lib.check(_lib.lib().mbx_dot(_lib.ptr(a), adt, _lib.ptr(b), bdt, _lib.ptr(w), int(a.numel()),
+                                      _lib.ptr(out), _lib.ptr(ws), int(ws.numel()), _lib.stream()))
+        return float(out.item())
+
+    # ------------------------------------------------------------------ public projections
+    def sparse_forward_project(self, voxel_values, pixel_indices, view_indices=None, output_device=None):
+        """(V', R, C) projection of the cylinders voxel_values (npix, Nz) at pixel_indices."""
+        nz = self.get_params('recon_shape')[2]
+        vox = self._tensor(voxel_values)
+        pix = self._indices(pixel_indices)
+        if vox.dim() != 2 or vox.shape[0] != pix.numel() or vox.shape[1] != nz:
+            raise ValueError('voxel_values must have shape[0:2] = (num_indices, num_slices)')
+        V = self.get_params('sinogram_shape')[0]
+        views = None if view_indices is None else self._indices(view_indices, V, 'view_indices')
+        return self._output(self._fwd(vox, pix, views), output_device)
+
+    def sparse_back_project(self, sinogram, pixel_indices, view_indices=None, coeff_power=1, output_device=None):
+        """(npix, Nz) back projection; view_indices select rows of the passed sinogram
+        (and their angles), as in mbirjax/tomography_model.py:661-663."""
+        pix = self._indices(pixel_indices)
+        sino = self.to_channel_major(sinogram)
+        V = self.get_params('sinogram_shape')[0]
+        views = None if view_indices is None else self._indices(view_indices, min(V, sino.shape[0]),
+                                                                   'view_indices')
+        if views is None and sino.shape[0] != V:
+            raise ValueError('sinogram must hold all views when view_indices is None')
+        return self._output(self._back(sino, pix, views, coeff_power=coeff_power), output_device)
+
+    def get_voxels_at_indices(self, recon, indices):
+        nz = self.get_params('recon_shape')[2]
+        r = self._tensor(recon).reshape(-1, nz)
+        return r[self._indices(indices).long()]
+
+    def forward_project(self, recon):
+        recon_shape = self.get_params('recon_shape')
+        full = vcd_utils.gen_full_indices(recon_shape, use_ror_mask=self.use_ror_mask)
+        vox = self.get_voxels_at_indices(recon, full)
+        return self.sparse_forward_project(vox, full)
+
+    def back_project(self, sinogram):
+        recon_shape = self.get_params('recon_shape')
+        full = vcd_utils.gen_full_indices(recon_shape, use_ror_mask=self.use_ror_mask)
+        cyl = self.sparse_back_project(sinogram, full)
+        recon = torch.zeros((recon_shape[0] * recon_shape[1], recon_shape[2]), dtype=torch.float32,
+                            device=cyl.device)
+        recon[torch.as_tensor(full, device=cyl.device).long()] = cyl
+        return recon.reshape(recon_shape)
+
+    def compute_hessian_diagonal(self, weights=None, output_device=None):
+        """diag(A^T W A) at every rows*cols pixel (coeff_power=2 back projection)."""
+        sinogram_shape, recon_shape = self.get_params(['sinogram_shape', 'recon_shape'])
+        dev = self._device()
+        if weights is None:
+            w = torch.ones((sinogram_shape[0], sinogram_shape[2], sinogram_shape[1]), dtype=torch.float32, device=dev)
+        else:
+            if tuple(weights.shape) != tuple(sinogram_shape):
+                raise ValueError('Weights must be constant or an array compatible with sinogram'
+                                 '\nGot weights.shape = {}, but sinogram.shape = {}'.format(weights.shape,
+                                                                                           sinogram_shape))
+            w = self.to_channel_major(weights)
+        h = self._hessian_vcr(w)
+        return self._output(h.reshape(recon_shape), output_device)
+
+    def _hessian_vcr(self, w_vcr):
+        rows, cols, nz = self.get_params('recon_shape')
+        pix = torch.arange(rows * cols, dtype=torch.int32, device=w_vcr.device)
+        return self._back(w_vcr, pix, coeff_power=2)
+
+    # ------------------------------------------------------------------ regularisation
+    def auto_set_regularization_params(self, sinogram, weights=None):
+        """sigma_y, sigma_x, sigma_prox from <= 20 sub-sampled views, with the
+        reference's float32 NumPy arithmetic (mbirjax/tomography_model.py:756-932)."""
+        if self.get_params('auto_regularize_flag'):
+            V = sinogram.shape[0]
+            step = V // np.minimum(20, V)
+            small = _as_host_f32(sinogram[::step])
+            small_w = 1 if weights is None else _as_host_f32(weights[::step])
+            indicator = self._get_sino_indicator(small)
+            self.auto_set_sigma_y(small, indicator, small_w)
+            recon_std = self._get_estimate_of_recon_std(small, indicator)
+            self.auto_set_sigma_x(recon_std)
+            self.auto_set_sigma_prox(recon_std)
+        names = ['sigma_y', 'sigma_x', 'sigma_prox']
+        return namedtuple('RegularizationParams', names)(*[float(v) for v in self.get_params(names)])._asdict()
+
+    def auto_set_sigma_y(self, sinogram, sino_indicator, weights=1):
+        snr_db = self.get_params('snr_db')
+        mag = self.get_magnification()
+        delta_voxel, delta_det_channel = self.get_params(['delta_voxel', 'delta_det_channel'])
+        signal_rms = float(np.average(weights * sinogram ** 2, None, sino_indicator) ** 0.5)
+        rel_noise_std = 10 ** (-snr_db / 20)
+        rel_pitch = delta_voxel / (delta_det_channel / mag)
+        sigma_y = np.float32(rel_noise_std * signal_rms * (rel_pitch ** 0.5))
+        self.set_params(no_warning=True, sigma_y=sigma_y, auto_regularize_flag=True)
+
+    def auto_set_sigma_x(self, recon_std):
+        sigma_x = np.float32(0.2 * (2 ** self.get_params('sharpness')) * recon_std)
+        self.set_params(no_warning=True, sigma_x=sigma_x, auto_regularize_flag=True)
+
+    def auto_set_sigma_prox(self, recon_std):
+        sigma_prox = np.float32(0.2 * (2 ** self.get_params('sharpness')) * recon_std)
+        self.set_params(no_warning=True, sigma_prox=sigma_prox, auto_regularize_flag=True)
+
+    @staticmethod
+    def _get_sino_indicator(sinogram):
+        percent_noise_floor = 5.0
+        return np.int8(sinogram > (0.01 * percent_noise_floor) * np.mean(np.fabs(sinogram)))
+
+    def _get_estimate_of_recon_std(self, sinogram, sino_indicator):
+        delta_det_channel, delta_voxel, recon_shape = self.get_params(['delta_det_channel', 'delta_voxel',
+                                                                       'recon_shape'])
+        mag = self.get_magnification()
+        typical = np.average(np.abs(sinogram), weights=sino_indicator)
+        path_space = (2 * recon_shape[0] * recon_shape[1]) / (recon_shape[0] + recon_shape[1]) * delta_voxel
+        path_sino = sinogram.shape[-1] * delta_det_channel / mag
+        return typical / np.minimum(path_space, path_sino)
+
+    # ------------------------------------------------------------------ direct recon
+    def direct_filter(self, sinogram, filter_name="ramp", view_batch_size=DIRECT_RECON_VIEW_BATCH_SIZE):
+        warnings.warn('direct_filter not implemented for TomographyModel.')
+        return torch.zeros_like(self._tensor(sinogram))
+
+    def direct_recon(self, sinogram, filter_name="ramp", view_batch_size=DIRECT_RECON_VIEW_BATCH_SIZE):
+        return self.back_project(self.direct_filter(sinogram, filter_name=filter_name,
+                                                    view_batch_size=view_batch_size))
+
+    @staticmethod
+    def _ramp_conv_rows(sino, recon_filter):
+        """'valid' convolution of every detector row (last axis, length C) with
+        the (2C-1)-tap filter: out[i] = sum_j row[j] h(i - j), via rFFT."""
+        C = sino.shape[-1]
+        L = 1 << int(math.ceil(math.log2(3 * C - 2)))
+        fr = torch.fft.rfft(sino, n=L, dim=-1)
+        fh = torch.fft.rfft(recon_filter, n=L)
+        full = torch.fft.irfft(fr * fh, n=L, dim=-1)
+        return full[..., C - 1:2 * C - 1].contiguous()
+
+    # ------------------------------------------------------------------ VCD
+    def recon(self, sinogram, weights=None, init_recon=None, max_iterations=15, stop_threshold_change_pct=0.2,
+              first_iteration=0, compute_prior_loss=False, num_iterations=None, logfile_path='./logs/recon.log',
+              print_logs=True, precision='fp32'):
+        """MBIR reconstruction with multi-granular VCD (mbirjax/tomography_model.py:966-1084).
+
+        precision: 'fp32' (plain) or 'mixed' (bfloat16 error/delta sinograms,
+        Kahan-compensated back projection, FP64 step-size and cost reductions).
+        """
+        if num_iterations is not None:
+            warnings.warn('num_iterations has been deprecated and will be removed in a future release.\n'
+                          'In the current run, the value of num_iterations will be used to set max_iterations.')
+            max_iterations = num_iterations
+        if first_iteration == 0 or self.logger is None:
+            self.setup_logger(logfile_path=logfile_path, print_logs=print_logs)
+        self.logger.info('GPU used for: full (MI355X kernels)')
+        if compute_prior_loss:
+            self.logger.warning('Computing the prior loss on every iteration uses significant memory and computing '
+                                'power.\nSet compute_prior_loss=False for most applications aside from debugging and '
+                                'demos.')
+        regularization_params = self.auto_set_regularization_params(sinogram, weights=weights)
+        recon_shape, granularity = self.get_params(['recon_shape', 'granularity'])
+        partitions = vcd_utils.gen_set_of_pixel_partitions(recon_shape, granularity, use_ror_mask=self.use_ror_mask)
+        partition_sequence = vcd_utils.gen_partition_sequence(self.get_params('partition_sequence'),
+                                                              max_iterations=max_iterations)
+        partition_sequence = partition_sequence[first_iteration:]
+        recon, loss_vectors = self.vcd_recon(sinogram, partitions, partition_sequence, stop_threshold_change_pct,
+                                             weights=weights, init_recon=init_recon,
+                                             compute_prior_loss=compute_prior_loss, first_iteration=first_iteration,
+                                             precision=precision)
+        partition_sequence = [int(v) for v in partition_sequence]
+        fm_rmse = [float(v) for v in loss_vectors[0]]
+        prior_loss = [float(v) for v in loss_vectors[1]] if compute_prior_loss else [0]
+        stop_pct = [100 * float(v) for v in loss_vectors[2]]
+        alpha_values = [float(v) for v in loss_vectors[3]]
+        recon_params = ReconParams(len(fm_rmse), granularity, partition_sequence, fm_rmse, prior_loss,
+                                   regularization_params, stop_pct, alpha_values)._asdict()
+        if logfile_path:
+            self.logger.info('Logs written to {}'.format(os.path.abspath(logfile_path)))
+        for h in list(self.logger.handlers):
+            h.flush()
+        notes = 'Reconstruction completed: {}\n\n'.format(datetime.datetime.now())
+        return recon, self.get_recon_dict(recon_params, notes=notes)
+
+    def get_recon_dict(self, recon_params=None, notes=None, save_log=True, save_model=True):
+        d = dict()
+        d['recon_params'] = "# Recon params not saved." if recon_params is None else recon_params
+        d['recon_log'] = "# Log info not saved." if (self.log_buffer is None or not save_log) else \
+            self.log_buffer.getvalue()
+        d['notes'] = '# No notes saved' if notes is None else notes
+        d['model_params'] = self.params.copy() if save_model else '# Model not saved'
+        return d
+
+    def _qggmrf_params(self):
+        nbr, sigma_x, p, q, T = self.get_params(['qggmrf_nbr_wts', 'sigma_x', 'p', 'q', 'T'])
+        return (_qggmrf.get_b_from_nbr_wts(nbr), float(sigma_x), float(p), float(q), float(T))
+
+    def vcd_recon(self, sinogram, partitions, partition_sequence, stop_threshold_change_pct, weights=None,
+                  init_recon=None, prox_input=None, compute_prior_loss=False, first_iteration=0, precision='fp32'):
+        """Multi-granular VCD on the device (mbirjax/tomography_model.py:1086-1255).
+        Returns (recon (rows, cols, Nz), (fm_rmse, prior_loss, nmae_update, alpha_values))."""
+        if precision not in ('fp32', 'mixed'):
+            raise ValueError("precision must be 'fp32' or 'mixed'")
+        dev = self._device()
+        recon_shape = self.get_params('recon_shape')
+        rows, cols, nz = recon_shape
+        y = self.to_channel_major(sinogram)
+        w = None if weights is None else self.to_channel_major(weights)
+        if init_recon is None:
+            self.logger.info('Starting direct recon for initial reconstruction')
+            init_recon = self.direct_recon(sinogram)
+        elif isinstance(init_recon, int) and init_recon == 0:
+            init_recon = torch.zeros(recon_shape, dtype=torch.float32, device=dev)
+        init_recon = self._tensor(init_recon)
+        if tuple(init_recon.shape) != tuple(recon_shape):
+            msg = "init_recon does not have the correct shape. \nExpected {}, but got shape {} for init_recon " \
+                  "shape.".format(recon_shape, tuple(init_recon.shape))
+            self.logger.error(msg)
+            raise ValueError(msg)
+
+        # error sinogram and the optimal scale of the initial recon
+        self.logger.info('Initializing error sinogram')
+        full = vcd_utils.gen_full_indices(recon_shape, use_ror_mask=self.use_ror_mask)
+        full_d = torch.as_tensor(full, device=dev)
+        Ax = self._fwd(init_recon.reshape(-1, nz)[full_d.long()].contiguous(), full_d, layout=_lib.LAYOUT_VCR)
+        nrm = self._dot(Ax, None, w)
+        alpha0 = self._dot(Ax, y, w) / nrm if nrm > 0 else 1.0
+        e_dtype = _BF16 if precision == 'mixed' else _F32
+        alpha_dev = torch.tensor([float(np.float32(alpha0))], dtype=torch.float64, device=dev)
+        e32 = y.clone()
+        _lib.check(_lib.lib().mbx_sino_axpy(_lib.ptr(e32), _F32, _lib.ptr(Ax), _F32, _lib.ptr(alpha_dev),
+                                            int(e32.numel()), _lib.stream()))
+        del Ax
+        if e_dtype == _BF16:
+            e = torch.empty(e32.shape, dtype=torch.bfloat16, device=dev)
+            V, C, R = e32.shape
+            _lib.check(_lib.lib().mbx_sino_transpose(_lib.ptr(e32), _F32, _lib.LAYOUT_VCR, _lib.ptr(e), _BF16,
+                                                     _lib.LAYOUT_VCR, V, R, C, _lib.stream()))
+            del e32
+        else:
+            e = e32
+        flat_recon = (np.float32(alpha0) * init_recon).reshape(-1, nz).contiguous()
+
+        if prox_input is not None:
+            prox_input = self._tensor(prox_input)
+            if tuple(prox_input.shape) != tuple(recon_shape):
+                msg = "prox_input does not have the correct size. \nExpected {}, but got shape {} for prox_input " \
+                      "shape.".format(recon_shape, tuple(prox_input.shape))
+                self.logger.error(msg)
+                raise ValueError(msg)
+            prox_input = prox_input.reshape(-1, nz).contiguous()
+
+        verbose, sigma_y = self.get_params(['verbose', 'sigma_y'])
+        self.logger.info('Computing Hessian diagonal')
+        w_h = w if w is not None else torch.ones_like(y)
+        fm_hessian = self._hessian_vcr(w_h)
+        del w_h, y
+
+        fm_constant = 1.0 / (float(sigma_y) ** 2.0)
+        qp = self._qggmrf_params()
+        qstruct = _lib.qggmrf_struct(*qp)
+        sigma_prox = float(self.get_params('sigma_prox'))
+        positivity = bool(self.get_params('positivity_flag'))
+        kahan = precision == 'mixed'
+        parts_dev = [torch.as_tensor(np.ascontiguousarray(p, dtype=np.int32), device=dev) for p in partitions]
+        nS_max = max(int(p.shape[1]) for p in partitions)
+        wsb = int(_lib.lib().mbx_vcd_workspace_bytes(self._geom, nS_max, e_dtype))
+        ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+        stats = torch.zeros(4, dtype=torch.float64, device=dev)
+
+        self.logger.info('Starting VCD iterations')
+        max_iters = int(np.size(partition_sequence))
+        fm_rmse = np.zeros(max_iters)
+        pm_loss = np.zeros(max_iters)
+        nmae_update = np.zeros(max_iters)
+        alpha_values = np.zeros(max_iters)
+        num_iters = 0
+        L = _lib.lib()
+        st = _lib.stream()
+        angles = _lib.ptr(self._angles())
+        n_e = e.numel()
+        avg_w = 1.0 if w is None else float(torch.mean(w, dtype=torch.float64).item())
+        for i in range(max_iters):
+            part = parts_dev[int(partition_sequence[i])]
+            order = np.random.permutation(part.shape[0])
+            stats.zero_()
+            for s in order:
+                subset = part[int(s)]
+                _lib.check(L.mbx_vcd_subset_update(
+                    self._geom, angles, _lib.ptr(flat_recon), _lib.ptr(e), e_dtype, _lib.ptr(w),
+                    _lib.ptr(fm_hessian), _lib.ptr(subset), int(subset.numel()), fm_constant, qstruct,
+                    _lib.ptr(prox_input), sigma_prox, int(positivity), e_dtype, int(kahan), _lib.ptr(stats),
+                    _lib.ptr(ws), wsb, st))
+            ell1, alpha_sum, _, _ = stats.cpu().tolist()
+            fm_rmse[i] = math.sqrt(fm_constant * self._dot(e, None, w) / (n_e * avg_w))
+            nmae_update[i] = ell1 / float(torch.sum(torch.abs(flat_recon), dtype=torch.float64).item())
+            es_rmse = math.sqrt(self._dot(e) / n_e)
+            alpha_values[i] = alpha_sum / part.shape[0]
+            if verbose >= 1:
+                msg = '\nAfter iteration {} of a max of {}: Pct change={:.4f}, Forward loss={:.4f}'.format(
+                    i + first_iteration, max_iters + first_iteration, 100 * nmae_update[i], fm_rmse[i])
+                if compute_prior_loss:
+                    pm_loss[i] = _qggmrf.qggmrf_loss(flat_recon.reshape(recon_shape), qp) / flat_recon.numel()
+                    total = (fm_rmse[i] * n_e + pm_loss[i] * flat_recon.numel()) / (0.5 * (n_e + flat_recon.numel()))
+                    msg += ', Prior loss={:.4f}, Weighted total loss={:.4f}'.format(pm_loss[i], total)
+                self.logger.info(msg)
+                self.logger.info(f'Relative step size (alpha)={alpha_values[i]:.2f}, Error sino RMSE={es_rmse:.4f}')
+                self.logger.info('Number subsets = {}'.format(part.shape[0]))
+            num_iters += 1
+            if nmae_update[i] < stop_threshold_change_pct / 100:
+                self.logger.warning('Change threshold stopping condition reached')
+                break
+        self.last_error_sinogram = e
+        return self.reshape_recon(flat_recon), (fm_rmse[:num_iters], pm_loss[:num_iters], nmae_update[:num_iters],
+                                               alpha_values[:num_iters])
+
+    @staticmethod
+    def get_forward_model_loss(error_sinogram, sigma_y, weights=None, normalize=True):
+        e = error_sinogram if isinstance(error_sinogram, torch.Tensor) else torch.as_tensor(np.asarray(error_sinogram))
+        e = e.to(torch.float64)
+        if weights is None:
+            w, avg = 1.0, 1.0
+        else:
+            w = weights if isinstance(weights, torch.Tensor) else torch.as_tensor(np.asarray(weights))
+            w = w.to(device=e.device, dtype=torch.float64)
+            avg = torch.mean(w)
+        if normalize:
+            return float(torch.sqrt((1.0 / sigma_y ** 2) * torch.mean((e * e) * (w / avg))))
+        return float((1.0 / (2 * sigma_y ** 2)) * torch.sum((e * e) * w))
+
+    def prox_map(self, prox_input, sinogram, weights=None, init_recon=None, stop_threshold_change_pct=0.2,
+                 max_iterations=3, first_iteration=0, logfile_path='./logs/recon.log', print_logs=True):
+        """Proximal map with prior (1/2 sigma_prox^2)||x - prox_input||^2 (mbirjax/tomography_model.py:1660-1697)."""
+        recon_shape, granularity = self.get_params(['recon_shape', 'granularity'])
+        partitions = vcd_utils.gen_set_of_pixel_partitions(recon_shape, granularity)
+        seq = vcd_utils.gen_partition_sequence(self.get_params('partition_sequence'),
+                                               max_iterations=max_iterations)[first_iteration:]
+        if first_iteration == 0 or self.logger is None:
+            self.setup_logger(logfile_path=logfile_path, print_logs=print_logs)
+        return self.vcd_recon(sinogram, partitions, seq, stop_threshold_change_pct, weights=weights,
+                              init_recon=init_recon, prox_input=prox_input, first_iteration=first_iteration)
+
+    def gen_weights(self, sinogram, weight_type):
+        """Noise-model weights (mbirjax/tomography_model.py:1727-1764)."""
+        s = self._tensor(sinogram)
+        if weight_type == 'unweighted':
+            return torch.ones_like(s)
+        if weight_type == 'transmission':
+            return torch.exp(-s)
+        if weight_type == 'transmission_root':
+            return torch.exp(-s / 2)
+        if weight_type == 'emission':
+            return 1.0 / (torch.abs(s) + 0.1)
+        raise Exception("gen_weights: undefined weight_type {}".format(weight_type))
+
+    def reshape_recon(self, recon):
+        return recon.reshape(self.get_params('recon_shape'))
+
+    def scale_recon_shape(self, row_scale=1.0, col_scale=1.0, slice_scale=1.0):
+        r, c, s = self.get_params('recon_shape')
+        nr, nc, ns = int(r * row_scale), int(c * col_scale), int(s * slice_scale)
+        self.set_params(recon_shape=(nr, nc, ns))
+        return nr - r, nc - c, ns - s
+
+    # ------------------------------------------------------------------ save / load
+    def to_file(self, filename):
+        return self.save_params(self.params, filename)
+
+    @classmethod
+    def from_file(cls, source):
+        required_names = cls.get_required_param_names()
+        required, params = ParameterHandler.load_param_dict(source, required_names, values_only=True)
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            model = cls(**required)
+            model.set_params(**params)
+        return model

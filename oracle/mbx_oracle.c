@@ -1,0 +1,207 @@
+/*
+ * ORACLE -- test infrastructure only.  CPU restatement of the mbirjax 0.6.7
+ * sparse projectors, used by tests/, __graft_entry__.smoke() and bench.py's
+ * CPU baseline as the checker.  The product (mbirjax_amd) never calls it.
+ *
+ * Compiled twice by oracle/build.sh: REAL=double (liboracle_f64.so, the parity
+ * oracle) and REAL=float (liboracle_f32.so, the CPU-baseline port).  OpenMP
+ * parallelises over views (forward, each view writes its own rows) and over
+ * pixels (back), so results do not depend on the thread count.
+ *
+ * Sinograms here are in the reference's own (views, rows, channels) layout.
+ * Every formula follows the reference line by line (paths relative to the
+ * reference root):
+ *   recon_ijk_to_xyz               mbirjax/cone_beam.py:646-666
+ *   geometry_xyz_to_uv_mag         mbirjax/cone_beam.py:668-681
+ *   compute_horizontal_data        mbirjax/cone_beam.py:593-644
+ *   forward horizontal fan         mbirjax/cone_beam.py:244-284
+ *   forward vertical fan           mbirjax/cone_beam.py:286-377
+ *   back horizontal fan            mbirjax/cone_beam.py:409-451
+ *   back vertical fan              mbirjax/cone_beam.py:480-591
+ *   parallel compute_proj_data     mbirjax/parallel_beam.py:223-284
+ *   parallel forward / back        mbirjax/parallel_beam.py:138-221
+ *   view_indices semantics         mbirjax/projectors.py:109-111, 155-157;
+ *                                  mbirjax/tomography_model.py:661-663
+ * Rounding: jnp.round is round-half-to-even, restated with rint() in the
+ * default FE_TONEAREST mode.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef REAL
+#define REAL double
+#endif
+
+typedef struct {
+    int32_t kind; /* 0 parallel, 1 cone */
+    int32_t num_views, num_det_rows, num_det_channels;
+    int32_t num_recon_rows, num_recon_cols, num_recon_slices;
+    int32_t psf_radius, bp_psf_radius, reserved;
+    double delta_det_row, delta_det_channel, det_row_offset, det_channel_offset;
+    double source_detector_dist, magnification, delta_voxel, recon_slice_offset;
+} ogeom;
+
+typedef REAL real;
+
+static real clipr(real x, real lo, real hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+/* Horizontal data for pixel p at angle th (cone_beam.py:593-644 / parallel_beam.py:223-263). */
+static void horizontal(const ogeom *g, int p, real th, real *n_p, int *n_c, real *W_c, real *cos_a, real *mag_out,
+                       real *y_out) {
+    const int rows = g->num_recon_rows, cols = g->num_recon_cols;
+    const int i = p / cols, j = p % cols;
+    const real dv = (real)g->delta_voxel;
+    const real yt = dv * ((real)i - (real)(rows - 1) / 2);
+    const real xt = dv * ((real)j - (real)(cols - 1) / 2);
+    const real c = cos(th), s = sin(th);
+    const real x = c * xt - s * yt;
+    const real y = s * xt + c * yt;
+    const real center = (real)(g->num_det_channels - 1) / 2;
+    if (g->kind == 1) {
+        const real sdd = (real)g->source_detector_dist;
+        const real mag = 1 / (1 / (real)g->magnification - y / sdd);
+        const real u = mag * x;
+        *n_p = (u + (real)g->det_channel_offset) / (real)g->delta_det_channel + center;
+        const real theta_p = atan2(u, sdd);
+        *cos_a = fmax(fabs(cos(th - theta_p)), fabs(sin(th - theta_p)));
+        *W_c = mag * (dv / (real)g->delta_det_channel) * (*cos_a / cos(theta_p));
+        *mag_out = mag;
+    } else {
+        *n_p = (x + (real)g->det_channel_offset) / (real)g->delta_det_channel + center;
+        *cos_a = fmax(fabs(c), fabs(s));
+        *W_c = (dv / (real)g->delta_det_channel) * *cos_a;
+        *mag_out = 1;
+    }
+    *y_out = y;
+    *n_c = (int)rint(*n_p);
+}
+
+static real chan_w(const ogeom *g, real n_p, int n, real W_c, real cos_a) {
+    if (n < 0 || n >= g->num_det_channels) return 0;
+    const real L = clipr((W_c + 1) / 2 - fabs(n_p - (real)n), 0, fmin((real)1, W_c));
+    return (real)g->delta_voxel * L / cos_a;
+}
+
+/* z of slice k (cone_beam.py:665) */
+static real slice_z(const ogeom *g, int k) {
+    return (real)g->delta_voxel * ((real)k - (real)(g->num_recon_slices - 1) / 2) + (real)g->recon_slice_offset;
+}
+
+/* detector row position of slice k of a pixel with magnification mag
+ * (detector_uv_to_mn, cone_beam.py:684-707) */
+static real slice_row(const ogeom *g, real mag, int k) {
+    const real v = mag * slice_z(g, k);
+    return (v + (real)g->det_row_offset) / (real)g->delta_det_row + (real)(g->num_det_rows - 1) / 2;
+}
+
+static real inv_cos_phi(const ogeom *g, real mag, int k) {
+    const real v = mag * slice_z(g, k);
+    return 1 / cos(atan2(v, (real)g->source_detector_dist));
+}
+
+/* sino (nviews, R, C) = A x over the view slots; angle index views[i] (or i). */
+void oracle_forward(const ogeom *g, const double *angles, const REAL *vox, const int32_t *pix, int npix,
+                    const int32_t *views, int nviews, REAL *sino) {
+    const int R = g->num_det_rows, C = g->num_det_channels, Nz = g->num_recon_slices;
+    memset(sino, 0, sizeof(REAL) * (size_t)nviews * R * C);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int vi = 0; vi < nviews; ++vi) {
+        const real th = (real)angles[views ? views[vi] : vi];
+        real *sv = sino + (size_t)vi * R * C;
+        real *col = (real *)malloc(sizeof(real) * (size_t)R);
+        for (int s = 0; s < npix; ++s) {
+            real n_p, W_c, cos_a, mag, y;
+            int n_c;
+            horizontal(g, pix[s], th, &n_p, &n_c, &W_c, &cos_a, &mag, &y);
+            const REAL *xs = vox + (size_t)s * Nz;
+            if (g->kind == 1) {
+                /* forward_vertical_fan_one_pixel_to_one_view: detector-driven over rows m,
+                 * slices k in round(k_m) +- bp_psf_radius (cone_beam.py:347-372) */
+                const real W = mag * (real)g->delta_voxel / (real)g->delta_det_row;
+                const real Lmax = fmin((real)1, W);
+                for (int m = 0; m < R; ++m) {
+                    const real v_m = ((real)m - (real)(R - 1) / 2) * (real)g->delta_det_row - (real)g->det_row_offset;
+                    const real z_m = v_m / mag;
+                    const real k_m = (z_m - (real)g->recon_slice_offset) / (real)g->delta_voxel + (real)(Nz - 1) / 2;
+                    const int kc = (int)rint(k_m);
+                    real acc = 0;
+                    for (int ko = -g->bp_psf_radius; ko <= g->bp_psf_radius; ++ko) {
+                        const int k = kc + ko;
+                        if (k < 0 || k >= Nz) continue;
+                        const real A = clipr((W + 1) / 2 - fabs(slice_row(g, mag, k) - (real)m), 0, Lmax);
+                        acc += A * (real)xs[k] * inv_cos_phi(g, mag, k);
+                    }
+                    col[m] = acc;
+                }
+            }
+            for (int no = -g->psf_radius; no <= g->psf_radius; ++no) {
+                const int n = n_c + no;
+                const real A = chan_w(g, n_p, n, W_c, cos_a);
+                if (A == 0) continue;
+                if (g->kind == 1) {
+                    for (int m = 0; m < R; ++m) sv[(size_t)m * C + n] += A * col[m];
+                } else {
+                    for (int k = 0; k < Nz; ++k) sv[(size_t)k * C + n] += A * (real)xs[k];
+                }
+            }
+        }
+        free(col);
+    }
+}
+
+/* out (npix, Nz) = sum over view slots of A_v^T sino[view]; the slot uses
+ * sinogram view and angle views[i] (or i).  coeff_power 1 or 2. */
+void oracle_back(const ogeom *g, const double *angles, const REAL *sino, const int32_t *pix, int npix,
+                 const int32_t *views, int nviews, int coeff_power, REAL *out) {
+    const int R = g->num_det_rows, C = g->num_det_channels, Nz = g->num_recon_slices;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int s = 0; s < npix; ++s) {
+        real *acc = (real *)calloc((size_t)Nz, sizeof(real));
+        real *col = (real *)malloc(sizeof(real) * (size_t)R);
+        for (int vi = 0; vi < nviews; ++vi) {
+            const int vd = views ? views[vi] : vi;
+            const real th = (real)angles[vd];
+            const REAL *sv = sino + (size_t)vd * R * C;
+            real n_p, W_c, cos_a, mag, y;
+            int n_c;
+            horizontal(g, pix[s], th, &n_p, &n_c, &W_c, &cos_a, &mag, &y);
+            /* back_horizontal_fan: column over rows */
+            for (int m = 0; m < R; ++m) col[m] = 0;
+            for (int no = -g->psf_radius; no <= g->psf_radius; ++no) {
+                const int n = n_c + no;
+                real A = chan_w(g, n_p, n, W_c, cos_a);
+                if (A == 0) continue;
+                if (coeff_power == 2) A = A * A;
+                for (int m = 0; m < R; ++m) col[m] += A * (real)sv[(size_t)m * C + n];
+            }
+            if (g->kind == 1) {
+                /* back_vertical_fan: voxel-driven, rows round(m_p) +- psf_radius */
+                const real W = mag * (real)g->delta_voxel / (real)g->delta_det_row;
+                const real Lmax = fmin((real)1, W);
+                for (int k = 0; k < Nz; ++k) {
+                    const real m_p = slice_row(g, mag, k);
+                    const int mc = (int)rint(m_p);
+                    const real icp = inv_cos_phi(g, mag, k);
+                    real t = 0;
+                    for (int mo = -g->psf_radius; mo <= g->psf_radius; ++mo) {
+                        const int m = mc + mo;
+                        if (m < 0 || m >= R) continue;
+                        real A = clipr((W + 1) / 2 - fabs(m_p - (real)m), 0, Lmax) * icp;
+                        if (coeff_power == 2) A = A * A;
+                        t += A * col[m];
+                    }
+                    acc[k] += t;
+                }
+            } else {
+                for (int k = 0; k < Nz; ++k) acc[k] += col[k];
+            }
+        }
+        for (int k = 0; k < Nz; ++k) out[(size_t)s * Nz + k] = (REAL)acc[k];
+        free(acc);
+        free(col);
+    }
+}
+
+int oracle_real_size(void) { return (int)sizeof(REAL); }

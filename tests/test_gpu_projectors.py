@@ -1,0 +1,190 @@
+"""GPU parity of the libmbx projectors against the FP64 oracle.
+
+Bar: relative L2 <= 1e-5 against the FP64 oracle on the same float32 inputs
+(the reference's own FP32 projections are pinned only through the oracle: JAX
+cannot run here).  Adjoint identity in float32 <= 1e-6 relative.
+"""
+import numpy as np
+import pytest
+import torch
+
+import mbirjax_amd as mj
+from oracle import projectors as op
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def rel(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    return float(np.linalg.norm(a.astype(np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def make_cone(V, R, C, sdd, sid, span=2 * np.pi, **params):
+    ang = np.linspace(0, span, V, endpoint=False)
+    m = mj.ConeBeamModel((V, R, C), ang, source_detector_dist=sdd, source_iso_dist=sid)
+    if params:
+        m.set_params(**params)
+    g = op.Geometry("cone", (V, R, C), ang, sdd, sid, recon_shape=m.get_params('recon_shape'),
+                    delta_det_row=m.get_params('delta_det_row'), delta_det_channel=m.get_params('delta_det_channel'),
+                    det_row_offset=m.get_params('det_row_offset'),
+                    det_channel_offset=m.get_params('det_channel_offset'), delta_voxel=m.get_params('delta_voxel'),
+                    recon_slice_offset=m.get_params('recon_slice_offset'))
+    return m, g
+
+
+def make_par(V, R, C, span=np.pi, **params):
+    ang = np.linspace(0, span, V, endpoint=False)
+    m = mj.ParallelBeamModel((V, R, C), ang)
+    if params:
+        m.set_params(**params)
+    g = op.Geometry("parallel", (V, R, C), ang, recon_shape=m.get_params('recon_shape'),
+                    delta_det_channel=m.get_params('delta_det_channel'),
+                    det_channel_offset=m.get_params('det_channel_offset'), delta_voxel=m.get_params('delta_voxel'))
+    return m, g
+
+
+CASES = {
+    "cone_m2": lambda: make_cone(24, 20, 36, 4 * 36, 2 * 36),
+    "cone_m1_ref_test_geometry": lambda: make_cone(16, 10, 32, 4 * 32, 4 * 32, span=np.pi),
+    "cone_offsets_odd": lambda: make_cone(13, 70, 45, 200.0, 90.0, det_row_offset=1.3, det_channel_offset=-2.2,
+                                          recon_slice_offset=0.7),
+    "cone_tall": lambda: make_cone(8, 150, 20, 160.0, 80.0),
+    "parallel": lambda: make_par(20, 12, 30),
+    "parallel_offset_tall": lambda: make_par(9, 130, 17, det_channel_offset=0.6),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_full_projection_parity(gpu, case):
+    print("case", case, flush=True)
+    m, g = CASES[case]()
+    rows, cols, nz = m.get_params('recon_shape')
+    rng = np.random.default_rng(11)
+    pix = np.arange(rows * cols, dtype=np.int32)
+    x = rng.standard_normal((pix.size, nz)).astype(np.float32)
+    s = m.sparse_forward_project(x, pix)
+    assert rel(s, op.forward(g, x, pix)) <= TOL
+    y = rng.standard_normal(g.sinogram_shape).astype(np.float32)
+    b = m.sparse_back_project(y, pix)
+    assert rel(b, op.back(g, y, pix)) <= TOL
+    h = m.sparse_back_project(np.abs(y), pix, coeff_power=2)
+    assert rel(h, op.back(g, np.abs(y), pix, coeff_power=2)) <= TOL
+
+
+@pytest.mark.parametrize("case", ["cone_m2", "parallel", "cone_offsets_odd"])
+def test_subset_and_view_parity(gpu, case):
+    print("case", case, flush=True)
+    m, g = CASES[case]()
+    rows, cols, nz = m.get_params('recon_shape')
+    np.random.seed(3)
+    part = mj.gen_pixel_partition(m.get_params('recon_shape'), 7)
+    rng = np.random.default_rng(12)
+    V = g.sinogram_shape[0]
+    views = np.array([V - 1, 0, 2, 2, V // 2], dtype=np.int32)
+    for sub in part[:3]:
+        x = rng.standard_normal((sub.size, nz)).astype(np.float32)
+        assert rel(m.sparse_forward_project(x, sub), op.forward(g, x, sub)) <= TOL
+        assert rel(m.sparse_forward_project(x, sub, view_indices=views), op.forward(g, x, sub, views=views)) <= TOL
+        y = rng.standard_normal(g.sinogram_shape).astype(np.float32)
+        assert rel(m.sparse_back_project(y, sub), op.back(g, y, sub)) <= TOL
+        assert rel(m.sparse_back_project(y, sub, view_indices=views), op.back(g, y, sub, views=views)) <= TOL
+
+
+def test_benchmark_c2_full_projections(gpu):
+    """Cone 128 views x 128 x 192, M=2, recon 192x192x128: full forward and back."""
+    m, g = make_cone(128, 128, 192, 768.0, 384.0)
+    assert m.get_params('recon_shape') == (192, 192, 128)
+    phantom = mj.utilities.gen_random_ellipsoid_phantom((192, 192, 128), seed=0)
+    np.random.seed(0)
+    full = mj.gen_full_indices((192, 192, 128))
+    vox = phantom.reshape(-1, 128)[full]
+    s = m.sparse_forward_project(vox, full)
+    ref = op.forward(g, vox, full)
+    r = rel(s, ref)
+    print("C2 forward rel-L2 %.3e" % r, flush=True)
+    assert r <= TOL
+    b = m.sparse_back_project(ref.astype(np.float32), full)
+    rb = rel(b, op.back(g, ref.astype(np.float32), full))
+    print("C2 back rel-L2 %.3e" % rb, flush=True)
+    assert rb <= TOL
+
+
+@pytest.mark.parametrize("case", ["cone_m2", "parallel"])
+def test_adjoint_fp32(gpu, case):
+    m, g = CASES[case]()
+    rows, cols, nz = m.get_params('recon_shape')
+    pix = np.arange(rows * cols, dtype=np.int32)
+    rng = np.random.default_rng(13)
+    x = torch.as_tensor(rng.standard_normal((pix.size, nz)).astype(np.float32), device='cuda')
+    y = torch.as_tensor(rng.standard_normal(g.sinogram_shape).astype(np.float32), device='cuda')
+    Ax = m.sparse_forward_project(x, pix)
+    Aty = m.sparse_back_project(y, pix)
+    lhs = torch.sum(Ax.double() * y.double()).item()
+    rhs = torch.sum(Aty.double() * x.double()).item()
+    denom = torch.linalg.norm(Ax.double()).item() * torch.linalg.norm(y.double()).item()
+    assert abs(lhs - rhs) / denom <= 1e-6
+
+
+def test_hessian_matches_impulse(gpu):
+    """compute_hessian_diagonal == (A^T A e_ijk)_ijk (mbirjax tests/test_projectors.py:264-303)."""
+    m, g = CASES["cone_m2"]()
+    rows, cols, nz = m.get_params('recon_shape')
+    H = m.compute_hessian_diagonal().cpu().numpy()
+    i, j, k = rows // 3, cols // 2, nz // 2
+    x = np.zeros((rows * cols, nz), np.float32)
+    x[i * cols + j, k] = 0.01
+    pix = np.arange(rows * cols, dtype=np.int32)
+    AtAx = m.sparse_back_project(m.sparse_forward_project(x, pix), pix).cpu().numpy()
+    assert np.allclose(H[i, j, k], AtAx[i * cols + j, k] / 0.01, rtol=1e-5)
+    Href = op.back(g, np.ones(g.sinogram_shape), pix, coeff_power=2).reshape(rows, cols, nz)
+    assert rel(H, Href) <= TOL
+
+
+def test_bf16_and_kahan_back_projection(gpu):
+    m, g = CASES["cone_m2"]()
+    rows, cols, nz = m.get_params('recon_shape')
+    pix = torch.arange(rows * cols, dtype=torch.int32, device='cuda')
+    rng = np.random.default_rng(14)
+    y = rng.standard_normal(g.sinogram_shape).astype(np.float32)
+    y16 = torch.as_tensor(y, device='cuda').to(torch.bfloat16)
+    y_rounded = y16.float().cpu().numpy()
+    ref = op.back(g, y_rounded, pix.cpu().numpy())
+    vcr16 = m.to_channel_major(y16.float(), dtype=mj._lib.DTYPE_BF16)
+    for kahan in (False, True):
+        b = m._back(vcr16, pix, kahan=kahan)
+        assert rel(b, ref) <= TOL
+
+
+def test_transpose_round_trip_and_determinism(gpu):
+    m, g = CASES["cone_offsets_odd"]()
+    rng = np.random.default_rng(15)
+    y = torch.as_tensor(rng.standard_normal(g.sinogram_shape).astype(np.float32), device='cuda')
+    back = m.from_channel_major(m.to_channel_major(y))
+    assert torch.equal(back, y)
+    rows, cols, nz = m.get_params('recon_shape')
+    np.random.seed(1)
+    sub = mj.gen_pixel_partition(m.get_params('recon_shape'), 4)[1]
+    x = rng.standard_normal((sub.size, nz)).astype(np.float32)
+    a1 = m.sparse_forward_project(x, sub)
+    a2 = m.sparse_forward_project(x, sub)
+    assert torch.equal(a1, a2)
+    b1 = m.sparse_back_project(y, sub)
+    b2 = m.sparse_back_project(y, sub)
+    assert torch.equal(b1, b2)
+
+
+def test_forward_accumulate_and_sumsq(gpu):
+    m, g = CASES["cone_m2"]()
+    rows, cols, nz = m.get_params('recon_shape')
+    pix = torch.arange(rows * cols, dtype=torch.int32, device='cuda')
+    rng = np.random.default_rng(16)
+    x = torch.as_tensor(rng.standard_normal((rows * cols, nz)).astype(np.float32), device='cuda')
+    w = torch.as_tensor(rng.random(g.sinogram_shape[:1] + g.sinogram_shape[2:] + g.sinogram_shape[1:2])
+                        .astype(np.float32), device='cuda')
+    s, ss = m._fwd(x, pix, layout=mj._lib.LAYOUT_VCR, weights=w, want_sumsq=True)
+    assert abs(ss.item() - torch.sum(s.double() ** 2 * w.double()).item()) <= 1e-9 * ss.item()
+    s2 = s.clone()
+    m._fwd(x, pix, layout=mj._lib.LAYOUT_VCR, out=s2, accumulate=True)
+    assert torch.allclose(s2, 2 * s, rtol=1e-6, atol=1e-6)

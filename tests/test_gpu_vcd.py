@@ -1,0 +1,125 @@
+"""GPU parity of the qGGMRF kernel and of VCD reconstruction against the oracle.
+
+Bars: qGGMRF golden vectors at the reference's allclose (rtol 1e-5); VCD
+reconstructions after a fixed iteration count within NRMSE <= 1e-3 of the FP64
+oracle run with the same seeded partitions.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import mbirjax_amd as mj
+from oracle import partitions as opart
+from oracle import projectors as op
+from oracle import qggmrf as oq
+from oracle import vcd as ovcd
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "qggmrf_golden.json")
+
+
+def test_qggmrf_golden_gpu(gpu):
+    with open(GOLDEN) as f:
+        g = json.load(f)
+    params = (tuple(g["b"]), g["sigma_x"], g["p"], g["q"], g["T"])
+    grad, hess = mj.qggmrf_gradient_and_hessian_at_indices(np.array(g["flat_recon"], np.float32), (3, 3, 3),
+                                                           np.arange(9), params)
+    assert np.allclose(grad.cpu().numpy(), np.array(g["grad_ref"]), rtol=1e-5, atol=1e-6)
+    assert np.allclose(hess.cpu().numpy(), np.array(g["hess_ref"]), rtol=1e-5, atol=1e-6)
+
+
+def test_qggmrf_random_vs_oracle(gpu):
+    rng = np.random.default_rng(21)
+    shape = (17, 13, 70)
+    x = rng.random((shape[0] * shape[1], shape[2])).astype(np.float32)
+    params = (mj.get_b_from_nbr_wts([1.0, 2.0, 0.5]), 0.3, 2.0, 1.2, 1.0)
+    idx = rng.choice(shape[0] * shape[1], size=60, replace=False)
+    g, h = mj.qggmrf_gradient_and_hessian_at_indices(x, shape, idx, params)
+    go, ho = oq.gradient_and_hessian_at_indices(x.astype(np.float64), shape, idx, params)
+    assert np.linalg.norm(g.cpu().numpy() - go) / np.linalg.norm(go) <= 1e-5
+    assert np.linalg.norm(h.cpu().numpy() - ho) / np.linalg.norm(ho) <= 1e-5
+
+
+def problem(kind):
+    if kind == "cone":
+        V, R, C = 24, 16, 32
+        ang = np.linspace(0, 2 * np.pi, V, endpoint=False)
+        m = mj.ConeBeamModel((V, R, C), ang, source_detector_dist=4 * C, source_iso_dist=2 * C)
+        g = op.Geometry("cone", (V, R, C), ang, 4 * C, 2 * C)
+    else:
+        V, R, C = 32, 16, 32
+        ang = np.linspace(0, np.pi, V, endpoint=False)
+        m = mj.ParallelBeamModel((V, R, C), ang)
+        g = op.Geometry("parallel", (V, R, C), ang)
+    assert m.get_params('recon_shape') == g.recon_shape
+    phantom = mj.utilities.gen_random_ellipsoid_phantom(g.recon_shape, seed=0)
+    rows, cols, nz = g.recon_shape
+    ror = np.flatnonzero(opart.get_2d_ror_mask(g.recon_shape))
+    sino = op.forward(g, phantom.reshape(-1, nz)[ror], ror)
+    noise = np.random.default_rng(1).standard_normal(sino.shape)
+    sino = (sino + 0.01 * np.max(sino) * noise).astype(np.float32)
+    return m, g, phantom, sino
+
+
+@pytest.mark.parametrize("kind", ["parallel", "cone"])
+def test_vcd_recon_matches_oracle(gpu, kind):
+    print("vcd parity", kind, flush=True)
+    m, g, phantom, sino = problem(kind)
+    m.set_params(sharpness=0.0, verbose=0)
+    np.random.seed(5)
+    recon, rd = m.recon(sino, max_iterations=6, stop_threshold_change_pct=0, print_logs=False, logfile_path=None)
+    recon = recon.cpu().numpy().astype(np.float64)
+    om = ovcd.OracleModel(g, sharpness=0.0)
+    np.random.seed(5)
+    ref, stats = om.recon(sino, max_iterations=6, stop_threshold_change_pct=0)
+    nrmse = np.linalg.norm(recon - ref) / np.linalg.norm(ref)
+    print("%s recon NRMSE vs oracle %.3e, vs phantom %.3f" % (
+        kind, nrmse, np.linalg.norm(recon - phantom) / np.linalg.norm(phantom)), flush=True)
+    assert nrmse <= 1e-3
+    rp = rd['recon_params']
+    assert rp['num_iterations'] == 6
+    assert np.allclose(rp['fm_rmse'], stats['fm_rmse'], rtol=1e-3)
+    assert np.allclose(rp['alpha_values'], stats['alpha_values'], rtol=1e-3, atol=1e-6)
+    assert np.isclose(rp['regularization_params']['sigma_y'], stats['regularization_params']['sigma_y'], rtol=1e-7)
+
+
+def test_error_sinogram_invariant(gpu):
+    """e == y - A x after the reconstruction (the overcomplete VCD state)."""
+    m, g, phantom, sino = problem("cone")
+    m.set_params(sharpness=0.0, verbose=0)
+    np.random.seed(2)
+    recon, _ = m.recon(sino, max_iterations=4, stop_threshold_change_pct=0, print_logs=False, logfile_path=None)
+    e = m.from_channel_major(m.last_error_sinogram.float())
+    Ax = m.forward_project(recon)
+    y = torch.as_tensor(sino, device='cuda')
+    resid = y - Ax
+    assert (torch.linalg.norm(resid - e) / torch.linalg.norm(y)).item() <= 1e-5
+
+
+def test_recon_quality_reference_tolerances(gpu):
+    """Phantom tolerances of mbirjax tests/test_vcd.py:21-22 at its geometry
+    (64 views x 40 rows x 128 channels; cone with sdd = sid = 4*128)."""
+    V, R, C = 64, 40, 128
+    for kind, tol in (("parallel", (0.15, 0.38, 0.04)), ("cone", (0.19, 0.56, 0.05))):
+        if kind == "cone":
+            cone_angle = 2 * np.arctan2(C / 2, 4 * C)
+            ang = np.linspace(-(np.pi + cone_angle) / 2, (np.pi + cone_angle) / 2, V, endpoint=False)
+            m = mj.ConeBeamModel((V, R, C), ang, source_detector_dist=4 * C, source_iso_dist=4 * C)
+        else:
+            ang = np.linspace(-np.pi / 2, np.pi / 2, V, endpoint=False)
+            m = mj.ParallelBeamModel((V, R, C), ang)
+        np.random.seed(0)
+        phantom = mj.utilities.generate_3d_shepp_logan_low_dynamic_range(m.get_params('recon_shape'))
+        sino = m.forward_project(phantom)
+        m.set_params(verbose=0)
+        recon, _ = m.recon(sino, print_logs=False, logfile_path=None)
+        r = recon.cpu().numpy()
+        nrmse = np.linalg.norm(r - phantom) / np.linalg.norm(phantom)
+        max_diff = np.max(np.abs(r - phantom))
+        p95 = np.percentile(np.abs(r - phantom), 95)
+        print(kind, "nrmse %.3f max %.3f p95 %.3f" % (nrmse, max_diff, p95), flush=True)
+        assert nrmse < tol[0] and max_diff < tol[1] and p95 < tol[2]
