@@ -417,31 +417,37 @@ class TomographyModel(ParameterHandler):
         nbr, sigma_x, p, q, T = self.get_params(['qggmrf_nbr_wts', 'sigma_x', 'p', 'q', 'T'])
         return (_qggmrf.get_b_from_nbr_wts(nbr), float(sigma_x), float(p), float(q), float(T))
 
-    def vcd_recon(self, sinogram, partitions, partition_sequence, stop_threshold_change_pct, weights=None,
-                  init_recon=None, prox_input=None, compute_prior_loss=False, first_iteration=0, precision='fp32'):
-        """Multi-granular VCD on the device (mbirjax/tomography_model.py:1086-1255).
-        Returns (recon (rows, cols, Nz), (fm_rmse, prior_loss, nmae_update, alpha_values))."""
+    def vcd_setup(self, sinogram, weights=None, init_recon=None, prox_input=None, precision='fp32',
+                  max_subset_size=None):
+        """Initial VCD state on the device: e = y - alpha0 A x0 with the optimal
+        scale alpha0 of the initial recon, the forward-model Hessian diagonal and
+        the subset-update workspace (mbirjax/tomography_model.py:1110-1192).
+        sinogram may already be a channel-major device tensor (views, C, R)."""
         if precision not in ('fp32', 'mixed'):
             raise ValueError("precision must be 'fp32' or 'mixed'")
         dev = self._device()
         recon_shape = self.get_params('recon_shape')
         rows, cols, nz = recon_shape
-        y = self.to_channel_major(sinogram)
+        V, R, C = self.get_params('sinogram_shape')
+        if isinstance(sinogram, torch.Tensor) and sinogram.is_cuda and tuple(sinogram.shape) == (V, C, R) and \
+                sinogram.dtype == torch.float32 and (R != C):
+            y = sinogram.contiguous()
+        else:
+            y = self.to_channel_major(sinogram)
         w = None if weights is None else self.to_channel_major(weights)
         if init_recon is None:
-            self.logger.info('Starting direct recon for initial reconstruction')
-            init_recon = self.direct_recon(sinogram)
+            self._log('info', 'Starting direct recon for initial reconstruction')
+            init_recon = self.direct_recon(self.from_channel_major(y))
         elif isinstance(init_recon, int) and init_recon == 0:
             init_recon = torch.zeros(recon_shape, dtype=torch.float32, device=dev)
         init_recon = self._tensor(init_recon)
         if tuple(init_recon.shape) != tuple(recon_shape):
             msg = "init_recon does not have the correct shape. \nExpected {}, but got shape {} for init_recon " \
                   "shape.".format(recon_shape, tuple(init_recon.shape))
-            self.logger.error(msg)
+            self._log('error', msg)
             raise ValueError(msg)
 
-        # error sinogram and the optimal scale of the initial recon
-        self.logger.info('Initializing error sinogram')
+        self._log('info', 'Initializing error sinogram')
         full = vcd_utils.gen_full_indices(recon_shape, use_ror_mask=self.use_ror_mask)
         full_d = torch.as_tensor(full, device=dev)
         Ax = self._fwd(init_recon.reshape(-1, nz)[full_d.long()].contiguous(), full_d, layout=_lib.LAYOUT_VCR)
@@ -455,7 +461,6 @@ class TomographyModel(ParameterHandler):
         del Ax
         if e_dtype == _BF16:
             e = torch.empty(e32.shape, dtype=torch.bfloat16, device=dev)
-            V, C, R = e32.shape
             _lib.check(_lib.lib().mbx_sino_transpose(_lib.ptr(e32), _F32, _lib.LAYOUT_VCR, _lib.ptr(e), _BF16,
                                                      _lib.LAYOUT_VCR, V, R, C, _lib.stream()))
             del e32
@@ -468,73 +473,66 @@ class TomographyModel(ParameterHandler):
             if tuple(prox_input.shape) != tuple(recon_shape):
                 msg = "prox_input does not have the correct size. \nExpected {}, but got shape {} for prox_input " \
                       "shape.".format(recon_shape, tuple(prox_input.shape))
-                self.logger.error(msg)
+                self._log('error', msg)
                 raise ValueError(msg)
             prox_input = prox_input.reshape(-1, nz).contiguous()
 
-        verbose, sigma_y = self.get_params(['verbose', 'sigma_y'])
-        self.logger.info('Computing Hessian diagonal')
+        self._log('info', 'Computing Hessian diagonal')
         w_h = w if w is not None else torch.ones_like(y)
         fm_hessian = self._hessian_vcr(w_h)
         del w_h, y
+        if max_subset_size is None:
+            max_subset_size = rows * cols
+        return VCDState(self, e, e_dtype, w, flat_recon, fm_hessian, prox_input, precision, max_subset_size)
 
-        fm_constant = 1.0 / (float(sigma_y) ** 2.0)
-        qp = self._qggmrf_params()
-        qstruct = _lib.qggmrf_struct(*qp)
-        sigma_prox = float(self.get_params('sigma_prox'))
-        positivity = bool(self.get_params('positivity_flag'))
-        kahan = precision == 'mixed'
-        parts_dev = [torch.as_tensor(np.ascontiguousarray(p, dtype=np.int32), device=dev) for p in partitions]
+    def _log(self, level, msg):
+        if self.logger is not None:
+            getattr(self.logger, level)(msg)
+
+    def vcd_recon(self, sinogram, partitions, partition_sequence, stop_threshold_change_pct, weights=None,
+                  init_recon=None, prox_input=None, compute_prior_loss=False, first_iteration=0, precision='fp32'):
+        """Multi-granular VCD on the device (mbirjax/tomography_model.py:1086-1255).
+        Returns (recon (rows, cols, Nz), (fm_rmse, prior_loss, nmae_update, alpha_values))."""
+        recon_shape = self.get_params('recon_shape')
         nS_max = max(int(p.shape[1]) for p in partitions)
-        wsb = int(_lib.lib().mbx_vcd_workspace_bytes(self._geom, nS_max, e_dtype))
-        ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
-        stats = torch.zeros(4, dtype=torch.float64, device=dev)
-
-        self.logger.info('Starting VCD iterations')
+        state = self.vcd_setup(sinogram, weights=weights, init_recon=init_recon, prox_input=prox_input,
+                               precision=precision, max_subset_size=nS_max)
+        dev = state.e.device
+        parts_dev = [torch.as_tensor(np.ascontiguousarray(p, dtype=np.int32), device=dev) for p in partitions]
+        verbose = self.get_params('verbose')
+        self._log('info', 'Starting VCD iterations')
         max_iters = int(np.size(partition_sequence))
         fm_rmse = np.zeros(max_iters)
         pm_loss = np.zeros(max_iters)
         nmae_update = np.zeros(max_iters)
         alpha_values = np.zeros(max_iters)
         num_iters = 0
-        L = _lib.lib()
-        st = _lib.stream()
-        angles = _lib.ptr(self._angles())
-        n_e = e.numel()
-        avg_w = 1.0 if w is None else float(torch.mean(w, dtype=torch.float64).item())
         for i in range(max_iters):
             part = parts_dev[int(partition_sequence[i])]
             order = np.random.permutation(part.shape[0])
-            stats.zero_()
-            for s in order:
-                subset = part[int(s)]
-                _lib.check(L.mbx_vcd_subset_update(
-                    self._geom, angles, _lib.ptr(flat_recon), _lib.ptr(e), e_dtype, _lib.ptr(w),
-                    _lib.ptr(fm_hessian), _lib.ptr(subset), int(subset.numel()), fm_constant, qstruct,
-                    _lib.ptr(prox_input), sigma_prox, int(positivity), e_dtype, int(kahan), _lib.ptr(stats),
-                    _lib.ptr(ws), wsb, st))
-            ell1, alpha_sum, _, _ = stats.cpu().tolist()
-            fm_rmse[i] = math.sqrt(fm_constant * self._dot(e, None, w) / (n_e * avg_w))
-            nmae_update[i] = ell1 / float(torch.sum(torch.abs(flat_recon), dtype=torch.float64).item())
-            es_rmse = math.sqrt(self._dot(e) / n_e)
+            ell1, alpha_sum = state.iterate(part, order)
+            fm_rmse[i] = state.forward_model_rmse()
+            nmae_update[i] = ell1 / float(torch.sum(torch.abs(state.flat_recon), dtype=torch.float64).item())
             alpha_values[i] = alpha_sum / part.shape[0]
             if verbose >= 1:
                 msg = '\nAfter iteration {} of a max of {}: Pct change={:.4f}, Forward loss={:.4f}'.format(
                     i + first_iteration, max_iters + first_iteration, 100 * nmae_update[i], fm_rmse[i])
                 if compute_prior_loss:
-                    pm_loss[i] = _qggmrf.qggmrf_loss(flat_recon.reshape(recon_shape), qp) / flat_recon.numel()
-                    total = (fm_rmse[i] * n_e + pm_loss[i] * flat_recon.numel()) / (0.5 * (n_e + flat_recon.numel()))
+                    n_e, n_x = state.e.numel(), state.flat_recon.numel()
+                    pm_loss[i] = _qggmrf.qggmrf_loss(state.flat_recon.reshape(recon_shape), state.qp) / n_x
+                    total = (fm_rmse[i] * n_e + pm_loss[i] * n_x) / (0.5 * (n_e + n_x))
                     msg += ', Prior loss={:.4f}, Weighted total loss={:.4f}'.format(pm_loss[i], total)
-                self.logger.info(msg)
-                self.logger.info(f'Relative step size (alpha)={alpha_values[i]:.2f}, Error sino RMSE={es_rmse:.4f}')
-                self.logger.info('Number subsets = {}'.format(part.shape[0]))
+                self._log('info', msg)
+                self._log('info', 'Relative step size (alpha)={:.2f}, Error sino RMSE={:.4f}'.format(
+                    alpha_values[i], state.error_rmse()))
+                self._log('info', 'Number subsets = {}'.format(part.shape[0]))
             num_iters += 1
             if nmae_update[i] < stop_threshold_change_pct / 100:
-                self.logger.warning('Change threshold stopping condition reached')
+                self._log('warning', 'Change threshold stopping condition reached')
                 break
-        self.last_error_sinogram = e
-        return self.reshape_recon(flat_recon), (fm_rmse[:num_iters], pm_loss[:num_iters], nmae_update[:num_iters],
-                                               alpha_values[:num_iters])
+        self.last_error_sinogram = state.e
+        return self.reshape_recon(state.flat_recon), (fm_rmse[:num_iters], pm_loss[:num_iters],
+                                                      nmae_update[:num_iters], alpha_values[:num_iters])
 
     @staticmethod
     def get_forward_model_loss(error_sinogram, sigma_y, weights=None, normalize=True):
@@ -597,3 +595,56 @@ class TomographyModel(ParameterHandler):
             model = cls(**required)
             model.set_params(**params)
         return model
+
+
+class VCDState:
+    """Device state of one VCD reconstruction: the recon (rows*cols, Nz), the
+    error sinogram e = y - A x in channel-major layout (float32, or bfloat16
+    for precision='mixed'), weights, Hessian diagonal and workspace.  iterate()
+    runs one partition through mbx_vcd_subset_update with no host syncs."""
+
+    def __init__(self, model, e, e_dtype, w, flat_recon, fm_hessian, prox_input, precision, max_subset_size):
+        self.model = model
+        self.e = e
+        self.e_dtype = e_dtype
+        self.w = w
+        self.flat_recon = flat_recon
+        self.fm_hessian = fm_hessian
+        self.prox_input = prox_input
+        self.precision = precision
+        sigma_y = model.get_params('sigma_y')
+        self.fm_constant = 1.0 / (float(sigma_y) ** 2.0)
+        self.qp = model._qggmrf_params()
+        self.qstruct = _lib.qggmrf_struct(*self.qp)
+        self.sigma_prox = float(model.get_params('sigma_prox'))
+        self.positivity = bool(model.get_params('positivity_flag'))
+        self.kahan = precision == 'mixed'
+        self.wsb = int(_lib.lib().mbx_vcd_workspace_bytes(model._geom, int(max_subset_size), e_dtype))
+        self.ws = torch.empty(self.wsb, dtype=torch.uint8, device=e.device)
+        self.stats = torch.zeros(4, dtype=torch.float64, device=e.device)
+        self.avg_w = 1.0 if w is None else float(torch.mean(w, dtype=torch.float64).item())
+
+    def update_subset(self, subset):
+        """One subset update (vcd_subset_updater); subset is an int32 device tensor."""
+        m = self.model
+        _lib.check(_lib.lib().mbx_vcd_subset_update(
+            m._geom, _lib.ptr(m._angles()), _lib.ptr(self.flat_recon), _lib.ptr(self.e), self.e_dtype,
+            _lib.ptr(self.w), _lib.ptr(self.fm_hessian), _lib.ptr(subset), int(subset.numel()), self.fm_constant,
+            self.qstruct, _lib.ptr(self.prox_input), self.sigma_prox, int(self.positivity), self.e_dtype,
+            int(self.kahan), _lib.ptr(self.stats), _lib.ptr(self.ws), self.wsb, _lib.stream()))
+
+    def iterate(self, part, order):
+        """All subsets of a partition (K, nS) in the given order (vcd_partition_iterator);
+        returns (sum |alpha delta|, sum alpha) -- one host sync."""
+        self.stats.zero_()
+        for s in order:
+            self.update_subset(part[int(s)])
+        ell1, alpha_sum, _, _ = self.stats.cpu().tolist()
+        return ell1, alpha_sum
+
+    def forward_model_rmse(self):
+        n = self.e.numel()
+        return math.sqrt(self.fm_constant * self.model._dot(self.e, None, self.w) / (n * self.avg_w))
+
+    def error_rmse(self):
+        return math.sqrt(self.model._dot(self.e) / self.e.numel())
