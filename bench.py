@@ -1,0 +1,243 @@
+"""Benchmark: cone-beam 512^3 VCD iterations/s and projector HBM throughput.
+
+Workload (one GPU): ConeBeamModel, 720 views over [0, 2pi), detector 512 rows x
+768 channels, source-detector 3072, source-iso 1536 (magnification 2), recon
+512 x 512 x 512, seeded random-ellipsoid phantom plus 1% Gaussian noise.  A step
+is one steady-state VCD iteration: all 128 subsets of the K=128 partition
+(1602 pixels each), every subset a full back projection + forward projection
++ update over all 720 views.  Inputs are resident in HBM before timing.
+
+Prints one JSON line (bench contract) with the roofline of the dominant kernel
+(HIP events on the launch stream) and a CPU baseline (the float32 oracle port
+on the host cores, a bounded view sample extrapolated to one iteration).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cone-beam 512³ VCD iter/s + projector HBM GB/s, BF16-mixed vs FP32 at equal NRMSE"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (views, rows, channels, sdd, sid, recon_shape, K)
+    "cone512": (720, 512, 768, 3072.0, 1536.0, (512, 512, 512), 128),
+    "cone192": (128, 128, 192, 768.0, 384.0, (192, 192, 128), 128),
+    "cone128": (64, 128, 192, 768.0, 384.0, (128, 128, 128), 128),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed(fn, reps, stream=None):
+    import torch
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(reps):
+        fn()
+    end.record(stream)
+    end.synchronize()
+    return start.elapsed_time(end) / 1e3 / reps
+
+
+def build_problem(name, seed=0):
+    import torch
+    import mbirjax_amd as mj
+    V, R, C, sdd, sid, rshape, K = WORKLOADS[name]
+    angles = np.linspace(0, 2 * np.pi, V, endpoint=False)
+    model = mj.ConeBeamModel((V, R, C), angles, source_detector_dist=sdd, source_iso_dist=sid)
+    model.set_params(recon_shape=rshape, sharpness=0.0, verbose=0)
+    t0 = time.time()
+    phantom = mj.utilities.gen_random_ellipsoid_phantom(rshape, seed=seed)
+    log("phantom %.1fs" % (time.time() - t0))
+    dev = torch.device("cuda")
+    nz = rshape[2]
+    from mbirjax_amd import vcd_utils
+    ror = np.flatnonzero(vcd_utils.get_2d_ror_mask(rshape)).astype(np.int32)
+    ror_d = torch.as_tensor(ror, device=dev)
+    vox = torch.as_tensor(phantom.reshape(-1, nz)[ror], device=dev)
+    del phantom
+    y = model._fwd(vox, ror_d, layout=0)  # channel-major (V, C, R)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed + 1)
+    y += 0.01 * float(y.max()) * torch.randn(y.shape, generator=g, device=dev)
+    torch.cuda.synchronize()
+    del vox
+    # regularisation from the noisy data as recon() would do it
+    model.auto_set_regularization_params(model.from_channel_major(y[::max(1, V // 20)].contiguous()).cpu().numpy())
+    np.random.seed(seed)
+    part = mj.gen_pixel_partition(rshape, K)
+    return model, y, part
+
+
+def algorithmic_bytes(model, nS, b_e, b_d):
+    V, R, C = model.get_params('sinogram_shape')
+    nz = model.get_params('recon_shape')[2]
+    b_back = 4 * nS + b_e * V * R * C + 4 * nS * nz
+    b_fwd = 4 * nS + 4 * nS * nz + b_d * V * R * C
+    return b_back, b_fwd
+
+
+def run_precision(model, y, part_np, precision, steps, warmup):
+    import torch
+    from mbirjax_amd import _lib
+    dev = y.device
+    nS = part_np.shape[1]
+    state = model.vcd_setup(y, init_recon=0, precision=precision, max_subset_size=nS, channel_major=True)
+    part = torch.as_tensor(part_np, device=dev)
+    K = part.shape[0]
+    rng = np.random.RandomState(7)
+    for _ in range(warmup):
+        state.iterate(part, rng.permutation(K))
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    t_wall = time.time()
+    start.record(stream)
+    for _ in range(steps):
+        state.iterate(part, rng.permutation(K))
+    end.record(stream)
+    end.synchronize()
+    torch.cuda.synchronize()
+    t_wall = time.time() - t_wall
+    t_iter = start.elapsed_time(end) / 1e3 / steps
+
+    # dominant kernels alone, same stream, same subset and state
+    sub = part[0]
+    nz = model.get_params('recon_shape')[2]
+    out = torch.empty((nS, nz), dtype=torch.float32, device=dev)
+    kahan = precision == 'mixed'
+    t_back = timed(lambda: model._back(state.e, sub, kahan=kahan, out=out), 20, stream)
+    d_dtype = state.e_dtype
+    vox = torch.randn((nS, nz), device=dev)
+    V, R, C = model.get_params('sinogram_shape')
+    ds = torch.empty((V, C, R), dtype=state.e.dtype, device=dev)
+    t_fwd = timed(lambda: model._fwd(vox, sub, layout=_lib.LAYOUT_VCR, out_dtype=d_dtype, out=ds), 20, stream)
+    b_e = 2 if precision == 'mixed' else 4
+    b_back, b_fwd = algorithmic_bytes(model, nS, b_e, b_e)
+    res = dict(precision=precision, t_iter=t_iter, iter_per_s=1.0 / t_iter, wall_per_iter=t_wall / steps,
+               t_back_ms=t_back * 1e3, t_fwd_ms=t_fwd * 1e3, back_GBps=b_back / t_back / 1e9,
+               fwd_GBps=b_fwd / t_fwd / 1e9, proj_GBps=(b_back + b_fwd) / (t_back + t_fwd) / 1e9,
+               b_back=b_back, b_fwd=b_fwd, fm_rmse=state.forward_model_rmse())
+    del state
+    torch.cuda.empty_cache()
+    return res
+
+
+def cpu_baseline(model, part_np, view_sample=72):
+    """float32 oracle port (oracle/liboracle_f32.so, OpenMP over the host cores)
+    on one K=128 subset, back + forward over a sample of views, scaled to one
+    iteration (K subsets x all views)."""
+    from oracle import projectors as op
+    V, R, C = model.get_params('sinogram_shape')
+    rshape = model.get_params('recon_shape')
+    geom = op.Geometry("cone", (V, R, C), model.get_params('angles'), model.get_params('source_detector_dist'),
+                       model.get_params('source_iso_dist'), recon_shape=rshape)
+    cores = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
+    sub = part_np[0]
+    views = np.linspace(0, V - 1, view_sample).astype(np.int32)
+    rng = np.random.default_rng(0)
+    vox = rng.standard_normal((sub.size, rshape[2])).astype(np.float32)
+    sino = np.zeros((V, R, C), np.float32)
+    sino[views] = rng.standard_normal((view_sample, R, C)).astype(np.float32)
+    t0 = time.time()
+    op.back(geom, sino, sub, views=views, real=np.float32)
+    op.forward(geom, vox, sub, views=views, real=np.float32)
+    dt = time.time() - t0
+    t_iter = dt * (V / view_sample) * part_np.shape[0]
+    return dict(value=1.0 / t_iter, unit="iter/s", cores=cores, kind="port",
+                sample="float32 oracle port, back+forward of one %d-pixel subset over %d of %d views (%.2f s), "
+                       "scaled x%d views x%d subsets" % (sub.size, view_sample, V, dt, V // view_sample,
+                                                         part_np.shape[0]))
+
+
+def copy_calibration(nbytes=1 << 30):
+    """A plain device copy of known size (for rocprofv3 FETCH_SIZE/WRITE_SIZE scaling)."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t = timed(lambda: b.copy_(a), 10)
+    return dict(bytes=2 * nbytes, GBps=2 * nbytes / t / 1e9)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cone512", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="both", choices=["fp32", "mixed", "both"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (views would shard across "
+              "GPUs with one all-reduce per subset; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    torch.manual_seed(0)
+    t0 = time.time()
+    model, y, part = build_problem(args.workload)
+    log("setup %.1fs" % (time.time() - t0))
+    results = {}
+    precisions = ["fp32", "mixed"] if args.precision == "both" else [args.precision]
+    for p in precisions:
+        results[p] = run_precision(model, y, part, p, args.steps, args.warmup)
+        log(p, json.dumps(results[p]))
+    main_p = "fp32" if "fp32" in results else precisions[0]
+    r = results[main_p]
+    dominant = "back" if r["t_back_ms"] >= r["t_fwd_ms"] else "fwd"
+    t_dom = r["t_%s_ms" % dominant] / 1e3
+    b_dom = r["b_%s" % dominant]
+    achieved = b_dom / t_dom / 1e9
+    V, R, C = model.get_params('sinogram_shape')
+    line = {
+        "metric": METRIC,
+        "value": r["iter_per_s"],
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["t_iter"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if main_p == "fp32" else "bf16-storage/f32-accumulate",
+        "data": "synthetic (seeded random-ellipsoid phantom + 1% Gaussian noise)",
+        "config": {"workload": "%s: cone-beam %d views x %d rows x %d channels, recon %s, K=%d subsets of %d pixels, "
+                               "steady-state VCD iteration" % (args.workload, V, R, C,
+                                                               "x".join(map(str, model.get_params('recon_shape'))),
+                                                               part.shape[0], part.shape[1]),
+                   "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "back_kernel" if dominant == "back" else "fwd_kernel",
+                     "bytes_per_launch": b_dom, "avg_launch_ms": t_dom * 1e3},
+        "projector_GBps": r["proj_GBps"],
+        "precisions": {k: {kk: vv for kk, vv in v.items() if kk not in ("b_back", "b_fwd")}
+                       for k, v in results.items()},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(model, part)
+    if args.copy_calibration:
+        line["copy_calibration"] = copy_calibration()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

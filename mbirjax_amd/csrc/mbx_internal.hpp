@@ -37,35 +37,55 @@ int make_dev_geom(const mbx_geom *g, DevGeom *out);
 inline int dtype_size(int dt) { return dt == MBX_DTYPE_BF16 ? 2 : 4; }
 inline bool dtype_ok(int dt) { return dt == MBX_DTYPE_F32 || dt == MBX_DTYPE_BF16; }
 
+// Per-(view, pixel) geometry table shared by both projectors.
+int table_chunk(const DevGeom &g, int npix, int nviews);  // pixels per table
+int64_t table_bytes(int chunk, int nviews);                // records + n_c array
+int launch_table(const DevGeom &g, const float *angles, const int32_t *pix, int npix, const int32_t *views, int nviews,
+                 float *T, int *NC, hipStream_t st);
+inline int *table_nc(float *T, int chunk, int nviews) {
+    return (int *)((char *)T + (((int64_t)chunk * nviews * 16 * 4 + 255) & ~255ll));
+}
+
 // Back projection planning: number of view splits (partials) and slice chunking.
 struct BackPlan {
     int nsplit;  // view groups reduced at the end (1 = write directly)
     int kpl;     // slices per lane (chunk = 64*kpl slices)
     int nchunk;  // slice chunks
+    int cap;     // LDS floats per wave for the detector column
 };
 BackPlan plan_back(const DevGeom &g, int npix, int nviews);
 int64_t back_workspace_bytes(const DevGeom &g, int npix, int nviews);
 
 struct FwdPlan {
-    int cb;      // channels per block
-    int ncb;     // channel blocks
-    int rows_pb; // detector rows per block
-    int nrb;     // row blocks
-    int list;    // candidate list capacity
-    size_t lds;  // dynamic LDS bytes
+    int nw;       // waves per block
+    int rpw;      // detector rows per wave
+    int rows_pb;  // detector rows per block
+    int nrb;      // row blocks
+    int cb;       // channels per block
+    int ncb;      // channel blocks
+    int list;     // candidate list capacity
+    size_t lds;   // dynamic LDS bytes
     int64_t nblocks;
 };
 FwdPlan plan_fwd(const DevGeom &g, int nviews);
-int64_t fwd_workspace_bytes(const DevGeom &g, int nviews);
+int64_t fwd_workspace_bytes(const DevGeom &g, int npix, int nviews);
 
 // Launchers used by the C ABI and by the VCD subset update.  All arguments are
-// already validated.
+// already validated.  launch_back / launch_fwd build the geometry table in
+// their workspace (chunking large pixel sets); the _tab variants take a
+// table that is already built.
 int launch_back(const DevGeom &g, const float *angles, const void *sino, int sino_dtype, const float *weights,
                 const int32_t *pix, int npix, const int32_t *views, int nviews, int coeff_power, int kahan,
                 float scale, int accumulate, float *out, void *ws, hipStream_t st);
 int launch_fwd(const DevGeom &g, const float *angles, const float *vox, const int32_t *pix, int npix,
                const int32_t *views, int nviews, void *sino, int sino_dtype, int layout, int accumulate,
                const float *weights, double *sumsq, void *ws, hipStream_t st);
+int launch_back_tab(const DevGeom &g, const float *T, int npix, const void *sino, int sino_dtype,
+                    const float *weights, const int32_t *views, int nviews, int coeff_power, int kahan, float scale,
+                    int accumulate, float *out, float *part, hipStream_t st);
+int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float *vox, int npix, int nviews, void *sino,
+                   int sino_dtype, int layout, int accumulate, const float *weights, double *part, double *sumsq,
+                   hipStream_t st);
 // Sum `n` doubles in fixed order into out[0] (one block).
 int launch_sum_partials(const double *partials, int64_t n, double *out, int accumulate, hipStream_t st);
 

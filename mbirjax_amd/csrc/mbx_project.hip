@@ -95,6 +95,95 @@ int make_dev_geom(const mbx_geom *g, DevGeom *d) {
 }
 
 // ======================================================================
+// Per-(view, pixel) geometry table
+// ======================================================================
+//
+// One 16-float record per (view slot, subset pixel), computed once per call
+// (or once per VCD subset update) and read by both projectors with scalar
+// loads, so both directions use bit-identical weights:
+//   [0] n_c = rint(n_p) (int bits)   [1..5] Ac(n_c-2 .. n_c+2), zero outside the
+//   tap radius and outside [0, C)    [6] a  [7] W  [8] (W+1)/2  [9] min(1, W)
+//   [10] 1/W  [11] c0  [12] c1        [13] first row [14] last row touched by
+//   any slice (clamped to the detector, int bits)   [15] unused
+// plus n_c again in a dense (view, pixel) int array for the forward scan.
+
+constexpr int kRec = 16;
+constexpr int64_t kTableBudget = 512ll << 20;
+
+int table_chunk(const DevGeom &g, int npix, int nviews) {
+    const int64_t per_pixel = (int64_t)std::max(nviews, 1) * (kRec * 4 + 4);
+    const int64_t c = std::max<int64_t>(1024, kTableBudget / per_pixel);
+    return (int)std::min<int64_t>(npix, c);
+}
+
+int64_t table_bytes(int chunk, int nviews) {
+    const int64_t rec = (int64_t)chunk * nviews * kRec * 4;
+    const int64_t nc = (int64_t)chunk * nviews * 4;
+    return ((rec + 255) & ~255ll) + ((nc + 255) & ~255ll);
+}
+
+template <int GEOM, int RAD>
+__global__ __launch_bounds__(256) void pv_table_kernel(DevGeom g, const float *__restrict__ angles,
+                                                       const int32_t *__restrict__ pix, int npix,
+                                                       const int32_t *__restrict__ views, float *__restrict__ T,
+                                                       int *__restrict__ NC) {
+    const int vi = blockIdx.y;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= npix) return;
+    const int va = views ? views[vi] : vi;
+    float sn, cs;
+    sincosf(angles[va], &sn, &cs);
+    const PV q = pv_full<GEOM>(g, pv_base<GEOM>(g, pix[s], cs, sn));
+    const int nc = (int)rintf(q.np);
+    float r[kRec];
+    r[0] = __int_as_float(nc);
+#pragma unroll
+    for (int j = -2; j <= 2; ++j) r[3 + j] = (j >= -RAD && j <= RAD) ? chan_weight(q, nc + j, g.C) : 0.0f;
+    r[6] = q.a;
+    r[7] = q.W;
+    r[8] = q.hv;
+    r[9] = q.Lv;
+    r[10] = q.invW;
+    r[11] = q.c0;
+    r[12] = q.c1;
+    int mlo = 0, mhi = g.R - 1;
+    if (GEOM == MBX_GEOM_CONE) {
+        mlo = max(0, (int)rintf(q.a) - RAD);
+        mhi = min(g.R - 1, (int)rintf(fmaf(q.W, (float)(g.Nz - 1), q.a)) + RAD);
+    }
+    r[13] = __int_as_float(mlo);
+    r[14] = __int_as_float(mhi);
+    r[15] = 0.0f;
+    float4 *dst = (float4 *)(T + ((size_t)vi * npix + s) * kRec);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dst[i] = make_float4(r[4 * i], r[4 * i + 1], r[4 * i + 2], r[4 * i + 3]);
+    NC[(size_t)vi * npix + s] = nc;
+}
+
+int launch_table(const DevGeom &g, const float *angles, const int32_t *pix, int npix, const int32_t *views, int nviews,
+                 float *T, int *NC, hipStream_t st) {
+    if (npix == 0 || nviews == 0) return MBX_OK;
+    dim3 grid((npix + 255) / 256, nviews);
+    if (g.kind == MBX_GEOM_CONE) {
+        if (g.rh == 1)
+            hipLaunchKernelGGL((pv_table_kernel<MBX_GEOM_CONE, 1>), grid, dim3(256), 0, st, g, angles, pix, npix, views,
+                               T, NC);
+        else
+            hipLaunchKernelGGL((pv_table_kernel<MBX_GEOM_CONE, 2>), grid, dim3(256), 0, st, g, angles, pix, npix, views,
+                               T, NC);
+    } else {
+        if (g.rh == 1)
+            hipLaunchKernelGGL((pv_table_kernel<MBX_GEOM_PARALLEL, 1>), grid, dim3(256), 0, st, g, angles, pix, npix,
+                               views, T, NC);
+        else
+            hipLaunchKernelGGL((pv_table_kernel<MBX_GEOM_PARALLEL, 2>), grid, dim3(256), 0, st, g, angles, pix, npix,
+                               views, T, NC);
+    }
+    MBX_LAUNCH_CHECK();
+    return MBX_OK;
+}
+
+// ======================================================================
 // Back projection
 // ======================================================================
 
@@ -107,36 +196,54 @@ BackPlan plan_back(const DevGeom &g, int npix, int nviews) {
     // Small subsets: split the views over the 8 XCDs so that each XCD's L2
     // streams only its eighth of the sinogram; large passes are VALU-bound.
     p.nsplit = (waves <= 32768 && nviews >= 8) ? 8 : 1;
+    // LDS detector-column capacity per wave: the rows one slice chunk can reach,
+    // from the largest pixel magnification in the recon disc (plus taps), at
+    // least the detector height + 2 (the clamped fallback layout).
+    p.cap = g.R + 4;
+    if (g.kind == MBX_GEOM_CONE) {
+        const double rmax = g.dv * std::sqrt((double)g.img_row_center * g.img_row_center +
+                                             (double)g.img_col_center * g.img_col_center) + g.dv;
+        const double den = (double)g.inv_M - rmax * (double)g.inv_sdd;
+        if (den > 0) {
+            const double wmax = g.dv_over_dr / den;
+            const double want = std::ceil((kWave * p.kpl - 1) * wmax) + 2.0 * g.rh + 8.0;
+            if (want < 16384.0) p.cap = std::max(p.cap, (int)want);
+        }
+    }
     return p;
 }
 
 int64_t back_workspace_bytes(const DevGeom &g, int npix, int nviews) {
-    BackPlan p = plan_back(g, npix, nviews);
-    if (p.nsplit == 1) return 0;
-    return (int64_t)p.nsplit * npix * g.Nz * (int64_t)sizeof(float);
+    if (npix == 0 || nviews == 0) return 0;
+    const int chunk = table_chunk(g, npix, nviews);
+    BackPlan p = plan_back(g, chunk, nviews);
+    int64_t part = p.nsplit == 1 ? 0 : (int64_t)p.nsplit * chunk * g.Nz * (int64_t)sizeof(float);
+    return table_bytes(chunk, nviews) + ((part + 255) & ~255ll);
 }
 
+// One wavefront per (pixel, slice chunk, view group).  The record of each
+// (view, pixel) is wave-uniform (scalar loads); lanes gather the pixel's
+// channel columns row-by-row into an LDS detector column, then each lane
+// resamples the column onto its slices.
 template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN>
-__global__ __launch_bounds__(256) void back_kernel(DevGeom g, const float *__restrict__ angles,
-                                                   const ET *__restrict__ sino, const float *__restrict__ w,
-                                                   const int32_t *__restrict__ pix, int npix,
+__global__ __launch_bounds__(256) void back_kernel(DevGeom g, const float *__restrict__ T, const ET *__restrict__ sino,
+                                                   const float *__restrict__ w, int npix,
                                                    const int32_t *__restrict__ views, int nviews, int nsplit,
-                                                   int nchunk, float scale, int accumulate, float *__restrict__ out,
-                                                   float *__restrict__ part) {
+                                                   int nchunk, int cap, float scale, int accumulate,
+                                                   float *__restrict__ out, float *__restrict__ part) {
     extern __shared__ float lds[];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int split = blockIdx.x % nsplit;
-    const int64_t grp = blockIdx.x / nsplit;
-    const int chunk = (int)(grp % nchunk);
-    const int64_t s = (grp / nchunk) * 4 + wave;
+    const int grp = blockIdx.x / nsplit;
+    const int chunk = grp % nchunk;
+    const int s = __builtin_amdgcn_readfirstlane((grp / nchunk) * 4 + wave);
     if (s >= npix) return;  // whole wavefront leaves; no block barriers below
-    const int p = pix[s];
     const int k0 = chunk * kWave * KPL;
     const int kend = min(g.Nz, k0 + kWave * KPL);
     const int vb = (int)(((int64_t)nviews * split) / nsplit);
     const int ve = (int)(((int64_t)nviews * (split + 1)) / nsplit);
-    float *col = lds + wave * (g.R + 4);
+    float *col = lds + wave * cap;
 
     float acc[KPL], cmp[KPL];
 #pragma unroll
@@ -146,72 +253,102 @@ __global__ __launch_bounds__(256) void back_kernel(DevGeom g, const float *__res
     }
 
     for (int vi = vb; vi < ve; ++vi) {
-        const int vd = views ? views[vi] : vi;
-        float sn, cs;
-        sincosf(angles[vd], &sn, &cs);
-        const PV q = pv_full<GEOM>(g, pv_base<GEOM>(g, p, cs, sn));
-        const int nc = (int)rintf(q.np);
+        const float *rec = T + ((size_t)vi * npix + s) * kRec;
+        const int nc = __float_as_int(rec[0]);
         float Aw[2 * RAD + 1];
 #pragma unroll
         for (int j = -RAD; j <= RAD; ++j) {
-            float a = chan_weight(q, nc + j, g.C);
+            const float a = rec[3 + j];
             Aw[j + RAD] = (POW == 2) ? a * a : a;
         }
+        const int vd = views ? views[vi] : vi;
         const size_t vbase = (size_t)vd * g.C;
 
         if (GEOM == MBX_GEOM_CONE) {
-            const float t0 = fmaf(q.W, (float)k0, q.a);
-            const float t1 = fmaf(q.W, (float)(kend - 1), q.a);
-            const int mlo = max(0, (int)rintf(t0) - RAD);
-            const int mhi = min(g.R - 1, (int)rintf(t1) + RAD);
+            const float a = rec[6], W = rec[7], hv = rec[8], Lv = rec[9], c0 = rec[11], c1 = rec[12];
+            // rows reached by the taps of slices [k0, kend): unclamped window and its part on the detector
+            const int mfirst = (int)rintf(fmaf(W, (float)k0, a)) - RAD;
+            const int mlast = (int)rintf(fmaf(W, (float)(kend - 1), a)) + RAD;
+            const int mlo = max(mfirst, 0), mhi = min(mlast, g.R - 1);
             if (mlo > mhi) continue;  // wave-uniform: this view misses the detector
-            const int span = mhi - mlo + 1;
-            // detector column of this pixel: col[m] = sum_j Ac_j * e[v, nc+j, m]
-            for (int m = lane; m < span; m += kWave) {
+            // Fast layout: col[m - mfirst] for every row of the window (zeros off
+            // the detector), so the taps need no clamping.  If the window does not
+            // fit, fall back to the detector part plus one zero on each side.
+            const bool fit = mlast - mfirst + 1 <= cap;
+            const int mstart = fit ? mfirst : mlo - 1;
+            const int count = fit ? mlast - mfirst + 1 : mhi - mlo + 3;
+            const int64_t cofs = ((int64_t)vbase + (nc - RAD)) * (int64_t)g.R;  // channel nc-RAD, row 0
+            for (int i = lane; i < count; i += kWave) {
+                const int m = mstart + i;
                 float c = 0.0f;
+                if (m >= mlo && m <= mhi) {
 #pragma unroll
-                for (int j = -RAD; j <= RAD; ++j) {
-                    const int n = nc + j;
-                    if (n >= 0 && n < g.C) {
-                        const size_t off = (vbase + n) * (size_t)g.R + (size_t)(mlo + m);
-                        float e = to_float(sino[off]);
-                        if (w) e *= w[off];
-                        c = fmaf(Aw[j + RAD], e, c);
+                    for (int j = 0; j <= 2 * RAD; ++j) {
+                        const int n = nc - RAD + j;
+                        if (n >= 0 && n < g.C) {
+                            const int64_t off = cofs + (int64_t)j * g.R + m;
+                            float e = to_float(sino[off]);
+                            if (w) e *= w[off];
+                            c = fmaf(Aw[j], e, c);
+                        }
                     }
                 }
-                col[1 + m] = c;
-            }
-            if (lane == 0) {
-                col[0] = 0.0f;
-                col[span + 1] = 0.0f;
+                col[i] = c;
             }
             wave_sync();
+            const float hm1 = hv - 1.0f;
 #pragma unroll
             for (int i = 0; i < KPL; ++i) {
-                const int k = k0 + lane + kWave * i;
-                const float t = fmaf(q.W, (float)k, q.a);
+                // lanes past the last slice repeat it (their sums are never stored)
+                const float kf = (float)min(k0 + lane + kWave * i, kend - 1);
+                const float t = fmaf(W, kf, a);
                 const float mc = rintf(t);
-                const float d = t - mc;
-                const int base = (int)mc - mlo + 1;
-                float sum = 0.0f;
+                const float d = t - mc;  // |d| <= 1/2
+                const int base = (int)mc - RAD - mstart;  // buffer index of row mc - RAD
+                float sum;
+                if (RAD == 1) {
+                    float wm = __builtin_amdgcn_fmed3f(hm1 - d, 0.0f, Lv);        // row mc-1
+                    float w0 = __builtin_amdgcn_fmed3f(hv - fabsf(d), 0.0f, Lv);  // row mc
+                    float wp = __builtin_amdgcn_fmed3f(hm1 + d, 0.0f, Lv);        // row mc+1
+                    if (POW == 2) {
+                        w0 *= w0;
+                        wp *= wp;
+                        wm *= wm;
+                    }
+                    float cm, c0v, cp;
+                    if (fit) {
+                        const float *cb = col + base;
+                        cm = cb[0];
+                        c0v = cb[1];
+                        cp = cb[2];
+                    } else {
+                        cm = col[min(max(base, 0), count - 1)];
+                        c0v = col[min(max(base + 1, 0), count - 1)];
+                        cp = col[min(max(base + 2, 0), count - 1)];
+                    }
+                    sum = w0 * c0v;
+                    sum = fmaf(wp, cp, sum);
+                    sum = fmaf(wm, cm, sum);
+                } else {
+                    sum = 0.0f;
 #pragma unroll
-                for (int j = -RAD; j <= RAD; ++j) {
-                    float wj = row_weight(q, d - (float)j);
-                    if (POW == 2) wj *= wj;
-                    const int idx = min(max(base + j, 0), span + 1);
-                    sum = fmaf(wj, col[idx], sum);
+                    for (int j = 0; j <= 2 * RAD; ++j) {
+                        float wj = __builtin_amdgcn_fmed3f(hv - fabsf(d - (float)(j - RAD)), 0.0f, Lv);
+                        if (POW == 2) wj *= wj;
+                        const int idx = fit ? base + j : min(max(base + j, 0), count - 1);
+                        sum = fmaf(wj, col[idx], sum);
+                    }
                 }
-                const float u = fmaf(q.c1, (float)k, q.c0);
+                const float u = fmaf(c1, kf, c0);
                 const float s2 = fmaf(u, u, 1.0f);
                 const float f = (POW == 2) ? s2 : __builtin_amdgcn_sqrtf(s2);
-                const float contrib = sum * f;
                 if (KAHAN) {
-                    const float y = contrib - cmp[i];
+                    const float y = sum * f - cmp[i];
                     const float tt = acc[i] + y;
                     cmp[i] = (tt - acc[i]) - y;
                     acc[i] = tt;
                 } else {
-                    acc[i] += contrib;
+                    acc[i] = fmaf(sum, f, acc[i]);
                 }
             }
             wave_sync();
@@ -271,88 +408,88 @@ __global__ __launch_bounds__(256) void back_reduce_kernel(const float *__restric
 }
 
 template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN>
-static void launch_back_t(const DevGeom &g, const BackPlan &pl, const float *angles, const ET *sino, const float *w,
-                          const int32_t *pix, int npix, const int32_t *views, int nviews, float scale, int accumulate,
-                          float *out, float *part, hipStream_t st) {
+static void launch_back_t(const DevGeom &g, const BackPlan &pl, const float *T, const ET *sino, const float *w,
+                          int npix, const int32_t *views, int nviews, float scale, int accumulate, float *out,
+                          float *part, hipStream_t st) {
     const int64_t quads = ((int64_t)npix + 3) / 4;
     const int64_t blocks = quads * pl.nchunk * pl.nsplit;
-    const size_t lds = GEOM == MBX_GEOM_CONE ? 4 * (size_t)(g.R + 4) * sizeof(float) : 0;
-    hipLaunchKernelGGL((back_kernel<GEOM, KPL, RAD, ET, POW, KAHAN>), dim3((unsigned)blocks), dim3(256), lds, st, g,
-                       angles, sino, w, pix, npix, views, nviews, pl.nsplit, pl.nchunk, scale, accumulate, out, part);
+    const size_t lds = GEOM == MBX_GEOM_CONE ? 4 * (size_t)pl.cap * sizeof(float) : 0;
+    hipLaunchKernelGGL((back_kernel<GEOM, KPL, RAD, ET, POW, KAHAN>), dim3((unsigned)blocks), dim3(256), lds, st, g, T,
+                       sino, w, npix, views, nviews, pl.nsplit, pl.nchunk, pl.cap, scale, accumulate, out, part);
 }
 
 template <int GEOM, int KPL, int RAD>
-static int dispatch_back_3(const DevGeom &g, const BackPlan &pl, const float *angles, const void *sino, int dt,
-                           const float *w, const int32_t *pix, int npix, const int32_t *views, int nviews, int pw,
-                           int kahan, float scale, int accumulate, float *out, float *part, hipStream_t st) {
+static int dispatch_back_3(const DevGeom &g, const BackPlan &pl, const float *T, const void *sino, int dt,
+                           const float *w, int npix, const int32_t *views, int nviews, int pw, int kahan, float scale,
+                           int accumulate, float *out, float *part, hipStream_t st) {
     if (pw == 2) {
         if (dt != MBX_DTYPE_F32 || kahan)
             return fail(MBX_EUNSUPPORTED, "coeff_power=2 supports float32 input without Kahan summation");
-        launch_back_t<GEOM, KPL, RAD, float, 2, false>(g, pl, angles, (const float *)sino, w, pix, npix, views, nviews,
-                                                       scale, accumulate, out, part, st);
+        launch_back_t<GEOM, KPL, RAD, float, 2, false>(g, pl, T, (const float *)sino, w, npix, views, nviews, scale,
+                                                       accumulate, out, part, st);
     } else if (dt == MBX_DTYPE_F32) {
         if (kahan)
-            launch_back_t<GEOM, KPL, RAD, float, 1, true>(g, pl, angles, (const float *)sino, w, pix, npix, views,
-                                                          nviews, scale, accumulate, out, part, st);
+            launch_back_t<GEOM, KPL, RAD, float, 1, true>(g, pl, T, (const float *)sino, w, npix, views, nviews, scale,
+                                                          accumulate, out, part, st);
         else
-            launch_back_t<GEOM, KPL, RAD, float, 1, false>(g, pl, angles, (const float *)sino, w, pix, npix, views,
-                                                           nviews, scale, accumulate, out, part, st);
+            launch_back_t<GEOM, KPL, RAD, float, 1, false>(g, pl, T, (const float *)sino, w, npix, views, nviews,
+                                                           scale, accumulate, out, part, st);
     } else {
         if (kahan)
-            launch_back_t<GEOM, KPL, RAD, bf16, 1, true>(g, pl, angles, (const bf16 *)sino, w, pix, npix, views,
-                                                         nviews, scale, accumulate, out, part, st);
+            launch_back_t<GEOM, KPL, RAD, bf16, 1, true>(g, pl, T, (const bf16 *)sino, w, npix, views, nviews, scale,
+                                                         accumulate, out, part, st);
         else
-            launch_back_t<GEOM, KPL, RAD, bf16, 1, false>(g, pl, angles, (const bf16 *)sino, w, pix, npix, views,
-                                                          nviews, scale, accumulate, out, part, st);
+            launch_back_t<GEOM, KPL, RAD, bf16, 1, false>(g, pl, T, (const bf16 *)sino, w, npix, views, nviews, scale,
+                                                          accumulate, out, part, st);
     }
     return MBX_OK;
 }
 
 template <int GEOM, int KPL>
-static int dispatch_back_2(const DevGeom &g, const BackPlan &pl, const float *angles, const void *sino, int dt,
-                           const float *w, const int32_t *pix, int npix, const int32_t *views, int nviews, int pw,
-                           int kahan, float scale, int accumulate, float *out, float *part, hipStream_t st) {
+static int dispatch_back_2(const DevGeom &g, const BackPlan &pl, const float *T, const void *sino, int dt,
+                           const float *w, int npix, const int32_t *views, int nviews, int pw, int kahan, float scale,
+                           int accumulate, float *out, float *part, hipStream_t st) {
     if (g.rh == 1)
-        return dispatch_back_3<GEOM, KPL, 1>(g, pl, angles, sino, dt, w, pix, npix, views, nviews, pw, kahan, scale,
-                                             accumulate, out, part, st);
-    return dispatch_back_3<GEOM, KPL, 2>(g, pl, angles, sino, dt, w, pix, npix, views, nviews, pw, kahan, scale,
-                                         accumulate, out, part, st);
+        return dispatch_back_3<GEOM, KPL, 1>(g, pl, T, sino, dt, w, npix, views, nviews, pw, kahan, scale, accumulate,
+                                             out, part, st);
+    return dispatch_back_3<GEOM, KPL, 2>(g, pl, T, sino, dt, w, npix, views, nviews, pw, kahan, scale, accumulate, out,
+                                         part, st);
 }
 
 template <int GEOM>
-static int dispatch_back_1(const DevGeom &g, const BackPlan &pl, const float *angles, const void *sino, int dt,
-                           const float *w, const int32_t *pix, int npix, const int32_t *views, int nviews, int pw,
-                           int kahan, float scale, int accumulate, float *out, float *part, hipStream_t st) {
+static int dispatch_back_1(const DevGeom &g, const BackPlan &pl, const float *T, const void *sino, int dt,
+                           const float *w, int npix, const int32_t *views, int nviews, int pw, int kahan, float scale,
+                           int accumulate, float *out, float *part, hipStream_t st) {
     switch (pl.kpl) {
         case 2:
-            return dispatch_back_2<GEOM, 2>(g, pl, angles, sino, dt, w, pix, npix, views, nviews, pw, kahan, scale,
-                                            accumulate, out, part, st);
+            return dispatch_back_2<GEOM, 2>(g, pl, T, sino, dt, w, npix, views, nviews, pw, kahan, scale, accumulate,
+                                            out, part, st);
         case 4:
-            return dispatch_back_2<GEOM, 4>(g, pl, angles, sino, dt, w, pix, npix, views, nviews, pw, kahan, scale,
-                                            accumulate, out, part, st);
+            return dispatch_back_2<GEOM, 4>(g, pl, T, sino, dt, w, npix, views, nviews, pw, kahan, scale, accumulate,
+                                            out, part, st);
         case 8:
-            return dispatch_back_2<GEOM, 8>(g, pl, angles, sino, dt, w, pix, npix, views, nviews, pw, kahan, scale,
-                                            accumulate, out, part, st);
+            return dispatch_back_2<GEOM, 8>(g, pl, T, sino, dt, w, npix, views, nviews, pw, kahan, scale, accumulate,
+                                            out, part, st);
         default:
-            return dispatch_back_2<GEOM, 16>(g, pl, angles, sino, dt, w, pix, npix, views, nviews, pw, kahan, scale,
-                                             accumulate, out, part, st);
+            return dispatch_back_2<GEOM, 16>(g, pl, T, sino, dt, w, npix, views, nviews, pw, kahan, scale, accumulate,
+                                             out, part, st);
     }
 }
 
-int launch_back(const DevGeom &g, const float *angles, const void *sino, int sino_dtype, const float *weights,
-                const int32_t *pix, int npix, const int32_t *views, int nviews, int coeff_power, int kahan, float scale,
-                int accumulate, float *out, void *ws, hipStream_t st) {
+// Back projection of pixels whose table T (nviews x npix records) is built.
+int launch_back_tab(const DevGeom &g, const float *T, int npix, const void *sino, int sino_dtype,
+                    const float *weights, const int32_t *views, int nviews, int coeff_power, int kahan, float scale,
+                    int accumulate, float *out, float *part, hipStream_t st) {
     if (npix == 0) return MBX_OK;
     const BackPlan pl = plan_back(g, npix, nviews);
-    float *part = pl.nsplit > 1 ? (float *)ws : nullptr;
-    if (pl.nsplit > 1 && !part) return fail(MBX_EWORKSPACE, "back projection needs a workspace for this size");
+    if (pl.nsplit > 1 && !part) return fail(MBX_EWORKSPACE, "back projection needs a partials workspace");
     int rc;
     if (g.kind == MBX_GEOM_CONE)
-        rc = dispatch_back_1<MBX_GEOM_CONE>(g, pl, angles, sino, sino_dtype, weights, pix, npix, views, nviews,
-                                            coeff_power, kahan, scale, accumulate, out, part, st);
+        rc = dispatch_back_1<MBX_GEOM_CONE>(g, pl, T, sino, sino_dtype, weights, npix, views, nviews, coeff_power,
+                                            kahan, scale, accumulate, out, part, st);
     else
-        rc = dispatch_back_1<MBX_GEOM_PARALLEL>(g, pl, angles, sino, sino_dtype, weights, pix, npix, views, nviews,
-                                                coeff_power, kahan, scale, accumulate, out, part, st);
+        rc = dispatch_back_1<MBX_GEOM_PARALLEL>(g, pl, T, sino, sino_dtype, weights, npix, views, nviews, coeff_power,
+                                                kahan, scale, accumulate, out, part, st);
     if (rc != MBX_OK) return rc;
     MBX_LAUNCH_CHECK();
     if (pl.nsplit > 1) {
@@ -365,34 +502,61 @@ int launch_back(const DevGeom &g, const float *angles, const void *sino, int sin
     return MBX_OK;
 }
 
+int launch_back(const DevGeom &g, const float *angles, const void *sino, int sino_dtype, const float *weights,
+                const int32_t *pix, int npix, const int32_t *views, int nviews, int coeff_power, int kahan, float scale,
+                int accumulate, float *out, void *ws, hipStream_t st) {
+    if (npix == 0 || nviews == 0) return MBX_OK;
+    if (!ws) return fail(MBX_EWORKSPACE, "back projection needs a workspace");
+    const int chunk = table_chunk(g, npix, nviews);
+    char *w8 = (char *)ws;
+    float *T = (float *)w8;
+    int *NC = table_nc(T, chunk, nviews);
+    float *part = (float *)(w8 + table_bytes(chunk, nviews));
+    for (int c0 = 0; c0 < npix; c0 += chunk) {
+        const int n = std::min(chunk, npix - c0);
+        int rc = launch_table(g, angles, pix + c0, n, views, nviews, T, NC, st);
+        if (rc != MBX_OK) return rc;
+        rc = launch_back_tab(g, T, n, sino, sino_dtype, weights, views, nviews, coeff_power, kahan, scale, accumulate,
+                             out + (size_t)c0 * g.Nz, part, st);
+        if (rc != MBX_OK) return rc;
+    }
+    return MBX_OK;
+}
+
 // ======================================================================
 // Forward projection
 // ======================================================================
 
-constexpr int kFwdWaves = 4;
-constexpr int kFwdRowsPerBlock = kFwdWaves * kWave;
+constexpr int kFwdBatch = 8;
 
 FwdPlan plan_fwd(const DevGeom &g, int nviews) {
     FwdPlan p;
-    p.cb = g.C >= 32 ? 32 : (g.C >= 16 ? 16 : (g.C >= 8 ? 8 : 4));
-    p.ncb = (g.C + p.cb - 1) / p.cb;
-    p.rows_pb = kFwdRowsPerBlock;
+    const int rows = std::min(g.R, 512);
+    p.nw = 4;                                                       // waves per block
+    p.rpw = kWave * ((rows + p.nw * kWave - 1) / (p.nw * kWave));  // phase-2 rows per wave
+    p.rows_pb = std::min(p.nw * p.rpw, ((g.R + kWave - 1) / kWave) * kWave);
     p.nrb = (g.R + p.rows_pb - 1) / p.rows_pb;
-    p.list = 512;
+    p.cb = g.C >= 16 ? 16 : (g.C >= 8 ? 8 : 4);
+    p.ncb = (g.C + p.cb - 1) / p.cb;
+    p.list = 1024;
     const size_t tile = (size_t)p.cb * (p.rows_pb + 1) * sizeof(float);
-    const size_t list = (size_t)p.list * 5 * sizeof(float);
-    const size_t cyl = g.kind == MBX_GEOM_CONE ? (size_t)kFwdWaves * (g.Nz + 4) * sizeof(float) : 0;
+    const size_t cols = (size_t)kFwdBatch * p.rows_pb * sizeof(float) + kFwdBatch * 6 * sizeof(float);
+    const size_t list = (size_t)(p.list + kFwdBatch) * sizeof(int);
+    const size_t cyl = g.kind == MBX_GEOM_CONE ? (size_t)p.nw * (g.Nz + 4) * sizeof(float) : 0;
     const size_t scan = 16 * sizeof(int);
-    p.lds = tile + list + cyl + scan;
+    p.lds = tile + cols + list + cyl + scan;
     p.nblocks = (int64_t)nviews * p.ncb * p.nrb;
     return p;
 }
 
-int64_t fwd_workspace_bytes(const DevGeom &g, int nviews) { return plan_fwd(g, nviews).nblocks * (int64_t)sizeof(double); }
+int64_t fwd_workspace_bytes(const DevGeom &g, int npix, int nviews) {
+    if (npix == 0 || nviews == 0) return 0;
+    const int chunk = table_chunk(g, npix, nviews);
+    return table_bytes(chunk, nviews) + ((plan_fwd(g, nviews).nblocks * (int64_t)sizeof(double) + 255) & ~255ll);
+}
 
 template <typename T>
-__device__ __forceinline__ T block_sum_256(T v, T *scratch) {
-    // wave reduction with shuffles, then across the 4 waves through LDS
+__device__ __forceinline__ T block_sum(T v, T *scratch) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     __syncthreads();
@@ -404,59 +568,62 @@ __device__ __forceinline__ T block_sum_256(T v, T *scratch) {
     return r;  // valid in thread 0
 }
 
+// One workgroup per (view, channel block, row block).  The subset pixels whose
+// channel taps meet the block are compacted in pixel order, then handled in
+// batches of kFwdBatch:
+//   phase 1: one wave per pixel resamples the pixel's cylinder onto the block's
+//            detector rows (row-driven vertical fan, mbirjax/cone_beam.py:347-372)
+//            into an LDS column;
+//   phase 2: wave w adds the horizontal taps (mbirjax/cone_beam.py:276-282) of
+//            every column of the batch, in batch order, to its own rows of the
+//            LDS tile -- one writer per tile element, fixed order.
+// The tile is written to HBM once.
 template <int GEOM, int RAD, int RB, typename OT, int LAYOUT>
-__global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__restrict__ angles,
-                                                  const float *__restrict__ vox, const int32_t *__restrict__ pix,
-                                                  int npix, const int32_t *__restrict__ views, int cb, int ncb,
-                                                  int nrb, int list_cap, OT *__restrict__ sino, int accumulate,
-                                                  const float *__restrict__ w, double *__restrict__ part_sumsq) {
+__global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__restrict__ T, const int *__restrict__ NC,
+                                                  const float *__restrict__ vox, int npix, int cb, int ncb, int nrb,
+                                                  int rows_pb, int rpw, int list_cap, OT *__restrict__ sino,
+                                                  int accumulate, const float *__restrict__ w,
+                                                  double *__restrict__ part_sumsq) {
     extern __shared__ float lds[];
-    const int rows_pb = kFwdRowsPerBlock;
     const int tstride = rows_pb + 1;
-    float *tile = lds;
-    int *l_s = (int *)(tile + (size_t)cb * tstride);
-    float *l_mag = (float *)(l_s + list_cap);
-    float *l_np = l_mag + list_cap;
-    float *l_cosa = l_np + list_cap;
-    float *l_ctp = l_cosa + list_cap;
-    float *cyl_all = l_ctp + list_cap;
-    int *scan = (int *)(cyl_all + (GEOM == MBX_GEOM_CONE ? kFwdWaves * (g.Nz + 4) : 0));
+    const int nw = blockDim.x >> 6;
+    float *tile = lds;                                        // [cb][rows_pb + 1]
+    float *colbuf = tile + (size_t)cb * tstride;              // [kFwdBatch][rows_pb]
+    float *slot_a = colbuf + (size_t)kFwdBatch * rows_pb;     // [kFwdBatch][2*RAD+1]
+    int *slot_nc = (int *)(slot_a + kFwdBatch * (2 * RAD + 1));  // [kFwdBatch]
+    int *l_s = slot_nc + kFwdBatch;                           // [list_cap]
+    float *cyl_all = (float *)(l_s + list_cap);               // [nw][Nz + 4] (cone)
+    int *scan = (int *)(cyl_all + (GEOM == MBX_GEOM_CONE ? nw * (g.Nz + 4) : 0));
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const int64_t b = blockIdx.x;
-    const int vi = (int)(b / ((int64_t)ncb * nrb));
-    const int rem = (int)(b % ((int64_t)ncb * nrb));
-    const int cbi = rem / nrb, rbi = rem % nrb;
+    const int b = blockIdx.x;
+    const int vi = b / (ncb * nrb);
+    const int rem = b - vi * (ncb * nrb);
+    const int cbi = rem / nrb, rbi = rem - (rem / nrb) * nrb;
     const int n0 = cbi * cb, n1 = min(g.C, n0 + cb);
     const int m0 = rbi * rows_pb, m1 = min(g.R, m0 + rows_pb);
-    const int va = views ? views[vi] : vi;
-    float sn, cs;
-    sincosf(angles[va], &sn, &cs);
+    const int nrows = m1 - m0;
+    const int nri = (nrows + kWave - 1) / kWave;  // row iterations of a whole-block column
 
     for (int i = tid; i < cb * tstride; i += blockDim.x) tile[i] = 0.0f;
     float *cyl = cyl_all + wave * (g.Nz + 4);
-    const int wm0 = m0 + wave * kWave;
-    const int wm1 = min(m1, wm0 + kWave);
-    const int m = wm0 + lane;
-    const bool live = m < wm1;
+    const int wm0 = m0 + wave * rpw;           // rows owned in phase 2
+    const int wm1 = min(m1, wm0 + rpw);
+    const int *ncv = NC + (size_t)vi * npix;
+    const float *Tv = T + (size_t)vi * npix * kRec;
     __syncthreads();
 
     int count = 0;
     for (int base = 0; base < npix; base += blockDim.x) {
         const int s = base + tid;
         bool flag = false;
-        PVBase pb;
-        pb.mag = pb.np = pb.cosa = pb.ctp = 0.0f;
         if (s < npix) {
-            pb = pv_base<GEOM>(g, pix[s], cs, sn);
-            const int nc = (int)rintf(pb.np);
+            const int nc = ncv[s];
             flag = (nc + RAD >= n0) && (nc - RAD <= n1 - 1);
-            if (GEOM == MBX_GEOM_CONE && flag) {
-                const PV q = pv_full<GEOM>(g, pb);
-                const float ta = q.a;
-                const float tb = fmaf(q.W, (float)(g.Nz - 1), q.a);
-                flag = ((int)rintf(tb) + RAD + 1 >= m0) && ((int)rintf(ta) - RAD - 1 <= m1 - 1);
+            if (GEOM == MBX_GEOM_CONE && flag && nrb > 1) {
+                const float *rec = Tv + (size_t)s * kRec;
+                flag = (__float_as_int(rec[13]) <= m1 - 1) && (__float_as_int(rec[14]) >= m0);
             }
         }
         const unsigned long long bal = __ballot(flag);
@@ -465,92 +632,108 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
         if (lane == 0) scan[wave] = wcount;
         __syncthreads();
         int woff = 0, total = 0;
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+        for (int i = 0; i < nw; ++i) {
             const int c = scan[i];
             if (i < wave) woff += c;
             total += c;
         }
-        if (flag) {
-            const int idx = count + woff + lpre;
-            l_s[idx] = s;
-            l_mag[idx] = pb.mag;
-            l_np[idx] = pb.np;
-            l_cosa[idx] = pb.cosa;
-            l_ctp[idx] = pb.ctp;
-        }
+        if (flag) l_s[count + woff + lpre] = s;
         count += total;
         __syncthreads();
-        if (count > list_cap - (int)blockDim.x || base + (int)blockDim.x >= npix) {
-            // ---- accumulate the listed pixels into this wave's 64 rows
-            if (wm0 < wm1) {
-                for (int c = 0; c < count; ++c) {
-                    PVBase pbc;
-                    pbc.mag = l_mag[c];
-                    pbc.np = l_np[c];
-                    pbc.cosa = l_cosa[c];
-                    pbc.ctp = l_ctp[c];
-                    const int sc = l_s[c];
-                    const PV q = pv_full<GEOM>(g, pbc);
-                    const int nc = (int)rintf(q.np);
-                    float colv = 0.0f;
-                    if (GEOM == MBX_GEOM_CONE) {
-                        // slices whose taps can reach rows [wm0, wm1)
-                        int kf = (int)rintf(((float)wm0 - q.a) * q.invW) - RB;
-                        int kl = (int)rintf(((float)(wm1 - 1) - q.a) * q.invW) + RB;
-                        kf = max(kf, 0);
-                        kl = min(kl, g.Nz - 1);
-                        if (kf > kl) continue;  // wave-uniform
-                        const int span = kl - kf + 1;
-                        const float *xs = vox + (size_t)sc * g.Nz;
-                        for (int kk = lane; kk < span; kk += kWave) {
-                            const int k = kf + kk;
-                            const float u = fmaf(q.c1, (float)k, q.c0);
-                            cyl[1 + kk] = xs[k] * __builtin_amdgcn_sqrtf(fmaf(u, u, 1.0f));
+        if (!(count > list_cap - (int)blockDim.x || base + (int)blockDim.x >= npix)) continue;
+
+        for (int b0 = 0; b0 < count; b0 += kFwdBatch) {
+            const int nb = min(kFwdBatch, count - b0);
+            // ---- phase 1: columns of the batch, one wave per pixel
+            for (int slot = wave; slot < nb; slot += nw) {
+                const int sc = __builtin_amdgcn_readfirstlane(l_s[b0 + slot]);
+                const float *rec = Tv + (size_t)sc * kRec;
+                float *colv = colbuf + slot * rows_pb;
+                if (lane == 0) slot_nc[slot] = __float_as_int(rec[0]);
+                if (lane <= 2 * RAD) slot_a[slot * (2 * RAD + 1) + lane] = rec[3 - RAD + lane];
+                if (GEOM == MBX_GEOM_CONE) {
+                    const float a = rec[6], W = rec[7], hv = rec[8], Lv = rec[9], invW = rec[10], c0 = rec[11],
+                                c1 = rec[12];
+                    const int rlo = max(m0, __float_as_int(rec[13]));
+                    const int rhi = min(m1 - 1, __float_as_int(rec[14]));
+                    int kf = (int)rintf(((float)rlo - a) * invW) - RB;
+                    int kl = (int)rintf(((float)rhi - a) * invW) + RB;
+                    kf = max(kf, 0);
+                    kl = min(kl, g.Nz - 1);
+                    if (rlo > rhi || kf > kl) {  // wave-uniform: no row of this block is reached
+                        for (int i = 0; i < nri; ++i) {
+                            const int r = lane + kWave * i;
+                            if (r < nrows) colv[r] = 0.0f;
                         }
-                        if (lane == 0) {
-                            cyl[0] = 0.0f;
-                            cyl[span + 1] = 0.0f;
-                        }
-                        wave_sync();
-                        if (live) {
-                            const float kc = rintf(((float)m - q.a) * q.invW);
-                            const float dm = fmaf(q.W, kc, q.a) - (float)m;
-                            const int bidx = (int)kc - kf + 1;
-#pragma unroll
-                            for (int j = -RB; j <= RB; ++j) {
-                                const float wt = row_weight(q, fmaf(q.W, (float)j, dm));
-                                const int idx = min(max(bidx + j, 0), span + 1);
-                                colv = fmaf(wt, cyl[idx], colv);
-                            }
-                        }
-                        wave_sync();
-                    } else {
-                        if (live) colv = vox[(size_t)sc * g.Nz + m];
+                        continue;
                     }
-                    if (live) {
+                    const int span = kl - kf + 1;
+                    const float *xs = vox + (size_t)sc * g.Nz;
+                    for (int kk = lane; kk < span; kk += kWave) {
+                        const float k = (float)(kf + kk);
+                        const float u = fmaf(c1, k, c0);
+                        cyl[1 + kk] = xs[kf + kk] * __builtin_amdgcn_sqrtf(fmaf(u, u, 1.0f));
+                    }
+                    if (lane == 0) {
+                        cyl[0] = 0.0f;
+                        cyl[span + 1] = 0.0f;
+                    }
+                    wave_sync();
+                    const int base0 = 1 - kf;
+                    const float ainv = a * invW;
+                    for (int i = 0; i < nri; ++i) {
+                        const int r = lane + kWave * i;
+                        const float mf = (float)(m0 + r);
+                        const float kc = rintf(fmaf(mf, invW, -ainv));
+                        const float dm = fmaf(W, kc, a) - mf;  // t_kc - m
+                        const int bidx = (int)kc + base0;
+                        float acc = 0.0f;
 #pragma unroll
-                        for (int j = -RAD; j <= RAD; ++j) {
-                            const int n = nc + j;
-                            if (n >= n0 && n < n1) {
-                                const float a = chan_weight(q, n, g.C);
-                                atomicAdd(&tile[(n - n0) * tstride + (m - m0)], a * colv);
-                            }
+                        for (int j = -RB; j <= RB; ++j) {
+                            const float wt = __builtin_amdgcn_fmed3f(hv - fabsf(fmaf(W, (float)j, dm)), 0.0f, Lv);
+                            const int idx = min(max(bidx + j, 0), span + 1);
+                            acc = fmaf(wt, cyl[idx], acc);
+                        }
+                        if (r < nrows) colv[r] = acc;
+                    }
+                    wave_sync();
+                } else {
+                    const float *xs = vox + (size_t)sc * g.Nz;
+                    for (int i = 0; i < nri; ++i) {
+                        const int r = lane + kWave * i;
+                        if (r < nrows) colv[r] = xs[m0 + r];
+                    }
+                }
+            }
+            __syncthreads();
+            // ---- phase 2: horizontal taps into this wave's rows, batch order
+            if (wm0 < wm1) {
+                for (int slot = 0; slot < nb; ++slot) {
+                    const int nc = slot_nc[slot];
+                    const float *colv = colbuf + slot * rows_pb + (wm0 - m0);
+#pragma unroll
+                    for (int j = -RAD; j <= RAD; ++j) {
+                        const int n = nc + j;
+                        if (n >= n0 && n < n1) {
+                            const float aw = slot_a[slot * (2 * RAD + 1) + j + RAD];
+                            float *trow = tile + (n - n0) * tstride + (wm0 - m0);
+                            for (int r = lane; r < wm1 - wm0; r += kWave) trow[r] += aw * colv[r];
                         }
                     }
                 }
             }
-            count = 0;
             __syncthreads();
         }
+        count = 0;
     }
     __syncthreads();
 
     // ---- write the tile once (optionally adding to what is there)
-    const int ncol = n1 - n0, nrow = m1 - m0;
+    const int ncol = n1 - n0;
     double ss = 0.0;
     if (LAYOUT == MBX_LAYOUT_VCR) {
         for (int nn = 0; nn < ncol; ++nn) {
-            for (int mm = tid; mm < nrow; mm += blockDim.x) {
+            for (int mm = tid; mm < nrows; mm += blockDim.x) {
                 const size_t o = ((size_t)vi * g.C + (n0 + nn)) * (size_t)g.R + (m0 + mm);
                 float v = tile[nn * tstride + mm];
                 if (accumulate) v += to_float(sino[o]);
@@ -563,8 +746,8 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
             }
         }
     } else {
-        for (int idx = tid; idx < nrow * cb; idx += blockDim.x) {
-            const int mm = idx / cb, nn = idx % cb;
+        for (int idx = tid; idx < nrows * cb; idx += blockDim.x) {
+            const int mm = idx / cb, nn = idx - (idx / cb) * cb;
             if (nn < ncol) {
                 const size_t o = ((size_t)vi * g.R + (m0 + mm)) * (size_t)g.C + (n0 + nn);
                 float v = tile[nn * tstride + mm];
@@ -580,38 +763,74 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
     }
     if (part_sumsq) {
         double *dscratch = (double *)(scan + 8);
-        const double r = block_sum_256<double>(ss, dscratch);
+        const double r = block_sum<double>(ss, dscratch);
         if (tid == 0) part_sumsq[b] = r;
     }
 }
 
 template <int GEOM, int RAD, int RB, typename OT, int LAYOUT>
-static void launch_fwd_t(const DevGeom &g, const FwdPlan &pl, const float *angles, const float *vox,
-                         const int32_t *pix, int npix, const int32_t *views, OT *sino, int accumulate, const float *w,
-                         double *part, hipStream_t st) {
-    hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, OT, LAYOUT>), dim3((unsigned)pl.nblocks), dim3(256), pl.lds, st, g,
-                       angles, vox, pix, npix, views, pl.cb, pl.ncb, pl.nrb, pl.list, sino, accumulate, w, part);
+static void launch_fwd_t(const DevGeom &g, const FwdPlan &pl, const float *T, const int *NC, const float *vox,
+                         int npix, OT *sino, int accumulate, const float *w, double *part, hipStream_t st) {
+    hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, OT, LAYOUT>), dim3((unsigned)pl.nblocks), dim3(64 * pl.nw), pl.lds,
+                       st, g, T, NC, vox, npix, pl.cb, pl.ncb, pl.nrb, pl.rows_pb, pl.rpw, pl.list, sino, accumulate, w,
+                       part);
 }
 
 template <int GEOM, int RAD, int RB>
-static void dispatch_fwd_2(const DevGeom &g, const FwdPlan &pl, const float *angles, const float *vox,
-                           const int32_t *pix, int npix, const int32_t *views, void *sino, int dt, int layout,
-                           int accumulate, const float *w, double *part, hipStream_t st) {
+static void dispatch_fwd_2(const DevGeom &g, const FwdPlan &pl, const float *T, const int *NC, const float *vox,
+                           int npix, void *sino, int dt, int layout, int accumulate, const float *w, double *part,
+                           hipStream_t st) {
     if (dt == MBX_DTYPE_F32) {
         if (layout == MBX_LAYOUT_VCR)
-            launch_fwd_t<GEOM, RAD, RB, float, MBX_LAYOUT_VCR>(g, pl, angles, vox, pix, npix, views, (float *)sino,
-                                                               accumulate, w, part, st);
+            launch_fwd_t<GEOM, RAD, RB, float, MBX_LAYOUT_VCR>(g, pl, T, NC, vox, npix, (float *)sino, accumulate, w,
+                                                               part, st);
         else
-            launch_fwd_t<GEOM, RAD, RB, float, MBX_LAYOUT_VRC>(g, pl, angles, vox, pix, npix, views, (float *)sino,
-                                                               accumulate, w, part, st);
+            launch_fwd_t<GEOM, RAD, RB, float, MBX_LAYOUT_VRC>(g, pl, T, NC, vox, npix, (float *)sino, accumulate, w,
+                                                               part, st);
     } else {
         if (layout == MBX_LAYOUT_VCR)
-            launch_fwd_t<GEOM, RAD, RB, bf16, MBX_LAYOUT_VCR>(g, pl, angles, vox, pix, npix, views, (bf16 *)sino,
-                                                              accumulate, w, part, st);
+            launch_fwd_t<GEOM, RAD, RB, bf16, MBX_LAYOUT_VCR>(g, pl, T, NC, vox, npix, (bf16 *)sino, accumulate, w,
+                                                              part, st);
         else
-            launch_fwd_t<GEOM, RAD, RB, bf16, MBX_LAYOUT_VRC>(g, pl, angles, vox, pix, npix, views, (bf16 *)sino,
-                                                              accumulate, w, part, st);
+            launch_fwd_t<GEOM, RAD, RB, bf16, MBX_LAYOUT_VRC>(g, pl, T, NC, vox, npix, (bf16 *)sino, accumulate, w,
+                                                              part, st);
     }
+}
+
+// Forward projection of pixels whose table is built; part (nblocks doubles) and
+// sumsq may be NULL.
+int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float *vox, int npix, int nviews, void *sino,
+                   int sino_dtype, int layout, int accumulate, const float *weights, double *part, double *sumsq,
+                   hipStream_t st) {
+    const FwdPlan pl = plan_fwd(g, nviews);
+    if (pl.lds > 160 * 1024)
+        return fail(MBX_EUNSUPPORTED, "forward projection LDS footprint too large (num_recon_slices too big)");
+    if (sumsq && !part) return fail(MBX_EWORKSPACE, "forward sumsq needs a partials workspace");
+    double *p = sumsq ? part : nullptr;
+    if (g.kind == MBX_GEOM_CONE) {
+        if (g.rh == 1 && g.rb == 1)
+            dispatch_fwd_2<MBX_GEOM_CONE, 1, 1>(g, pl, T, NC, vox, npix, sino, sino_dtype, layout, accumulate, weights,
+                                                p, st);
+        else if (g.rh == 1)
+            dispatch_fwd_2<MBX_GEOM_CONE, 1, 2>(g, pl, T, NC, vox, npix, sino, sino_dtype, layout, accumulate, weights,
+                                                p, st);
+        else if (g.rb == 1)
+            dispatch_fwd_2<MBX_GEOM_CONE, 2, 1>(g, pl, T, NC, vox, npix, sino, sino_dtype, layout, accumulate, weights,
+                                                p, st);
+        else
+            dispatch_fwd_2<MBX_GEOM_CONE, 2, 2>(g, pl, T, NC, vox, npix, sino, sino_dtype, layout, accumulate, weights,
+                                                p, st);
+    } else {
+        if (g.rh == 1)
+            dispatch_fwd_2<MBX_GEOM_PARALLEL, 1, 1>(g, pl, T, NC, vox, npix, sino, sino_dtype, layout, accumulate,
+                                                    weights, p, st);
+        else
+            dispatch_fwd_2<MBX_GEOM_PARALLEL, 2, 1>(g, pl, T, NC, vox, npix, sino, sino_dtype, layout, accumulate,
+                                                    weights, p, st);
+    }
+    MBX_LAUNCH_CHECK();
+    if (sumsq) return launch_sum_partials(part, pl.nblocks, sumsq, 0, st);
+    return MBX_OK;
 }
 
 int launch_fwd(const DevGeom &g, const float *angles, const float *vox, const int32_t *pix, int npix,
@@ -621,37 +840,26 @@ int launch_fwd(const DevGeom &g, const float *angles, const float *vox, const in
         if (sumsq) MBX_HIP_CHECK(hipMemsetAsync(sumsq, 0, sizeof(double), st));
         return MBX_OK;
     }
-    const FwdPlan pl = plan_fwd(g, nviews);
-    if (pl.lds > 160 * 1024)
-        return fail(MBX_EUNSUPPORTED, "forward projection LDS footprint too large (num_recon_slices too big)");
-    double *part = nullptr;
-    if (sumsq) {
-        if (!ws) return fail(MBX_EWORKSPACE, "forward projection with sumsq needs a workspace");
-        part = (double *)ws;
+    if (!ws) return fail(MBX_EWORKSPACE, "forward projection needs a workspace");
+    const int chunk = table_chunk(g, std::max(npix, 1), nviews);
+    char *w8 = (char *)ws;
+    float *T = (float *)w8;
+    int *NC = table_nc(T, chunk, nviews);
+    double *part = (double *)(w8 + table_bytes(chunk, nviews));
+    if (npix == 0) {
+        // nothing to project: the kernel still writes (zeros or the unchanged sinogram)
+        return launch_fwd_tab(g, T, NC, vox, 0, nviews, sino, sino_dtype, layout, accumulate, weights, part, sumsq,
+                              st);
     }
-    if (g.kind == MBX_GEOM_CONE) {
-        if (g.rh == 1 && g.rb == 1)
-            dispatch_fwd_2<MBX_GEOM_CONE, 1, 1>(g, pl, angles, vox, pix, npix, views, sino, sino_dtype, layout,
-                                                accumulate, weights, part, st);
-        else if (g.rh == 1)
-            dispatch_fwd_2<MBX_GEOM_CONE, 1, 2>(g, pl, angles, vox, pix, npix, views, sino, sino_dtype, layout,
-                                                accumulate, weights, part, st);
-        else if (g.rb == 1)
-            dispatch_fwd_2<MBX_GEOM_CONE, 2, 1>(g, pl, angles, vox, pix, npix, views, sino, sino_dtype, layout,
-                                                accumulate, weights, part, st);
-        else
-            dispatch_fwd_2<MBX_GEOM_CONE, 2, 2>(g, pl, angles, vox, pix, npix, views, sino, sino_dtype, layout,
-                                                accumulate, weights, part, st);
-    } else {
-        if (g.rh == 1)
-            dispatch_fwd_2<MBX_GEOM_PARALLEL, 1, 1>(g, pl, angles, vox, pix, npix, views, sino, sino_dtype, layout,
-                                                    accumulate, weights, part, st);
-        else
-            dispatch_fwd_2<MBX_GEOM_PARALLEL, 2, 1>(g, pl, angles, vox, pix, npix, views, sino, sino_dtype, layout,
-                                                    accumulate, weights, part, st);
+    for (int c0 = 0; c0 < npix; c0 += chunk) {
+        const int n = std::min(chunk, npix - c0);
+        const bool last = c0 + n >= npix;
+        int rc = launch_table(g, angles, pix + c0, n, views, nviews, T, NC, st);
+        if (rc != MBX_OK) return rc;
+        rc = launch_fwd_tab(g, T, NC, vox + (size_t)c0 * g.Nz, n, nviews, sino, sino_dtype, layout,
+                            c0 == 0 ? accumulate : 1, weights, part, last ? sumsq : nullptr, st);
+        if (rc != MBX_OK) return rc;
     }
-    MBX_LAUNCH_CHECK();
-    if (sumsq) return launch_sum_partials(part, pl.nblocks, sumsq, 0, st);
     return MBX_OK;
 }
 
@@ -716,8 +924,7 @@ int64_t mbx_back_workspace_bytes(const mbx_geom *g, int32_t npix, int32_t nviews
 int64_t mbx_forward_workspace_bytes(const mbx_geom *g, int32_t npix, int32_t nviews) {
     DevGeom d;
     if (make_dev_geom(g, &d) != MBX_OK || npix < 0 || nviews < 0) return -1;
-    (void)npix;
-    return fwd_workspace_bytes(d, nviews);
+    return fwd_workspace_bytes(d, npix, nviews);
 }
 
 int mbx_forward_project(const mbx_geom *g, const float *angles, const float *vox, const int32_t *pix, int32_t npix,
@@ -735,9 +942,9 @@ int mbx_forward_project(const mbx_geom *g, const float *angles, const float *vox
         return fail(MBX_EINVAL, "unknown sinogram layout");
     if (!angles || !sino || (npix > 0 && (!vox || !pix)))
         return fail(MBX_EINVAL, "NULL device pointer (angles, sino, vox or pix)");
-    if (sumsq && workspace_bytes < fwd_workspace_bytes(d, nviews))
-        return fail(MBX_EWORKSPACE, "forward workspace too small: need " +
-                                        std::to_string(fwd_workspace_bytes(d, nviews)) + " bytes");
+    const int64_t need = fwd_workspace_bytes(d, npix, nviews);
+    if (workspace_bytes < need || (need > 0 && !workspace))
+        return fail(MBX_EWORKSPACE, "forward workspace too small: need " + std::to_string(need) + " bytes");
     return launch_fwd(d, angles, vox, pix, npix, views, nviews, sino, sino_dtype, sino_layout, accumulate, weights,
                       sumsq, workspace, (hipStream_t)stream);
 }

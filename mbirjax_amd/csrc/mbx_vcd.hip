@@ -308,17 +308,24 @@ __global__ __launch_bounds__(256) void axpy_scal_kernel(TE *__restrict__ e, cons
 }
 
 struct VcdLayout {
-    size_t back, bp, pg, ph, delta, dsino, fwd, dpart, l1part, scal, total;
+    int chunk;  // pixels per geometry table
+    size_t table, bpart, fpart, bp, pg, ph, delta, dsino, dpart, l1part, scal, total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static VcdLayout vcd_layout(const DevGeom &g, int npix, int ddt) {
     VcdLayout L;
+    L.chunk = table_chunk(g, std::max(npix, 1), g.V);
+    const BackPlan bpl = plan_back(g, L.chunk, g.V);
     size_t off = 0;
     const size_t cyl = (size_t)npix * g.Nz * sizeof(float);
-    L.back = off;
-    off += align256((size_t)back_workspace_bytes(g, npix, g.V));
+    L.table = off;
+    off += align256((size_t)table_bytes(L.chunk, g.V));
+    L.bpart = off;
+    off += align256(bpl.nsplit > 1 ? (size_t)bpl.nsplit * L.chunk * g.Nz * sizeof(float) : 0);
+    L.fpart = off;
+    off += align256((size_t)plan_fwd(g, g.V).nblocks * sizeof(double));
     L.bp = off;
     off += align256(cyl);
     L.pg = off;
@@ -329,8 +336,6 @@ static VcdLayout vcd_layout(const DevGeom &g, int npix, int ddt) {
     off += align256(cyl);
     L.dsino = off;
     off += align256((size_t)g.V * g.C * g.R * dtype_size(ddt));
-    L.fwd = off;
-    off += align256((size_t)fwd_workspace_bytes(g, g.V));
     L.dpart = off;
     off += align256(3 * kRedBlocks * sizeof(double));
     L.l1part = off;
@@ -444,9 +449,36 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
     const int64_t nsino = (int64_t)d.V * d.C * d.R;
     const int nb = red_blocks(ncyl);
 
+    float *T = (float *)(ws + L.table);
+    int *NC = table_nc(T, L.chunk, d.V);
+    float *bpart = (float *)(ws + L.bpart);
+    double *fpart = (double *)(ws + L.fpart);
+    const bool one_table = npix <= L.chunk;  // build the table once for all projections
+
     // (c) back projection of the weighted error sinogram
-    rc = launch_back(d, angles, e, e_dtype, weights, pix, npix, nullptr, d.V, 1, kahan, 1.0f, 0, bp, ws + L.back, st);
-    if (rc != MBX_OK) return rc;
+    for (int c0 = 0; c0 < npix; c0 += L.chunk) {
+        const int n = std::min(L.chunk, npix - c0);
+        rc = launch_table(d, angles, pix + c0, n, nullptr, d.V, T, NC, st);
+        if (rc != MBX_OK) return rc;
+        rc = launch_back_tab(d, T, n, e, e_dtype, weights, nullptr, d.V, 1, kahan, 1.0f, 0, bp + (size_t)c0 * d.Nz,
+                             bpart, st);
+        if (rc != MBX_OK) return rc;
+    }
+    // forward projection of delta over the subset (into dsino), optionally with forward_quadratic
+    auto project_delta = [&](double *sumsq, const float *w) -> int {
+        for (int c0 = 0; c0 < npix; c0 += L.chunk) {
+            const int n = std::min(L.chunk, npix - c0);
+            const bool last = c0 + n >= npix;
+            if (!one_table) {
+                int r = launch_table(d, angles, pix + c0, n, nullptr, d.V, T, NC, st);
+                if (r != MBX_OK) return r;
+            }
+            int r = launch_fwd_tab(d, T, NC, delta + (size_t)c0 * d.Nz, n, d.V, dsino, delta_dtype, MBX_LAYOUT_VCR,
+                                   c0 > 0, w, fpart, last ? sumsq : nullptr, st);
+            if (r != MBX_OK) return r;
+        }
+        return MBX_OK;
+    };
     // (a) prior gradient and Hessian
     float ph_const = 0.0f;
     const float *php = ph;
@@ -465,8 +497,7 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
                        (float)fm_constant, delta, dpart);
     MBX_LAUNCH_CHECK();
     // (g,h) delta sinogram and forward_quadratic
-    rc = launch_fwd(d, angles, delta, pix, npix, nullptr, d.V, dsino, delta_dtype, MBX_LAYOUT_VCR, 0, weights,
-                    scal + 0, ws + L.fwd, st);
+    rc = project_delta(scal + 0, weights);
     if (rc != MBX_OK) return rc;
     // (i) step size on the device
     hipLaunchKernelGGL(alpha_kernel, dim3(1), dim3(256), 0, st, dpart, nb, fm_constant, scal, stats);
@@ -475,8 +506,7 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
     if (positivity) {
         hipLaunchKernelGGL(positivity_kernel, dim3(nb), dim3(256), 0, st, flat_recon, pix, npix, d.Nz, scal, delta);
         MBX_LAUNCH_CHECK();
-        rc = launch_fwd(d, angles, delta, pix, npix, nullptr, d.V, dsino, delta_dtype, MBX_LAYOUT_VCR, 0, nullptr,
-                        nullptr, nullptr, st);
+        rc = project_delta(nullptr, nullptr);
         if (rc != MBX_OK) return rc;
     }
     // (k,m) recon update and ell1

@@ -187,13 +187,9 @@ class TomographyModel(ParameterHandler):
         if out is None:
             shape = (nv, C, R) if layout == _lib.LAYOUT_VCR else (nv, R, C)
             out = torch.empty(shape, dtype=_torch_dtype(out_dtype), device=vox.device)
-        sumsq = None
-        ws = None
-        wsb = 0
-        if want_sumsq:
-            sumsq = torch.zeros(1, dtype=torch.float64, device=vox.device)
-            wsb = int(_lib.lib().mbx_forward_workspace_bytes(self._geom, int(pix.numel()), nv))
-            ws = torch.empty(max(wsb, 8), dtype=torch.uint8, device=vox.device)
+        sumsq = torch.zeros(1, dtype=torch.float64, device=vox.device) if want_sumsq else None
+        wsb = int(_lib.lib().mbx_forward_workspace_bytes(self._geom, int(pix.numel()), nv))
+        ws = torch.empty(max(wsb, 256), dtype=torch.uint8, device=vox.device)
         _lib.check(_lib.lib().mbx_forward_project(
             self._geom, _lib.ptr(self._angles()), _lib.ptr(vox), _lib.ptr(pix), int(pix.numel()), _lib.ptr(views),
             nv, _lib.ptr(out), out_dtype, layout, int(accumulate), _lib.ptr(weights), _lib.ptr(sumsq),
@@ -418,23 +414,26 @@ class TomographyModel(ParameterHandler):
         return (_qggmrf.get_b_from_nbr_wts(nbr), float(sigma_x), float(p), float(q), float(T))
 
     def vcd_setup(self, sinogram, weights=None, init_recon=None, prox_input=None, precision='fp32',
-                  max_subset_size=None):
+                  max_subset_size=None, channel_major=False):
         """Initial VCD state on the device: e = y - alpha0 A x0 with the optimal
         scale alpha0 of the initial recon, the forward-model Hessian diagonal and
         the subset-update workspace (mbirjax/tomography_model.py:1110-1192).
-        sinogram may already be a channel-major device tensor (views, C, R)."""
+        With channel_major=True, sinogram (and weights) are already (views, C, R)
+        float32 device tensors."""
         if precision not in ('fp32', 'mixed'):
             raise ValueError("precision must be 'fp32' or 'mixed'")
         dev = self._device()
         recon_shape = self.get_params('recon_shape')
         rows, cols, nz = recon_shape
         V, R, C = self.get_params('sinogram_shape')
-        if isinstance(sinogram, torch.Tensor) and sinogram.is_cuda and tuple(sinogram.shape) == (V, C, R) and \
-                sinogram.dtype == torch.float32 and (R != C):
-            y = sinogram.contiguous()
+        if channel_major:
+            if tuple(sinogram.shape) != (V, C, R):
+                raise ValueError('channel-major sinogram must have shape {}'.format((V, C, R)))
+            y = self._tensor(sinogram)
+            w = None if weights is None else self._tensor(weights)
         else:
             y = self.to_channel_major(sinogram)
-        w = None if weights is None else self.to_channel_major(weights)
+            w = None if weights is None else self.to_channel_major(weights)
         if init_recon is None:
             self._log('info', 'Starting direct recon for initial reconstruction')
             init_recon = self.direct_recon(self.from_channel_major(y))
