@@ -1,0 +1,188 @@
+"""Host-side tests (no GPU): the C ABI library loads and exports every
+declared symbol, argument validation fails before any HIP call, and the
+TomographyModel host logic (parameters, geometry, partitions, regularisation)
+matches the oracle restatement."""
+import ctypes
+import os
+import re
+import warnings
+
+import numpy as np
+import pytest
+
+import mbirjax_amd as mj
+from mbirjax_amd import _lib
+from oracle import partitions as opart
+from oracle import projectors as op
+from oracle import vcd as ovcd
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "mbx.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char \*)\s*\*?\s*(mbx_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 13
+    L = _lib.lib()
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_lib.EXPORTS)
+    assert L.mbx_version() == 1
+
+
+def geom(**kw):
+    d = dict(kind=1, sinogram_shape=(8, 4, 6), recon_shape=(6, 6, 4), psf_radius=1, bp_psf_radius=1,
+             delta_det_row=1.0, delta_det_channel=1.0, det_row_offset=0.0, det_channel_offset=0.0,
+             source_detector_dist=24.0, magnification=2.0, delta_voxel=0.5, recon_slice_offset=0.0)
+    d.update(kw)
+    return _lib.np_geom_struct(**d)
+
+
+def test_validation_before_any_hip_call():
+    L = _lib.lib()
+    fake = ctypes.c_void_p(16)  # never dereferenced: every call below fails validation first
+    # NULL geometry
+    assert L.mbx_forward_project(None, fake, fake, fake, 1, None, 0, fake, 0, 0, 0, None, None, None, 0,
+                                 None) == _lib.MBX_EINVAL
+    assert b"NULL" in L.mbx_last_error()
+    # bad kind / radius / dims
+    assert L.mbx_forward_project(geom(kind=7), fake, fake, fake, 1, None, 0, fake, 0, 0, 0, None, None, None, 0,
+                                 None) == _lib.MBX_EINVAL
+    assert L.mbx_forward_project(geom(psf_radius=3), fake, fake, fake, 1, None, 0, fake, 0, 0, 0, None, None, None,
+                                 0, None) == _lib.MBX_EUNSUPPORTED
+    assert L.mbx_back_project(geom(sinogram_shape=(0, 4, 6)), fake, fake, 0, 8, None, fake, 1, None, 0, 1, 0, 1.0,
+                              0, fake, None, 0, None) == _lib.MBX_EINVAL
+    # parallel beam needs slices == rows
+    assert L.mbx_back_project(geom(kind=0, recon_shape=(6, 6, 5)), fake, fake, 0, 8, None, fake, 1, None, 0, 1, 0,
+                              1.0, 0, fake, None, 0, None) == _lib.MBX_EINVAL
+    # sinogram view count without view indices
+    assert L.mbx_back_project(geom(), fake, fake, 0, 7, None, fake, 1, None, 0, 1, 0, 1.0, 0, fake, None, 0,
+                              None) == _lib.MBX_EINVAL
+    # bad coeff power and dtype
+    assert L.mbx_back_project(geom(), fake, fake, 0, 8, None, fake, 1, None, 0, 3, 0, 1.0, 0, fake, None, 0,
+                              None) == _lib.MBX_EINVAL
+    assert L.mbx_back_project(geom(), fake, fake, 5, 8, None, fake, 1, None, 0, 1, 0, 1.0, 0, fake, None, 0,
+                              None) == _lib.MBX_EINVAL
+    # workspace too small
+    need = L.mbx_back_workspace_bytes(geom(), 1, 8)
+    assert need > 0
+    assert L.mbx_back_project(geom(), fake, fake, 0, 8, None, fake, 1, None, 0, 1, 0, 1.0, 0, fake, fake, need - 1,
+                              None) == _lib.MBX_EWORKSPACE
+    assert L.mbx_forward_project(geom(), fake, fake, fake, 1, None, 0, fake, 0, 0, 0, None, None, fake,
+                                 L.mbx_forward_workspace_bytes(geom(), 1, 8) - 1, None) == _lib.MBX_EWORKSPACE
+    # transpose aliasing, dot and axpy arguments
+    assert L.mbx_sino_transpose(fake, 0, 0, fake, 0, 1, 2, 3, 4, None) == _lib.MBX_EINVAL
+    assert L.mbx_dot(fake, 0, None, 0, None, -1, fake, fake, 8192, None) == _lib.MBX_EINVAL
+    assert L.mbx_dot(fake, 0, None, 0, None, 10, fake, fake, 8, None) == _lib.MBX_EWORKSPACE
+    assert L.mbx_sino_axpy(fake, 0, fake, 9, fake, 4, None) == _lib.MBX_EINVAL
+    # qGGMRF parameters
+    q = _lib.qggmrf_struct([1 / 6] * 6, -1.0, 2.0, 1.2, 1.0)
+    assert L.mbx_qggmrf_grad_hess(fake, 4, 4, 4, fake, 2, q, fake, fake, None) == _lib.MBX_EINVAL
+    # VCD update
+    q = _lib.qggmrf_struct([1 / 6] * 6, 1.0, 2.0, 1.2, 1.0)
+    assert L.mbx_vcd_subset_update(geom(), fake, fake, fake, 0, None, fake, fake, 0, 1.0, q, None, 1.0, 0, 0, 0,
+                                   None, fake, 1 << 30, None) == _lib.MBX_EINVAL
+    assert L.mbx_vcd_subset_update(geom(), fake, fake, fake, 0, None, fake, fake, 4, 1.0, q, None, 1.0, 0, 0, 0,
+                                   None, fake, 16, None) == _lib.MBX_EWORKSPACE
+    assert L.mbx_vcd_workspace_bytes(geom(), 4, 0) > L.mbx_vcd_workspace_bytes(geom(), 4, 1)
+
+
+def test_check_raises_with_message():
+    L = _lib.lib()
+    L.mbx_sino_transpose(None, 0, 0, None, 0, 0, 1, 1, 1, None)
+    with pytest.raises(ValueError, match="NULL"):
+        _lib.check(_lib.MBX_EINVAL)
+
+
+def test_compute_without_gpu_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    m = mj.ParallelBeamModel((8, 4, 6), np.linspace(0, np.pi, 8, endpoint=False))
+    with pytest.raises(RuntimeError, match="GPU"):
+        m.forward_project(np.zeros(m.get_params('recon_shape'), np.float32))
+
+
+# ---------------------------------------------------------------- parameters
+def test_set_params_semantics():
+    m = mj.ConeBeamModel((16, 8, 12), np.linspace(0, 2 * np.pi, 16, endpoint=False), source_detector_dist=48,
+                         source_iso_dist=24)
+    assert m.get_params('recon_shape') == (12, 12, 8)
+    with pytest.raises(ValueError):
+        m.set_params(not_a_param=1)
+    with warnings.catch_warnings(record=True):
+        warnings.simplefilter("always")
+        m.set_params(sigma_y=2.0)
+    assert m.get_params('auto_regularize_flag') is False
+    with warnings.catch_warnings(record=True):
+        warnings.simplefilter("always")
+        m.set_params(sharpness=0.5)
+    assert m.get_params('auto_regularize_flag') is True
+    old = m._geom.num_recon_slices
+    m.set_params(recon_shape=(12, 12, 10))
+    assert m._geom.num_recon_slices == 10 and old == 8
+    m.set_params(sigma_x=np.float32(0.25), no_warning=True)
+    assert type(m.get_params('sigma_x')) is float
+    with pytest.raises(ValueError):
+        m.set_params(use_gpu='sometimes')
+
+
+def test_yaml_round_trip(tmp_path):
+    m = mj.ConeBeamModel((16, 8, 12), np.linspace(0, 2 * np.pi, 16, endpoint=False), source_detector_dist=48,
+                         source_iso_dist=24)
+    m.set_params(det_row_offset=0.25, sharpness=0.3)
+    f = str(tmp_path / "model.yaml")
+    m.to_file(f)
+    m2 = mj.ConeBeamModel.from_file(f)
+    assert mj.ParameterHandler.compare_parameter_handlers(m, m2)
+    p = mj.ParallelBeamModel((10, 6, 9), np.linspace(0, np.pi, 10, endpoint=False))
+    p2 = mj.ParallelBeamModel.from_file(p.to_file(None))
+    assert mj.ParameterHandler.compare_parameter_handlers(p, p2)
+
+
+@pytest.mark.parametrize("shape,sdd,sid", [((128, 128, 192), 768.0, 384.0), ((64, 40, 128), 512.0, 512.0),
+                                           ((720, 512, 768), 3072.0, 1536.0), ((13, 70, 45), 200.0, 90.0)])
+def test_geometry_matches_oracle(shape, sdd, sid):
+    ang = np.linspace(0, 2 * np.pi, shape[0], endpoint=False)
+    m = mj.ConeBeamModel(shape, ang, source_detector_dist=sdd, source_iso_dist=sid)
+    g = op.Geometry("cone", shape, ang, sdd, sid)
+    assert m.get_params('recon_shape') == g.recon_shape
+    gp = m.get_geometry_parameters()
+    assert (gp.psf_radius, gp.bp_psf_radius) == g.psf_radii()
+    assert gp.magnification == g.magnification()
+
+
+def test_partitions_match_oracle_rng_stream():
+    shape = (40, 40, 5)
+    np.random.seed(123)
+    a = [mj.gen_pixel_partition(shape, k) for k in (1, 4, 16, 64, 128)] + [mj.gen_full_indices(shape)]
+    perm_a = np.random.permutation(10)
+    np.random.seed(123)
+    b = [opart.gen_pixel_partition(shape, k) for k in (1, 4, 16, 64, 128)] + [opart.gen_full_indices(shape)]
+    perm_b = np.random.permutation(10)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    assert np.array_equal(perm_a, perm_b)
+    assert np.array_equal(mj.get_2d_ror_mask((33, 40)), opart.get_2d_ror_mask((33, 40)))
+    assert list(mj.gen_partition_sequence([0, 2, 4, 6, 7], 9)) == [0, 2, 4, 6, 7, 7, 7, 7, 7]
+    assert list(mj.gen_partition_sequence([0, 2, 4, 6, 7], 3)) == [0, 2, 4]
+
+
+def test_auto_regularization_matches_oracle():
+    rng = np.random.default_rng(4)
+    shape = (30, 8, 16)
+    sino = np.abs(rng.standard_normal(shape)).astype(np.float32)
+    sino[:, :, :3] = 0
+    w = rng.random(shape).astype(np.float32)
+    ang = np.linspace(0, 2 * np.pi, 30, endpoint=False)
+    for weights in (None, w):
+        m = mj.ConeBeamModel(shape, ang, source_detector_dist=64, source_iso_dist=32)
+        m.set_params(sharpness=0.5)
+        r = m.auto_set_regularization_params(sino, weights=weights)
+        om = ovcd.OracleModel(op.Geometry("cone", shape, ang, 64, 32), sharpness=0.5)
+        ro = om.auto_set_regularization_params(sino, weights=weights)
+        assert r == ro
