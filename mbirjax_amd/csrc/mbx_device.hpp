@@ -46,6 +46,59 @@ __device__ __forceinline__ bf16 from_float<bf16>(float x) {
     return r;
 }
 
+// Four consecutive elements through one 16-byte (float) or 8-byte (bf16) access.
+__device__ __forceinline__ void load4(const float *p, float v[4]) {
+    const float4 t = *(const float4 *)p;
+    v[0] = t.x;
+    v[1] = t.y;
+    v[2] = t.z;
+    v[3] = t.w;
+}
+__device__ __forceinline__ void load4(const bf16 *p, float v[4]) {
+    const uint2 t = *(const uint2 *)p;
+    v[0] = __uint_as_float(t.x << 16);
+    v[1] = __uint_as_float(t.x & 0xffff0000u);
+    v[2] = __uint_as_float(t.y << 16);
+    v[3] = __uint_as_float(t.y & 0xffff0000u);
+}
+__device__ __forceinline__ void store4(float *p, const float v[4]) { *(float4 *)p = make_float4(v[0], v[1], v[2], v[3]); }
+__device__ __forceinline__ void store4(bf16 *p, const float v[4]) {
+    uint2 t;
+    t.x = (uint32_t)from_float<bf16>(v[0]).bits | ((uint32_t)from_float<bf16>(v[1]).bits << 16);
+    t.y = (uint32_t)from_float<bf16>(v[2]).bits | ((uint32_t)from_float<bf16>(v[3]).bits << 16);
+    *(uint2 *)p = t;
+}
+
+// ---------------------------------------------------------------- buffer loads
+// Raw buffer loads through a wave-uniform resource (base address in SGPRs):
+// one 32-bit per-lane byte offset plus an SGPR byte offset, no 64-bit
+// per-lane address arithmetic.  Offsets stay below 4 GiB of the base.
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
+__device__ float raw_load_f32(i32x4_t rsrc, int voffset, int soffset, int aux) __asm("llvm.amdgcn.raw.buffer.load.f32");
+__device__ short raw_load_i16(i32x4_t rsrc, int voffset, int soffset, int aux) __asm("llvm.amdgcn.raw.buffer.load.i16");
+
+__device__ __forceinline__ i32x4_t make_rsrc(const void *p) {
+    const uint64_t a = (uint64_t)p;
+    i32x4_t r;
+    r.x = __builtin_amdgcn_readfirstlane((int)(uint32_t)a);
+    r.y = __builtin_amdgcn_readfirstlane((int)((uint32_t)(a >> 32) & 0xffffu));  // stride 0
+    r.z = (int)0xffffffffu;                                                        // num_records: no clamp
+    r.w = 0x00020000;                                                              // raw dword access (gfx9 family)
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ float buf_load(i32x4_t rsrc, int voffset, int soffset);
+template <>
+__device__ __forceinline__ float buf_load<float>(i32x4_t rsrc, int voffset, int soffset) {
+    return raw_load_f32(rsrc, voffset, soffset, 0);
+}
+template <>
+__device__ __forceinline__ float buf_load<bf16>(i32x4_t rsrc, int voffset, int soffset) {
+    const unsigned short b = (unsigned short)raw_load_i16(rsrc, voffset, soffset, 0);
+    return __uint_as_float(((uint32_t)b) << 16);
+}
+
 // ---------------------------------------------------------------- wave sync
 // LDS hand-off between lanes of one wavefront (rocPRIM's wave_barrier idiom).
 __device__ __forceinline__ void wave_sync() {

@@ -48,6 +48,9 @@ int make_dev_geom(const mbx_geom *g, DevGeom *d) {
         return fail(MBX_EINVAL, "all sinogram and recon dimensions must be positive");
     if ((int64_t)g->num_recon_rows * g->num_recon_cols > INT32_MAX)
         return fail(MBX_EINVAL, "num_recon_rows*num_recon_cols exceeds int32");
+    // The kernels address within one view with 32-bit byte offsets.
+    if ((int64_t)g->num_det_rows * g->num_det_channels * 4 > INT32_MAX)
+        return fail(MBX_EINVAL, "num_det_rows*num_det_channels*4 bytes per view exceeds int32");
     if (!(g->delta_det_channel > 0) || !(g->delta_voxel > 0) || !std::isfinite(g->delta_det_channel) ||
         !std::isfinite(g->delta_voxel))
         return fail(MBX_EINVAL, "delta_det_channel and delta_voxel must be positive and finite");
@@ -108,6 +111,7 @@ int make_dev_geom(const mbx_geom *g, DevGeom *d) {
 // plus n_c again in a dense (view, pixel) int array for the forward scan.
 
 constexpr int kRec = 16;
+constexpr int kFill = 4;  // back projection: 64-row groups loaded per round
 constexpr int64_t kTableBudget = 512ll << 20;
 
 int table_chunk(const DevGeom &g, int npix, int nviews) {
@@ -190,26 +194,29 @@ int launch_table(const DevGeom &g, const float *angles, const int32_t *pix, int 
 BackPlan plan_back(const DevGeom &g, int npix, int nviews) {
     BackPlan p;
     const int per_lane = (g.Nz + kWave - 1) / kWave;
-    p.kpl = per_lane <= 2 ? 2 : per_lane <= 4 ? 4 : per_lane <= 8 ? 8 : 16;
+    p.kpl = per_lane <= 4 ? 4 : per_lane <= 8 ? 8 : 16;
     p.nchunk = (g.Nz + kWave * p.kpl - 1) / (kWave * p.kpl);
-    const int64_t waves = (int64_t)npix * p.nchunk;
+    const int64_t waves = ((int64_t)npix + 1) / 2 * p.nchunk;
     // Small subsets: split the views over the 8 XCDs so that each XCD's L2
     // streams only its eighth of the sinogram; large passes are VALU-bound.
-    p.nsplit = (waves <= 32768 && nviews >= 8) ? 8 : 1;
-    // LDS detector-column capacity per wave: the rows one slice chunk can reach,
-    // from the largest pixel magnification in the recon disc (plus taps), at
-    // least the detector height + 2 (the clamped fallback layout).
-    p.cap = g.R + 4;
+    p.nsplit = (waves <= 16384 && nviews >= 8) ? 8 : 1;
+    // LDS detector-column capacity per pixel: the rows one slice chunk can
+    // reach, from the largest pixel magnification in the recon disc (plus
+    // taps), padded by one load round; at least the detector height + 2 plus a
+    // round (the clamped fallback layout).
+    const int round = kWave * kFill;
+    p.cap = g.R + 2 + round;
     if (g.kind == MBX_GEOM_CONE) {
         const double rmax = g.dv * std::sqrt((double)g.img_row_center * g.img_row_center +
                                              (double)g.img_col_center * g.img_col_center) + g.dv;
         const double den = (double)g.inv_M - rmax * (double)g.inv_sdd;
         if (den > 0) {
             const double wmax = g.dv_over_dr / den;
-            const double want = std::ceil((kWave * p.kpl - 1) * wmax) + 2.0 * g.rh + 8.0;
+            const double want = std::ceil((kWave * p.kpl - 1) * wmax) + 2.0 * g.rh + 8.0 + round;
             if (want < 16384.0) p.cap = std::max(p.cap, (int)want);
         }
     }
+    p.cap = (p.cap + 63) & ~63;
     return p;
 }
 
@@ -221,12 +228,184 @@ int64_t back_workspace_bytes(const DevGeom &g, int npix, int nviews) {
     return table_bytes(chunk, nviews) + ((part + 255) & ~255ll);
 }
 
-// One wavefront per (pixel, slice chunk, view group).  The record of each
-// (view, pixel) is wave-uniform (scalar loads); lanes gather the pixel's
-// channel columns row-by-row into an LDS detector column, then each lane
-// resamples the column onto its slices.
-template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN>
-__global__ __launch_bounds__(256) void back_kernel(DevGeom g, const float *__restrict__ T, const ET *__restrict__ sino,
+// One wavefront per (pixel pair, slice chunk, view group).  For every view the
+// wave reads the two pixels' records with scalar loads, gathers each pixel's
+// 2r+1 channel columns row-by-row into its LDS detector column (clamped,
+// always-valid addresses; rows off the detector are zeroed by a select), then
+// every lane resamples both columns onto its slices.  Two pixels per wave keep
+// all waves of a subset resident at once, so the waves of an XCD sweep the
+// views together and each view is fetched from HBM once.
+// The 2r+1 channel columns of one pixel at one view: a buffer resource on the
+// view (and on its weights), the byte offset of each column (channels off the
+// detector are clamped to a valid column and get weight 0).
+template <int RAD>
+struct ColumnSource {
+    i32x4_t rs, rw;
+    int soff[2 * RAD + 1];
+    bool cv[2 * RAD + 1];
+};
+
+// Vertical fan of one pixel at one view (mbirjax/cone_beam.py:480-541): every
+// lane resamples the LDS detector column onto slices k0 + lane + 64 i.
+template <int KPL, int RAD, int POW, bool KAHAN, bool TAIL>
+__device__ __forceinline__ void back_vertical(const float *col, int shift, bool fit, int n_rows, float a, float W,
+                                              float hv, float Lv, float c0, float c1, float kf0, float klast,
+                                              float (&acc)[KPL], float (&cmp)[KPL]) {
+    const float hm1 = hv - 1.0f;
+    const float t0 = fmaf(W, kf0, a), u0 = fmaf(c1, kf0, c0);
+    const float W64 = W * (float)kWave, c164 = c1 * (float)kWave;
+#pragma unroll
+    for (int i = 0; i < KPL; ++i) {
+        float t, u;
+        if (TAIL) {  // lanes past the last slice repeat it (their sums are never stored)
+            const float kf = fminf(kf0 + (float)(kWave * i), klast);
+            t = fmaf(W, kf, a);
+            u = fmaf(c1, kf, c0);
+        } else {
+            t = fmaf(W64, (float)i, t0);
+            u = fmaf(c164, (float)i, u0);
+        }
+        const float mc = rintf(t);
+        const float d = t - mc;            // |d| <= 1/2
+        const int base = (int)mc - shift;  // buffer index of row mc - RAD
+        float sum;
+        if (RAD == 1) {
+            float wm = __builtin_amdgcn_fmed3f(hm1 - d, 0.0f, Lv);        // row mc-1
+            float w0 = __builtin_amdgcn_fmed3f(hv - fabsf(d), 0.0f, Lv);  // row mc
+            float wp = __builtin_amdgcn_fmed3f(hm1 + d, 0.0f, Lv);        // row mc+1
+            if (POW == 2) {
+                w0 *= w0;
+                wp *= wp;
+                wm *= wm;
+            }
+            float cm, cc, cp;
+            if (fit) {
+                cm = col[base];
+                cc = col[base + 1];
+                cp = col[base + 2];
+            } else {
+                cm = col[min(max(base, 0), n_rows - 1)];
+                cc = col[min(max(base + 1, 0), n_rows - 1)];
+                cp = col[min(max(base + 2, 0), n_rows - 1)];
+            }
+            sum = w0 * cc;
+            sum = fmaf(wp, cp, sum);
+            sum = fmaf(wm, cm, sum);
+        } else {
+            sum = 0.0f;
+#pragma unroll
+            for (int j = 0; j <= 2 * RAD; ++j) {
+                float wj = __builtin_amdgcn_fmed3f(hv - fabsf(d - (float)(j - RAD)), 0.0f, Lv);
+                if (POW == 2) wj *= wj;
+                const int idx = fit ? base + j : min(max(base + j, 0), n_rows - 1);
+                sum = fmaf(wj, col[idx], sum);
+            }
+        }
+        const float s2 = fmaf(u, u, 1.0f);
+        const float f = (POW == 2) ? s2 : __builtin_amdgcn_sqrtf(s2);
+        if (KAHAN) {
+            const float y = sum * f - cmp[i];
+            const float tt = acc[i] + y;
+            cmp[i] = (tt - acc[i]) - y;
+            acc[i] = tt;
+        } else {
+            acc[i] = fmaf(sum, f, acc[i]);
+        }
+    }
+}
+
+template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN, bool WGT>
+__device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *__restrict__ rec,
+                                                const ET *__restrict__ sino, const float *__restrict__ w, int vd,
+                                                int k0, int kend, int cap, float *col, int lane, float kf0,
+                                                float (&acc)[KPL], float (&cmp)[KPL]) {
+    const int nc = __float_as_int(rec[0]);
+    float Aw[2 * RAD + 1];
+#pragma unroll
+    for (int j = 0; j <= 2 * RAD; ++j) {
+        const float a = rec[3 - RAD + j];
+        Aw[j] = (POW == 2) ? a * a : a;
+    }
+    ColumnSource<RAD> src;
+    const size_t vofs = (size_t)vd * g.C * g.R;
+    src.rs = make_rsrc(sino + vofs);
+    if (WGT) src.rw = make_rsrc(w + vofs);
+#pragma unroll
+    for (int j = 0; j <= 2 * RAD; ++j) {
+        const int n = nc - RAD + j;
+        src.cv[j] = (n >= 0) && (n < g.C);
+        src.soff[j] = (src.cv[j] ? n : 0) * g.R * (int)sizeof(ET);
+    }
+    if (GEOM == MBX_GEOM_CONE) {
+        const float a = rec[6], W = rec[7], hv = rec[8], Lv = rec[9], c0 = rec[11], c1 = rec[12];
+        const int mfirst = (int)rintf(fmaf(W, (float)k0, a)) - RAD;
+        const int mlast = (int)rintf(fmaf(W, (float)(kend - 1), a)) + RAD;
+        const int mlo = max(mfirst, 0), mhi = min(mlast, g.R - 1);
+        if (mlo > mhi) return;  // wave-uniform: this view misses the detector
+        // One zero row of margin on each side: a lane's slice position may
+        // round to one row beyond the chunk's first or last centre row.
+        const int count = mlast - mfirst + 3;
+        const bool fit = count + kWave * kFill <= cap;
+        const int mstart = fit ? mfirst - 1 : mlo - 1;
+        const int n_rows = fit ? count : mhi - mlo + 3;
+        // ---- detector column col[i] = row mstart + i (zero off the detector)
+        for (int i0 = 0; i0 < n_rows; i0 += kWave * kFill) {
+            float ev[kFill][2 * RAD + 1];
+            bool ok[kFill];
+#pragma unroll
+            for (int u = 0; u < kFill; ++u) {
+                const int r = mstart + i0 + u * kWave + lane;
+                ok[u] = (r >= mlo) && (r <= mhi);
+                const int rc = min(max(r, mlo), mhi);
+#pragma unroll
+                for (int j = 0; j <= 2 * RAD; ++j) {
+                    float e = buf_load<ET>(src.rs, rc * (int)sizeof(ET), src.soff[j]);
+                    if (WGT) e *= buf_load<float>(src.rw, rc * 4, src.soff[j] * (int)(4 / sizeof(ET)));
+                    ev[u][j] = e;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kFill; ++u) {
+                float c = 0.0f;
+#pragma unroll
+                for (int j = 0; j <= 2 * RAD; ++j) c = fmaf(src.cv[j] ? Aw[j] : 0.0f, ev[u][j], c);
+                col[i0 + u * kWave + lane] = ok[u] ? c : 0.0f;  // cap holds a whole round past the rows
+            }
+        }
+        wave_sync();
+        const int shift = RAD + mstart;
+        if (kend - k0 == kWave * KPL)  // full chunk: no tail lanes
+            back_vertical<KPL, RAD, POW, KAHAN, false>(col, shift, fit, n_rows, a, W, hv, Lv, c0, c1, kf0,
+                                                       (float)(kend - 1), acc, cmp);
+        else
+            back_vertical<KPL, RAD, POW, KAHAN, true>(col, shift, fit, n_rows, a, W, hv, Lv, c0, c1, kf0,
+                                                      (float)(kend - 1), acc, cmp);
+        wave_sync();
+    } else {
+#pragma unroll
+        for (int i = 0; i < KPL; ++i) {
+            const int k = min(k0 + lane + kWave * i, kend - 1);
+            float c = 0.0f;
+#pragma unroll
+            for (int j = 0; j <= 2 * RAD; ++j) {
+                float e = buf_load<ET>(src.rs, k * (int)sizeof(ET), src.soff[j]);
+                if (WGT) e *= buf_load<float>(src.rw, k * 4, src.soff[j] * (int)(4 / sizeof(ET)));
+                c = fmaf(src.cv[j] ? Aw[j] : 0.0f, e, c);
+            }
+            if (KAHAN) {
+                const float y = c - cmp[i];
+                const float tt = acc[i] + y;
+                cmp[i] = (tt - acc[i]) - y;
+                acc[i] = tt;
+            } else {
+                acc[i] += c;
+            }
+        }
+    }
+}
+
+template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN, bool WGT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void back_kernel(DevGeom g, const float *__restrict__ T, const ET *__restrict__ sino,
                                                    const float *__restrict__ w, int npix,
                                                    const int32_t *__restrict__ views, int nviews, int nsplit,
                                                    int nchunk, int cap, float scale, int accumulate,
@@ -237,160 +416,48 @@ __global__ __launch_bounds__(256) void back_kernel(DevGeom g, const float *__res
     const int split = blockIdx.x % nsplit;
     const int grp = blockIdx.x / nsplit;
     const int chunk = grp % nchunk;
-    const int s = __builtin_amdgcn_readfirstlane((grp / nchunk) * 4 + wave);
-    if (s >= npix) return;  // whole wavefront leaves; no block barriers below
+    const int q = __builtin_amdgcn_readfirstlane((grp / nchunk) * 4 + wave);
+    const int sA = 2 * q;
+    if (sA >= npix) return;  // whole wavefront leaves; no block barriers below
+    const bool hasB = sA + 1 < npix;
+    const int sB = hasB ? sA + 1 : sA;
     const int k0 = chunk * kWave * KPL;
     const int kend = min(g.Nz, k0 + kWave * KPL);
     const int vb = (int)(((int64_t)nviews * split) / nsplit);
     const int ve = (int)(((int64_t)nviews * (split + 1)) / nsplit);
-    float *col = lds + wave * cap;
+    float *colA = lds + wave * 2 * cap;
+    float *colB = colA + cap;
 
-    float acc[KPL], cmp[KPL];
+    float accA[KPL], cmpA[KPL], accB[KPL], cmpB[KPL];
 #pragma unroll
-    for (int i = 0; i < KPL; ++i) {
-        acc[i] = 0.0f;
-        cmp[i] = 0.0f;
-    }
-
+    for (int i = 0; i < KPL; ++i) accA[i] = cmpA[i] = accB[i] = cmpB[i] = 0.0f;
+    const float kf0 = (float)(k0 + lane);
     for (int vi = vb; vi < ve; ++vi) {
-        const float *rec = T + ((size_t)vi * npix + s) * kRec;
-        const int nc = __float_as_int(rec[0]);
-        float Aw[2 * RAD + 1];
-#pragma unroll
-        for (int j = -RAD; j <= RAD; ++j) {
-            const float a = rec[3 + j];
-            Aw[j + RAD] = (POW == 2) ? a * a : a;
-        }
         const int vd = views ? views[vi] : vi;
-        const size_t vbase = (size_t)vd * g.C;
-
-        if (GEOM == MBX_GEOM_CONE) {
-            const float a = rec[6], W = rec[7], hv = rec[8], Lv = rec[9], c0 = rec[11], c1 = rec[12];
-            // rows reached by the taps of slices [k0, kend): unclamped window and its part on the detector
-            const int mfirst = (int)rintf(fmaf(W, (float)k0, a)) - RAD;
-            const int mlast = (int)rintf(fmaf(W, (float)(kend - 1), a)) + RAD;
-            const int mlo = max(mfirst, 0), mhi = min(mlast, g.R - 1);
-            if (mlo > mhi) continue;  // wave-uniform: this view misses the detector
-            // Fast layout: col[m - mfirst] for every row of the window (zeros off
-            // the detector), so the taps need no clamping.  If the window does not
-            // fit, fall back to the detector part plus one zero on each side.
-            const bool fit = mlast - mfirst + 1 <= cap;
-            const int mstart = fit ? mfirst : mlo - 1;
-            const int count = fit ? mlast - mfirst + 1 : mhi - mlo + 3;
-            const int64_t cofs = ((int64_t)vbase + (nc - RAD)) * (int64_t)g.R;  // channel nc-RAD, row 0
-            for (int i = lane; i < count; i += kWave) {
-                const int m = mstart + i;
-                float c = 0.0f;
-                if (m >= mlo && m <= mhi) {
-#pragma unroll
-                    for (int j = 0; j <= 2 * RAD; ++j) {
-                        const int n = nc - RAD + j;
-                        if (n >= 0 && n < g.C) {
-                            const int64_t off = cofs + (int64_t)j * g.R + m;
-                            float e = to_float(sino[off]);
-                            if (w) e *= w[off];
-                            c = fmaf(Aw[j], e, c);
-                        }
-                    }
-                }
-                col[i] = c;
-            }
-            wave_sync();
-            const float hm1 = hv - 1.0f;
-#pragma unroll
-            for (int i = 0; i < KPL; ++i) {
-                // lanes past the last slice repeat it (their sums are never stored)
-                const float kf = (float)min(k0 + lane + kWave * i, kend - 1);
-                const float t = fmaf(W, kf, a);
-                const float mc = rintf(t);
-                const float d = t - mc;  // |d| <= 1/2
-                const int base = (int)mc - RAD - mstart;  // buffer index of row mc - RAD
-                float sum;
-                if (RAD == 1) {
-                    float wm = __builtin_amdgcn_fmed3f(hm1 - d, 0.0f, Lv);        // row mc-1
-                    float w0 = __builtin_amdgcn_fmed3f(hv - fabsf(d), 0.0f, Lv);  // row mc
-                    float wp = __builtin_amdgcn_fmed3f(hm1 + d, 0.0f, Lv);        // row mc+1
-                    if (POW == 2) {
-                        w0 *= w0;
-                        wp *= wp;
-                        wm *= wm;
-                    }
-                    float cm, c0v, cp;
-                    if (fit) {
-                        const float *cb = col + base;
-                        cm = cb[0];
-                        c0v = cb[1];
-                        cp = cb[2];
-                    } else {
-                        cm = col[min(max(base, 0), count - 1)];
-                        c0v = col[min(max(base + 1, 0), count - 1)];
-                        cp = col[min(max(base + 2, 0), count - 1)];
-                    }
-                    sum = w0 * c0v;
-                    sum = fmaf(wp, cp, sum);
-                    sum = fmaf(wm, cm, sum);
-                } else {
-                    sum = 0.0f;
-#pragma unroll
-                    for (int j = 0; j <= 2 * RAD; ++j) {
-                        float wj = __builtin_amdgcn_fmed3f(hv - fabsf(d - (float)(j - RAD)), 0.0f, Lv);
-                        if (POW == 2) wj *= wj;
-                        const int idx = fit ? base + j : min(max(base + j, 0), count - 1);
-                        sum = fmaf(wj, col[idx], sum);
-                    }
-                }
-                const float u = fmaf(c1, kf, c0);
-                const float s2 = fmaf(u, u, 1.0f);
-                const float f = (POW == 2) ? s2 : __builtin_amdgcn_sqrtf(s2);
-                if (KAHAN) {
-                    const float y = sum * f - cmp[i];
-                    const float tt = acc[i] + y;
-                    cmp[i] = (tt - acc[i]) - y;
-                    acc[i] = tt;
-                } else {
-                    acc[i] = fmaf(sum, f, acc[i]);
-                }
-            }
-            wave_sync();
-        } else {
-#pragma unroll
-            for (int i = 0; i < KPL; ++i) {
-                const int k = k0 + lane + kWave * i;
-                if (k < kend) {
-                    float c = 0.0f;
-#pragma unroll
-                    for (int j = -RAD; j <= RAD; ++j) {
-                        const int n = nc + j;
-                        if (n >= 0 && n < g.C) {
-                            const size_t off = (vbase + n) * (size_t)g.R + (size_t)k;
-                            float e = to_float(sino[off]);
-                            if (w) e *= w[off];
-                            c = fmaf(Aw[j + RAD], e, c);
-                        }
-                    }
-                    if (KAHAN) {
-                        const float y = c - cmp[i];
-                        const float tt = acc[i] + y;
-                        cmp[i] = (tt - acc[i]) - y;
-                        acc[i] = tt;
-                    } else {
-                        acc[i] += c;
-                    }
-                }
-            }
-        }
+        const float *recA = T + ((size_t)vi * npix + sA) * kRec;
+        const float *recB = T + ((size_t)vi * npix + sB) * kRec;
+        back_pixel_view<GEOM, KPL, RAD, ET, POW, KAHAN, WGT>(g, recA, sino, w, vd, k0, kend, cap, colA, lane,
+                                                             kf0, accA, cmpA);
+        if (hasB)
+            back_pixel_view<GEOM, KPL, RAD, ET, POW, KAHAN, WGT>(g, recB, sino, w, vd, k0, kend, cap, colB,
+                                                                 lane, kf0, accB, cmpB);
     }
-
 #pragma unroll
     for (int i = 0; i < KPL; ++i) {
         const int k = k0 + lane + kWave * i;
         if (k < kend) {
             if (nsplit == 1) {
-                const size_t o = (size_t)s * g.Nz + k;
-                const float v = scale * acc[i];
-                out[o] = accumulate ? out[o] + v : v;
+                const size_t oA = (size_t)sA * g.Nz + k;
+                const float vA = scale * accA[i];
+                out[oA] = accumulate ? out[oA] + vA : vA;
+                if (hasB) {
+                    const size_t oB = (size_t)sB * g.Nz + k;
+                    const float vB = scale * accB[i];
+                    out[oB] = accumulate ? out[oB] + vB : vB;
+                }
             } else {
-                part[((size_t)split * npix + s) * g.Nz + k] = acc[i];
+                part[((size_t)split * npix + sA) * g.Nz + k] = accA[i];
+                if (hasB) part[((size_t)split * npix + sB) * g.Nz + k] = accB[i];
             }
         }
     }
@@ -411,11 +478,17 @@ template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN>
 static void launch_back_t(const DevGeom &g, const BackPlan &pl, const float *T, const ET *sino, const float *w,
                           int npix, const int32_t *views, int nviews, float scale, int accumulate, float *out,
                           float *part, hipStream_t st) {
-    const int64_t quads = ((int64_t)npix + 3) / 4;
-    const int64_t blocks = quads * pl.nchunk * pl.nsplit;
-    const size_t lds = GEOM == MBX_GEOM_CONE ? 4 * (size_t)pl.cap * sizeof(float) : 0;
-    hipLaunchKernelGGL((back_kernel<GEOM, KPL, RAD, ET, POW, KAHAN>), dim3((unsigned)blocks), dim3(256), lds, st, g, T,
-                       sino, w, npix, views, nviews, pl.nsplit, pl.nchunk, pl.cap, scale, accumulate, out, part);
+    const int64_t octets = ((int64_t)npix + 7) / 8;  // 4 waves x 2 pixels per block
+    const int64_t blocks = octets * pl.nchunk * pl.nsplit;
+    const size_t lds = GEOM == MBX_GEOM_CONE ? 4 * 2 * (size_t)pl.cap * sizeof(float) : 0;
+    if (w)
+        hipLaunchKernelGGL((back_kernel<GEOM, KPL, RAD, ET, POW, KAHAN, true>), dim3((unsigned)blocks), dim3(256), lds,
+                           st, g, T, sino, w, npix, views, nviews, pl.nsplit, pl.nchunk, pl.cap, scale, accumulate,
+                           out, part);
+    else
+        hipLaunchKernelGGL((back_kernel<GEOM, KPL, RAD, ET, POW, KAHAN, false>), dim3((unsigned)blocks), dim3(256),
+                           lds, st, g, T, sino, w, npix, views, nviews, pl.nsplit, pl.nchunk, pl.cap, scale,
+                           accumulate, out, part);
 }
 
 template <int GEOM, int KPL, int RAD>
@@ -461,9 +534,6 @@ static int dispatch_back_1(const DevGeom &g, const BackPlan &pl, const float *T,
                            const float *w, int npix, const int32_t *views, int nviews, int pw, int kahan, float scale,
                            int accumulate, float *out, float *part, hipStream_t st) {
     switch (pl.kpl) {
-        case 2:
-            return dispatch_back_2<GEOM, 2>(g, pl, T, sino, dt, w, npix, views, nviews, pw, kahan, scale, accumulate,
-                                            out, part, st);
         case 4:
             return dispatch_back_2<GEOM, 4>(g, pl, T, sino, dt, w, npix, views, nviews, pw, kahan, scale, accumulate,
                                             out, part, st);
@@ -527,12 +597,12 @@ int launch_back(const DevGeom &g, const float *angles, const void *sino, int sin
 // Forward projection
 // ======================================================================
 
-constexpr int kFwdBatch = 8;
+constexpr int kFwdBatch = 8;  // == waves per forward block: one column per wave per batch
 
 FwdPlan plan_fwd(const DevGeom &g, int nviews) {
     FwdPlan p;
     const int rows = std::min(g.R, 512);
-    p.nw = 4;                                                       // waves per block
+    p.nw = 8;                                                       // waves per block
     p.rpw = kWave * ((rows + p.nw * kWave - 1) / (p.nw * kWave));  // phase-2 rows per wave
     p.rows_pb = std::min(p.nw * p.rpw, ((g.R + kWave - 1) / kWave) * kWave);
     p.nrb = (g.R + p.rows_pb - 1) / p.rows_pb;
@@ -542,8 +612,8 @@ FwdPlan plan_fwd(const DevGeom &g, int nviews) {
     const size_t tile = (size_t)p.cb * (p.rows_pb + 1) * sizeof(float);
     const size_t cols = (size_t)kFwdBatch * p.rows_pb * sizeof(float) + kFwdBatch * 6 * sizeof(float);
     const size_t list = (size_t)(p.list + kFwdBatch) * sizeof(int);
-    const size_t cyl = g.kind == MBX_GEOM_CONE ? (size_t)p.nw * (g.Nz + 4) * sizeof(float) : 0;
-    const size_t scan = 16 * sizeof(int);
+    const size_t cyl = g.kind == MBX_GEOM_CONE ? (size_t)p.nw * (g.Nz + 4 + kWave * kFill) * sizeof(float) : 0;
+    const size_t scan = 32 * sizeof(int);  // wave counts, then one double per wave
     p.lds = tile + cols + list + cyl + scan;
     p.nblocks = (int64_t)nviews * p.ncb * p.nrb;
     return p;
@@ -579,7 +649,7 @@ __device__ __forceinline__ T block_sum(T v, T *scratch) {
 //            LDS tile -- one writer per tile element, fixed order.
 // The tile is written to HBM once.
 template <int GEOM, int RAD, int RB, typename OT, int LAYOUT>
-__global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__restrict__ T, const int *__restrict__ NC,
+__global__ __launch_bounds__(512) void fwd_kernel(DevGeom g, const float *__restrict__ T, const int *__restrict__ NC,
                                                   const float *__restrict__ vox, int npix, int cb, int ncb, int nrb,
                                                   int rows_pb, int rpw, int list_cap, OT *__restrict__ sino,
                                                   int accumulate, const float *__restrict__ w,
@@ -593,7 +663,7 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
     int *slot_nc = (int *)(slot_a + kFwdBatch * (2 * RAD + 1));  // [kFwdBatch]
     int *l_s = slot_nc + kFwdBatch;                           // [list_cap]
     float *cyl_all = (float *)(l_s + list_cap);               // [nw][Nz + 4] (cone)
-    int *scan = (int *)(cyl_all + (GEOM == MBX_GEOM_CONE ? nw * (g.Nz + 4) : 0));
+    int *scan = (int *)(cyl_all + (GEOM == MBX_GEOM_CONE ? nw * (g.Nz + 4 + kWave * kFill) : 0));
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -607,7 +677,7 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
     const int nri = (nrows + kWave - 1) / kWave;  // row iterations of a whole-block column
 
     for (int i = tid; i < cb * tstride; i += blockDim.x) tile[i] = 0.0f;
-    float *cyl = cyl_all + wave * (g.Nz + 4);
+    float *cyl = cyl_all + wave * (g.Nz + 4 + kWave * kFill);
     const int wm0 = m0 + wave * rpw;           // rows owned in phase 2
     const int wm1 = min(m1, wm0 + rpw);
     const int *ncv = NC + (size_t)vi * npix;
@@ -645,7 +715,8 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
         for (int b0 = 0; b0 < count; b0 += kFwdBatch) {
             const int nb = min(kFwdBatch, count - b0);
             // ---- phase 1: columns of the batch, one wave per pixel
-            for (int slot = wave; slot < nb; slot += nw) {
+            if (wave < nb) {
+                const int slot = wave;
                 const int sc = __builtin_amdgcn_readfirstlane(l_s[b0 + slot]);
                 const float *rec = Tv + (size_t)sc * kRec;
                 float *colv = colbuf + slot * rows_pb;
@@ -656,52 +727,54 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
                                 c1 = rec[12];
                     const int rlo = max(m0, __float_as_int(rec[13]));
                     const int rhi = min(m1 - 1, __float_as_int(rec[14]));
-                    int kf = (int)rintf(((float)rlo - a) * invW) - RB;
-                    int kl = (int)rintf(((float)rhi - a) * invW) + RB;
-                    kf = max(kf, 0);
-                    kl = min(kl, g.Nz - 1);
+                    const int kf = max((int)rintf(((float)rlo - a) * invW) - RB, 0);
+                    const int kl = min((int)rintf(((float)rhi - a) * invW) + RB, g.Nz - 1);
                     if (rlo > rhi || kf > kl) {  // wave-uniform: no row of this block is reached
-                        for (int i = 0; i < nri; ++i) {
-                            const int r = lane + kWave * i;
-                            if (r < nrows) colv[r] = 0.0f;
+                        for (int i = 0; i < nri; ++i) colv[lane + kWave * i] = 0.0f;
+                    } else {
+                        const int span = kl - kf + 1;
+                        const i32x4_t rs = make_rsrc(vox + (size_t)sc * g.Nz + kf);
+                        for (int q0 = 0; q0 < span; q0 += kWave * kFill) {
+                            float xv[kFill];  // all loads of the round in flight together
+#pragma unroll
+                            for (int u = 0; u < kFill; ++u) {
+                                const int kk = min(q0 + u * kWave + lane, span - 1);
+                                xv[u] = buf_load<float>(rs, kk * 4, 0);
+                            }
+#pragma unroll
+                            for (int u = 0; u < kFill; ++u) {
+                                const int kk = q0 + u * kWave + lane;  // cyl holds a round past the slices
+                                const float uu = fmaf(c1, (float)(kf + kk), c0);
+                                cyl[1 + kk] = xv[u] * __builtin_amdgcn_sqrtf(fmaf(uu, uu, 1.0f));
+                            }
                         }
-                        continue;
-                    }
-                    const int span = kl - kf + 1;
-                    const float *xs = vox + (size_t)sc * g.Nz;
-                    for (int kk = lane; kk < span; kk += kWave) {
-                        const float k = (float)(kf + kk);
-                        const float u = fmaf(c1, k, c0);
-                        cyl[1 + kk] = xs[kf + kk] * __builtin_amdgcn_sqrtf(fmaf(u, u, 1.0f));
-                    }
-                    if (lane == 0) {
                         cyl[0] = 0.0f;
                         cyl[span + 1] = 0.0f;
-                    }
-                    wave_sync();
-                    const int base0 = 1 - kf;
-                    const float ainv = a * invW;
-                    for (int i = 0; i < nri; ++i) {
-                        const int r = lane + kWave * i;
-                        const float mf = (float)(m0 + r);
-                        const float kc = rintf(fmaf(mf, invW, -ainv));
-                        const float dm = fmaf(W, kc, a) - mf;  // t_kc - m
-                        const int bidx = (int)kc + base0;
-                        float acc = 0.0f;
+                        wave_sync();
+                        const int base0 = 1 - kf;
+                        const float ainv = a * invW;
+                        for (int i = 0; i < nri; ++i) {
+                            const float mf = (float)(m0 + lane + kWave * i);
+                            const float kc = rintf(fmaf(mf, invW, -ainv));
+                            const float dm = fmaf(W, kc, a) - mf;  // t_kc - m
+                            const int bidx = (int)kc + base0;
+                            float acc = 0.0f;
 #pragma unroll
-                        for (int j = -RB; j <= RB; ++j) {
-                            const float wt = __builtin_amdgcn_fmed3f(hv - fabsf(fmaf(W, (float)j, dm)), 0.0f, Lv);
-                            const int idx = min(max(bidx + j, 0), span + 1);
-                            acc = fmaf(wt, cyl[idx], acc);
+                            for (int j = -RB; j <= RB; ++j) {
+                                const float wt =
+                                    __builtin_amdgcn_fmed3f(hv - fabsf(fmaf(W, (float)j, dm)), 0.0f, Lv);
+                                const int idx = min(max(bidx + j, 0), span + 1);
+                                acc = fmaf(wt, cyl[idx], acc);
+                            }
+                            colv[lane + kWave * i] = acc;  // colbuf rows are a multiple of 64
                         }
-                        if (r < nrows) colv[r] = acc;
+                        wave_sync();
                     }
-                    wave_sync();
                 } else {
-                    const float *xs = vox + (size_t)sc * g.Nz;
+                    const i32x4_t rs = make_rsrc(vox + (size_t)sc * g.Nz);
                     for (int i = 0; i < nri; ++i) {
-                        const int r = lane + kWave * i;
-                        if (r < nrows) colv[r] = xs[m0 + r];
+                        const int r = min(m0 + lane + kWave * i, g.Nz - 1);
+                        colv[lane + kWave * i] = buf_load<float>(rs, r * 4, 0);
                     }
                 }
             }
@@ -709,15 +782,17 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
             // ---- phase 2: horizontal taps into this wave's rows, batch order
             if (wm0 < wm1) {
                 for (int slot = 0; slot < nb; ++slot) {
-                    const int nc = slot_nc[slot];
+                    const int nc = __builtin_amdgcn_readfirstlane(slot_nc[slot]);
                     const float *colv = colbuf + slot * rows_pb + (wm0 - m0);
 #pragma unroll
                     for (int j = -RAD; j <= RAD; ++j) {
                         const int n = nc + j;
                         if (n >= n0 && n < n1) {
-                            const float aw = slot_a[slot * (2 * RAD + 1) + j + RAD];
+                            const int aw = __builtin_amdgcn_readfirstlane(
+                                __float_as_int(slot_a[slot * (2 * RAD + 1) + j + RAD]));
                             float *trow = tile + (n - n0) * tstride + (wm0 - m0);
-                            for (int r = lane; r < wm1 - wm0; r += kWave) trow[r] += aw * colv[r];
+                            for (int r = 0; r < rpw; r += kWave)  // rows past the block end are never written out
+                                trow[r + lane] += __int_as_float(aw) * colv[r + lane];
                         }
                     }
                 }
