@@ -57,13 +57,14 @@ BackPlan plan_back(const DevGeom &g, int npix, int nviews);
 int64_t back_workspace_bytes(const DevGeom &g, int npix, int nviews);
 
 struct FwdPlan {
-    int nw;       // waves per block
-    int rpw;      // detector rows per wave
+    int nw;       // waves per block (one 64-row slab each)
     int rows_pb;  // detector rows per block
     int nrb;      // row blocks
     int cb;       // channels per block
     int ncb;      // channel blocks
     int list;     // candidate list capacity
+    int nb;       // candidates staged per batch
+    int cstride;  // LDS floats per staged cylinder
     size_t lds;   // dynamic LDS bytes
     int64_t nblocks;
 };

@@ -597,24 +597,28 @@ int launch_back(const DevGeom &g, const float *angles, const void *sino, int sin
 // Forward projection
 // ======================================================================
 
-constexpr int kFwdBatch = 8;  // == waves per forward block: one column per wave per batch
+constexpr int kFwdCB = 8;         // channels per block: the per-lane register tile
+constexpr int kFwdPad = 5;        // zero slices either side of a staged cylinder (>= 2*bp_psf_radius + 1)
+constexpr int kFwdMaxBatch = 16;  // cylinders staged per batch
+constexpr int kFwdList = 1024;    // candidate list capacity
+static_assert(kFwdCB == 8, "the forward register tile is read as two float4 weight vectors");
 
 FwdPlan plan_fwd(const DevGeom &g, int nviews) {
     FwdPlan p;
-    const int rows = std::min(g.R, 512);
-    p.nw = 8;                                                       // waves per block
-    p.rpw = kWave * ((rows + p.nw * kWave - 1) / (p.nw * kWave));  // phase-2 rows per wave
-    p.rows_pb = std::min(p.nw * p.rpw, ((g.R + kWave - 1) / kWave) * kWave);
+    p.nw = (std::min(g.R, 512) + kWave - 1) / kWave;
+    p.rows_pb = p.nw * kWave;
     p.nrb = (g.R + p.rows_pb - 1) / p.rows_pb;
-    p.cb = g.C >= 16 ? 16 : (g.C >= 8 ? 8 : 4);
+    p.cb = kFwdCB;
     p.ncb = (g.C + p.cb - 1) / p.cb;
-    p.list = 1024;
-    const size_t tile = (size_t)p.cb * (p.rows_pb + 1) * sizeof(float);
-    const size_t cols = (size_t)kFwdBatch * p.rows_pb * sizeof(float) + kFwdBatch * 6 * sizeof(float);
-    const size_t list = (size_t)(p.list + kFwdBatch) * sizeof(int);
-    const size_t cyl = g.kind == MBX_GEOM_CONE ? (size_t)p.nw * (g.Nz + 4 + kWave * kFill) * sizeof(float) : 0;
-    const size_t scan = 32 * sizeof(int);  // wave counts, then one double per wave
-    p.lds = tile + cols + list + cyl + scan;
+    p.list = kFwdList;
+    // A staged cylinder holds the slices the block's rows reach (at most all
+    // slices), one load round past them and the zero pads.
+    const int span_max = g.kind == MBX_GEOM_CONE ? g.Nz : p.rows_pb;
+    p.cstride = ((span_max + kWave * kFill - 1) / (kWave * kFill)) * (kWave * kFill) + 2 * kFwdPad + 6;
+    const size_t budget = 40 * 1024;
+    p.nb = (int)std::max<size_t>(1, std::min<size_t>(kFwdMaxBatch, budget / ((size_t)p.cstride * sizeof(float))));
+    p.lds = (size_t)p.nb * p.cstride * sizeof(float) +
+            (size_t)(p.list + (2 + kFwdCB) * kFwdMaxBatch + 32) * sizeof(int);
     p.nblocks = (int64_t)nviews * p.ncb * p.nrb;
     return p;
 }
@@ -638,51 +642,53 @@ __device__ __forceinline__ T block_sum(T v, T *scratch) {
     return r;  // valid in thread 0
 }
 
-// One workgroup per (view, channel block, row block).  The subset pixels whose
-// channel taps meet the block are compacted in pixel order, then handled in
-// batches of kFwdBatch:
-//   phase 1: one wave per pixel resamples the pixel's cylinder onto the block's
-//            detector rows (row-driven vertical fan, mbirjax/cone_beam.py:347-372)
-//            into an LDS column;
-//   phase 2: wave w adds the horizontal taps (mbirjax/cone_beam.py:276-282) of
-//            every column of the batch, in batch order, to its own rows of the
-//            LDS tile -- one writer per tile element, fixed order.
-// The tile is written to HBM once.
+// One workgroup per (view, 8-channel block, row block); wave w owns the 64
+// detector rows m0 + 64w + lane and keeps their 8 channels in registers.  The
+// subset pixels whose channel taps meet the block are compacted in pixel
+// order and handled in batches:
+//   staging: one wave per pixel loads the slices of its cylinder that the
+//            block's rows reach, scaled by 1/cos(phi_k), into LDS with zero
+//            pads;
+//   update:  every wave resamples each staged cylinder onto its rows
+//            (row-driven vertical fan, mbirjax/cone_beam.py:347-372) and adds
+//            the horizontal taps (mbirjax/cone_beam.py:276-282) to its
+//            register tile, in batch order.
+// Each output element has one writer and a fixed summation order; the tile is
+// written to HBM once.
 template <int GEOM, int RAD, int RB, typename OT, int LAYOUT>
 __global__ __launch_bounds__(512) void fwd_kernel(DevGeom g, const float *__restrict__ T, const int *__restrict__ NC,
-                                                  const float *__restrict__ vox, int npix, int cb, int ncb, int nrb,
-                                                  int rows_pb, int rpw, int list_cap, OT *__restrict__ sino,
+                                                  const float *__restrict__ vox, int npix, int ncb, int nrb,
+                                                  int nbatch, int cstride, int list_cap, OT *__restrict__ sino,
                                                   int accumulate, const float *__restrict__ w,
                                                   double *__restrict__ part_sumsq) {
+    constexpr int CB = kFwdCB, P = kFwdPad;
     extern __shared__ float lds[];
-    const int tstride = rows_pb + 1;
     const int nw = blockDim.x >> 6;
-    float *tile = lds;                                        // [cb][rows_pb + 1]
-    float *colbuf = tile + (size_t)cb * tstride;              // [kFwdBatch][rows_pb]
-    float *slot_a = colbuf + (size_t)kFwdBatch * rows_pb;     // [kFwdBatch][2*RAD+1]
-    int *slot_nc = (int *)(slot_a + kFwdBatch * (2 * RAD + 1));  // [kFwdBatch]
-    int *l_s = slot_nc + kFwdBatch;                           // [list_cap]
-    float *cyl_all = (float *)(l_s + list_cap);               // [nw][Nz + 4] (cone)
-    int *scan = (int *)(cyl_all + (GEOM == MBX_GEOM_CONE ? nw * (g.Nz + 4 + kWave * kFill) : 0));
+    float *cyl_all = lds;                                     // [nbatch][cstride]
+    int *l_s = (int *)(cyl_all + (size_t)nbatch * cstride);  // [list_cap]
+    int *slot_kf = l_s + list_cap;                            // [kFwdMaxBatch]
+    int *slot_span = slot_kf + kFwdMaxBatch;                  // [kFwdMaxBatch]
+    float *slot_w = (float *)(slot_span + kFwdMaxBatch);      // [kFwdMaxBatch][CB] channel weights
+    int *scan = (int *)(slot_w + kFwdMaxBatch * CB);          // [32]: wave counts; doubles at +8
 
     const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int b = blockIdx.x;
     const int vi = b / (ncb * nrb);
     const int rem = b - vi * (ncb * nrb);
     const int cbi = rem / nrb, rbi = rem - (rem / nrb) * nrb;
-    const int n0 = cbi * cb, n1 = min(g.C, n0 + cb);
+    const int n0 = cbi * CB, n1 = min(g.C, n0 + CB);
+    const int rows_pb = nw * kWave;
     const int m0 = rbi * rows_pb, m1 = min(g.R, m0 + rows_pb);
-    const int nrows = m1 - m0;
-    const int nri = (nrows + kWave - 1) / kWave;  // row iterations of a whole-block column
-
-    for (int i = tid; i < cb * tstride; i += blockDim.x) tile[i] = 0.0f;
-    float *cyl = cyl_all + wave * (g.Nz + 4 + kWave * kFill);
-    const int wm0 = m0 + wave * rpw;           // rows owned in phase 2
-    const int wm1 = min(m1, wm0 + rpw);
+    const int wm0 = m0 + wave * kWave;  // this wave's slab
+    const int m = wm0 + lane;           // this lane's row
+    const float mf = (float)m;
     const int *ncv = NC + (size_t)vi * npix;
     const float *Tv = T + (size_t)vi * npix * kRec;
-    __syncthreads();
+
+    float acc[CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) acc[c] = 0.0f;
 
     int count = 0;
     for (int base = 0; base < npix; base += blockDim.x) {
@@ -712,120 +718,119 @@ __global__ __launch_bounds__(512) void fwd_kernel(DevGeom g, const float *__rest
         __syncthreads();
         if (!(count > list_cap - (int)blockDim.x || base + (int)blockDim.x >= npix)) continue;
 
-        for (int b0 = 0; b0 < count; b0 += kFwdBatch) {
-            const int nb = min(kFwdBatch, count - b0);
-            // ---- phase 1: columns of the batch, one wave per pixel
-            if (wave < nb) {
-                const int slot = wave;
+        for (int b0 = 0; b0 < count; b0 += nbatch) {
+            const int nb = min(nbatch, count - b0);
+            // ---- staging: one wave per cylinder
+            for (int slot = wave; slot < nb; slot += nw) {
                 const int sc = __builtin_amdgcn_readfirstlane(l_s[b0 + slot]);
                 const float *rec = Tv + (size_t)sc * kRec;
-                float *colv = colbuf + slot * rows_pb;
-                if (lane == 0) slot_nc[slot] = __float_as_int(rec[0]);
-                if (lane <= 2 * RAD) slot_a[slot * (2 * RAD + 1) + lane] = rec[3 - RAD + lane];
+                float *cyl = cyl_all + (size_t)slot * cstride;
+                int kf, span;
+                float c0 = 0.0f, c1 = 0.0f;
                 if (GEOM == MBX_GEOM_CONE) {
-                    const float a = rec[6], W = rec[7], hv = rec[8], Lv = rec[9], invW = rec[10], c0 = rec[11],
-                                c1 = rec[12];
+                    const float a = rec[6], invW = rec[10];
+                    c0 = rec[11];
+                    c1 = rec[12];
                     const int rlo = max(m0, __float_as_int(rec[13]));
                     const int rhi = min(m1 - 1, __float_as_int(rec[14]));
-                    const int kf = max((int)rintf(((float)rlo - a) * invW) - RB, 0);
+                    kf = max((int)rintf(((float)rlo - a) * invW) - RB, 0);
                     const int kl = min((int)rintf(((float)rhi - a) * invW) + RB, g.Nz - 1);
-                    if (rlo > rhi || kf > kl) {  // wave-uniform: no row of this block is reached
-                        for (int i = 0; i < nri; ++i) colv[lane + kWave * i] = 0.0f;
-                    } else {
-                        const int span = kl - kf + 1;
-                        const i32x4_t rs = make_rsrc(vox + (size_t)sc * g.Nz + kf);
-                        for (int q0 = 0; q0 < span; q0 += kWave * kFill) {
-                            float xv[kFill];  // all loads of the round in flight together
-#pragma unroll
-                            for (int u = 0; u < kFill; ++u) {
-                                const int kk = min(q0 + u * kWave + lane, span - 1);
-                                xv[u] = buf_load<float>(rs, kk * 4, 0);
-                            }
-#pragma unroll
-                            for (int u = 0; u < kFill; ++u) {
-                                const int kk = q0 + u * kWave + lane;  // cyl holds a round past the slices
-                                const float uu = fmaf(c1, (float)(kf + kk), c0);
-                                cyl[1 + kk] = xv[u] * __builtin_amdgcn_sqrtf(fmaf(uu, uu, 1.0f));
-                            }
-                        }
-                        cyl[0] = 0.0f;
-                        cyl[span + 1] = 0.0f;
-                        wave_sync();
-                        const int base0 = 1 - kf;
-                        const float ainv = a * invW;
-                        for (int i = 0; i < nri; ++i) {
-                            const float mf = (float)(m0 + lane + kWave * i);
-                            const float kc = rintf(fmaf(mf, invW, -ainv));
-                            const float dm = fmaf(W, kc, a) - mf;  // t_kc - m
-                            const int bidx = (int)kc + base0;
-                            float acc = 0.0f;
-#pragma unroll
-                            for (int j = -RB; j <= RB; ++j) {
-                                const float wt =
-                                    __builtin_amdgcn_fmed3f(hv - fabsf(fmaf(W, (float)j, dm)), 0.0f, Lv);
-                                const int idx = min(max(bidx + j, 0), span + 1);
-                                acc = fmaf(wt, cyl[idx], acc);
-                            }
-                            colv[lane + kWave * i] = acc;  // colbuf rows are a multiple of 64
-                        }
-                        wave_sync();
-                    }
+                    span = (rlo <= rhi && kf <= kl) ? kl - kf + 1 : 0;
                 } else {
-                    const i32x4_t rs = make_rsrc(vox + (size_t)sc * g.Nz);
-                    for (int i = 0; i < nri; ++i) {
-                        const int r = min(m0 + lane + kWave * i, g.Nz - 1);
-                        colv[lane + kWave * i] = buf_load<float>(rs, r * 4, 0);
+                    kf = m0;
+                    span = m1 - m0;
+                }
+                if (lane == 0) {
+                    slot_kf[slot] = kf;
+                    slot_span[slot] = span;
+                }
+                if (lane < CB) {  // weight of channel n0 + lane: Ac(n_c + j) for |j| <= RAD, else 0
+                    const int j = n0 + lane - __float_as_int(rec[0]);
+                    const float aw = rec[3 + min(max(j, -RAD), RAD)];
+                    slot_w[slot * CB + lane] = (j >= -RAD && j <= RAD) ? aw : 0.0f;
+                }
+                if (span > 0) {  // wave-uniform
+                    const i32x4_t rs = make_rsrc(vox + (size_t)sc * g.Nz + kf);
+                    for (int q0 = 0; q0 < span; q0 += kWave * kFill) {
+                        float xv[kFill];  // all loads of the round in flight together
+#pragma unroll
+                        for (int u = 0; u < kFill; ++u) {
+                            const int kk = min(q0 + u * kWave + lane, span - 1);
+                            xv[u] = buf_load<float>(rs, kk * 4, 0);
+                        }
+#pragma unroll
+                        for (int u = 0; u < kFill; ++u) {
+                            const int kk = q0 + u * kWave + lane;  // a round past the slices fits in cstride
+                            float v = xv[u];
+                            if (GEOM == MBX_GEOM_CONE) {
+                                const float uu = fmaf(c1, (float)(kf + kk), c0);
+                                v *= __builtin_amdgcn_sqrtf(fmaf(uu, uu, 1.0f));
+                            }
+                            cyl[P + kk] = kk < span ? v : 0.0f;
+                        }
+                    }
+                    if (lane < P) {
+                        cyl[lane] = 0.0f;
+                        cyl[P + span + lane] = 0.0f;
                     }
                 }
             }
             __syncthreads();
-            // ---- phase 2: horizontal taps into this wave's rows, batch order
-            if (wm0 < wm1) {
+            // ---- update: every wave adds the batch to its rows, batch order
+            if (wm0 < m1) {  // wave-uniform
+                // lane i holds slot i's staging data; the loop reads it with readlane
+                const int l_span = lane < nb ? slot_span[lane] : 0;
+                const int l_kf = lane < nb ? slot_kf[lane] : 0;
+                const int l_sc = lane < nb ? l_s[b0 + lane] : 0;
                 for (int slot = 0; slot < nb; ++slot) {
-                    const int nc = __builtin_amdgcn_readfirstlane(slot_nc[slot]);
-                    const float *colv = colbuf + slot * rows_pb + (wm0 - m0);
+                    const int span = __builtin_amdgcn_readlane(l_span, slot);
+                    if (span == 0) continue;
+                    const int kf = __builtin_amdgcn_readlane(l_kf, slot);
+                    const int sc = __builtin_amdgcn_readlane(l_sc, slot);
+                    const float4 *r4 = (const float4 *)(Tv + (size_t)sc * kRec);
+                    const float4 R1 = r4[1], R2 = r4[2], R3 = r4[3];
+                    if (GEOM == MBX_GEOM_CONE &&
+                        (__float_as_int(R3.y) > wm0 + kWave - 1 || __float_as_int(R3.z) < wm0))
+                        continue;  // no row of this slab is reached
+                    const float4 wA = *(const float4 *)(slot_w + slot * CB);
+                    const float4 wB = *(const float4 *)(slot_w + slot * CB + 4);
+                    const float wc[CB] = {wA.x, wA.y, wA.z, wA.w, wB.x, wB.y, wB.z, wB.w};
+                    const float *cyl = cyl_all + (size_t)slot * cstride;
+                    float v;
+                    if (GEOM == MBX_GEOM_CONE) {
+                        const float a = R1.z, W = R1.w, hv = R2.x, Lv = R2.y, invW = R2.z;
+                        const float kc = rintf(fmaf(mf, invW, -(a * invW)));
+                        const float dm = fmaf(W, kc, a) - mf;  // t_kc - m
+                        // taps past the staged slices read the zero pads
+                        const int off = min(max((int)kc + (P - kf), P - RB - 1), P + span + RB);
+                        v = 0.0f;
 #pragma unroll
-                    for (int j = -RAD; j <= RAD; ++j) {
-                        const int n = nc + j;
-                        if (n >= n0 && n < n1) {
-                            const int aw = __builtin_amdgcn_readfirstlane(
-                                __float_as_int(slot_a[slot * (2 * RAD + 1) + j + RAD]));
-                            float *trow = tile + (n - n0) * tstride + (wm0 - m0);
-                            for (int r = 0; r < rpw; r += kWave)  // rows past the block end are never written out
-                                trow[r + lane] += __int_as_float(aw) * colv[r + lane];
+                        for (int j = -RB; j <= RB; ++j) {
+                            const float wt = __builtin_amdgcn_fmed3f(hv - fabsf(fmaf(W, (float)j, dm)), 0.0f, Lv);
+                            v = fmaf(wt, cyl[off + j], v);
                         }
+                    } else {
+                        v = cyl[P + (m - m0)];  // row m sees slice m
                     }
+#pragma unroll
+                    for (int c = 0; c < CB; ++c) acc[c] = fmaf(wc[c], v, acc[c]);
                 }
             }
             __syncthreads();
         }
         count = 0;
     }
-    __syncthreads();
 
-    // ---- write the tile once (optionally adding to what is there)
-    const int ncol = n1 - n0;
+    // ---- write the register tile once (optionally adding to what is there)
     double ss = 0.0;
-    if (LAYOUT == MBX_LAYOUT_VCR) {
-        for (int nn = 0; nn < ncol; ++nn) {
-            for (int mm = tid; mm < nrows; mm += blockDim.x) {
-                const size_t o = ((size_t)vi * g.C + (n0 + nn)) * (size_t)g.R + (m0 + mm);
-                float v = tile[nn * tstride + mm];
-                if (accumulate) v += to_float(sino[o]);
-                const OT st = from_float<OT>(v);
-                sino[o] = st;
-                if (part_sumsq) {
-                    const float vs = to_float(st);
-                    ss += (double)vs * (double)vs * (w ? (double)w[o] : 1.0);
-                }
-            }
-        }
-    } else {
-        for (int idx = tid; idx < nrows * cb; idx += blockDim.x) {
-            const int mm = idx / cb, nn = idx - (idx / cb) * cb;
-            if (nn < ncol) {
-                const size_t o = ((size_t)vi * g.R + (m0 + mm)) * (size_t)g.C + (n0 + nn);
-                float v = tile[nn * tstride + mm];
+    if (m < m1) {
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+            const int n = n0 + c;
+            if (n < n1) {
+                const size_t o = LAYOUT == MBX_LAYOUT_VCR ? ((size_t)vi * g.C + n) * (size_t)g.R + m
+                                                          : ((size_t)vi * g.R + m) * (size_t)g.C + n;
+                float v = acc[c];
                 if (accumulate) v += to_float(sino[o]);
                 const OT st = from_float<OT>(v);
                 sino[o] = st;
@@ -847,7 +852,7 @@ template <int GEOM, int RAD, int RB, typename OT, int LAYOUT>
 static void launch_fwd_t(const DevGeom &g, const FwdPlan &pl, const float *T, const int *NC, const float *vox,
                          int npix, OT *sino, int accumulate, const float *w, double *part, hipStream_t st) {
     hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, OT, LAYOUT>), dim3((unsigned)pl.nblocks), dim3(64 * pl.nw), pl.lds,
-                       st, g, T, NC, vox, npix, pl.cb, pl.ncb, pl.nrb, pl.rows_pb, pl.rpw, pl.list, sino, accumulate, w,
+                       st, g, T, NC, vox, npix, pl.ncb, pl.nrb, pl.nb, pl.cstride, pl.list, sino, accumulate, w,
                        part);
 }
 
