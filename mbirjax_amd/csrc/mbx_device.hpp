@@ -120,6 +120,8 @@ struct DevGeom {
     float img_col_center;  // (cols-1)/2
     float slice_center;  // (Nz-1)/2
     float dv_over_dc, dv_over_dr;
+    int kref;   // reference slice (Nz/2) of the table's vertical parametrisation
+    mbx_geom h; // the caller's double-precision geometry (table kernel)
 };
 
 // The view-dependent part of one (pixel, view) pair that both projectors

@@ -48,9 +48,11 @@ int make_dev_geom(const mbx_geom *g, DevGeom *d) {
         return fail(MBX_EINVAL, "all sinogram and recon dimensions must be positive");
     if ((int64_t)g->num_recon_rows * g->num_recon_cols > INT32_MAX)
         return fail(MBX_EINVAL, "num_recon_rows*num_recon_cols exceeds int32");
-    // The kernels address within one view with 32-bit byte offsets.
+    // The kernels address within one view with 32-bit byte offsets; the table
+    // packs two row indices into one word.
     if ((int64_t)g->num_det_rows * g->num_det_channels * 4 > INT32_MAX)
         return fail(MBX_EINVAL, "num_det_rows*num_det_channels*4 bytes per view exceeds int32");
+    if (g->num_det_rows > 32767) return fail(MBX_EINVAL, "num_det_rows must be below 32768");
     if (!(g->delta_det_channel > 0) || !(g->delta_voxel > 0) || !std::isfinite(g->delta_det_channel) ||
         !std::isfinite(g->delta_voxel))
         return fail(MBX_EINVAL, "delta_det_channel and delta_voxel must be positive and finite");
@@ -94,6 +96,8 @@ int make_dev_geom(const mbx_geom *g, DevGeom *d) {
     o.slice_center = (float)((g->num_recon_slices - 1) / 2.0);
     o.dv_over_dc = (float)(g->delta_voxel / g->delta_det_channel);
     o.dv_over_dr = g->kind == MBX_GEOM_CONE ? (float)(g->delta_voxel / g->delta_det_row) : 1.0f;
+    o.kref = g->num_recon_slices / 2;
+    o.h = *g;
     return MBX_OK;
 }
 
@@ -102,16 +106,23 @@ int make_dev_geom(const mbx_geom *g, DevGeom *d) {
 // ======================================================================
 //
 // One 16-float record per (view slot, subset pixel), computed once per call
-// (or once per VCD subset update) and read by both projectors with scalar
-// loads, so both directions use bit-identical weights:
-//   [0] n_c = rint(n_p) (int bits)   [1..5] Ac(n_c-2 .. n_c+2), zero outside the
-//   tap radius and outside [0, C)    [6] a  [7] W  [8] (W+1)/2  [9] min(1, W)
-//   [10] 1/W  [11] c0  [12] c1        [13] first row [14] last row touched by
-//   any slice (clamped to the detector, int bits)   [15] unused
+// (or once per VCD subset update) in FP64 from the caller's double geometry
+// and read by both projectors with scalar loads, so both directions use
+// bit-identical weights:
+//   [0] n_c = rint(n_p) (int bits)    [1..5] Ac(n_c-2 .. n_c+2), zero outside
+//   the tap radius and outside [0, C)
+//   vertical fan, relative to the reference slice kref = Nz/2 and the integer
+//   row ai = rint(t_kref):  t_k - ai = acr + (W + W_lo) * (k - kref)
+//   [6] acr   [7] W (rounded)   [8] W_lo = W - float(W)   [9] (W+1)/2
+//   [10] min(1, W)   [11] 1/W   [12] u at kref (v/sdd)   [13] du/dk
+//   [14] ai (int bits)   [15] first | last row touched by any slice << 16
 // plus n_c again in a dense (view, pixel) int array for the forward scan.
+// Keeping the row positions relative to ai lets the kernels form t_k - m
+// without rounding at the magnitude of a detector row index.
 
 constexpr int kRec = 16;
-constexpr int kFill = 4;  // back projection: 64-row groups loaded per round
+constexpr int kFill = 4;  // 64-element groups loaded per round (forward staging)
+constexpr int kBackFill = 3;  // 64-row chunks loaded per round (back projection)
 constexpr int64_t kTableBudget = 512ll << 20;
 
 int table_chunk(const DevGeom &g, int npix, int nviews) {
@@ -134,35 +145,85 @@ __global__ __launch_bounds__(256) void pv_table_kernel(DevGeom g, const float *_
     const int vi = blockIdx.y;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= npix) return;
+    const mbx_geom &h = g.h;
     const int va = views ? views[vi] : vi;
-    float sn, cs;
-    sincosf(angles[va], &sn, &cs);
-    const PV q = pv_full<GEOM>(g, pv_base<GEOM>(g, pix[s], cs, sn));
-    const int nc = (int)rintf(q.np);
+    double sn, cs;
+    sincos((double)angles[va], &sn, &cs);
+    // recon_ijk_to_xyz and geometry_xyz_to_uv_mag (mbirjax/cone_beam.py:646-681),
+    // compute_horizontal_data (593-644); parallel: parallel_beam.py:223-284
+    const int p = pix[s];
+    const int i = p / g.cols, j = p - (p / g.cols) * g.cols;
+    const double dv = h.delta_voxel, dc = h.delta_det_channel;
+    const double yt = dv * ((double)i - 0.5 * (h.num_recon_rows - 1));
+    const double xt = dv * ((double)j - 0.5 * (h.num_recon_cols - 1));
+    const double x = cs * xt - sn * yt;
+    double mag = 1.0, np, cosa, ctp = 1.0;
+    if (GEOM == MBX_GEOM_CONE) {
+        const double y = sn * xt + cs * yt;
+        mag = 1.0 / (1.0 / h.magnification - y / h.source_detector_dist);
+        const double u = mag * x;
+        np = (u + h.det_channel_offset) / dc + 0.5 * (h.num_det_channels - 1);
+        const double us = u / h.source_detector_dist;
+        ctp = 1.0 / sqrt(1.0 + us * us);  // cos(theta_p)
+        const double stp = us * ctp;      // sin(theta_p)
+        cosa = fmax(fabs(cs * ctp + sn * stp), fabs(sn * ctp - cs * stp));
+    } else {
+        np = (x + h.det_channel_offset) / dc + 0.5 * (h.num_det_channels - 1);
+        cosa = fmax(fabs(cs), fabs(sn));
+    }
+    const double Wc = (GEOM == MBX_GEOM_CONE ? mag * (cosa / ctp) : cosa) * (dv / dc);
+    const double hc = 0.5 * (Wc + 1.0), Lc = fmin(1.0, Wc), sc = dv / cosa;
+    const int nc = (int)rint(np);
     float r[kRec];
     r[0] = __int_as_float(nc);
 #pragma unroll
-    for (int j = -2; j <= 2; ++j) r[3 + j] = (j >= -RAD && j <= RAD) ? chan_weight(q, nc + j, g.C) : 0.0f;
-    r[6] = q.a;
-    r[7] = q.W;
-    r[8] = q.hv;
-    r[9] = q.Lv;
-    r[10] = q.invW;
-    r[11] = q.c0;
-    r[12] = q.c1;
-    int mlo = 0, mhi = g.R - 1;
-    if (GEOM == MBX_GEOM_CONE) {
-        mlo = max(0, (int)rintf(q.a) - RAD);
-        mhi = min(g.R - 1, (int)rintf(fmaf(q.W, (float)(g.Nz - 1), q.a)) + RAD);
+    for (int t = -2; t <= 2; ++t) {
+        const int n = nc + t;
+        const double L = fmin(fmax(hc - fabs(np - (double)n), 0.0), Lc);
+        r[3 + t] = (t >= -RAD && t <= RAD && n >= 0 && n < g.C) ? (float)(sc * L) : 0.0f;
     }
-    r[13] = __int_as_float(mlo);
-    r[14] = __int_as_float(mhi);
-    r[15] = 0.0f;
+    int mlo = 0, mhi = g.R - 1, ai = g.kref;
+    if (GEOM == MBX_GEOM_CONE) {
+        // compute_vertical_data_single_pixel (mbirjax/cone_beam.py:543-591):
+        // t_k = (mag * z_k + det_row_offset) / dr + (R-1)/2, linear in k
+        const double dr = h.delta_det_row;
+        const double W = mag * dv / dr;
+        const double zref = h.recon_slice_offset + dv * ((double)g.kref - 0.5 * (h.num_recon_slices - 1));
+        const double tref = (mag * zref + h.det_row_offset) / dr + 0.5 * (h.num_det_rows - 1);
+        ai = (int)rint(tref);
+        const float Wf = (float)W;
+        r[6] = (float)(tref - ai);
+        r[7] = Wf;
+        r[8] = (float)(W - (double)Wf);
+        r[9] = (float)(0.5 * (W + 1.0));
+        r[10] = (float)fmin(1.0, W);
+        r[11] = (float)(1.0 / W);
+        r[12] = (float)(mag * zref / h.source_detector_dist);
+        r[13] = (float)(mag * dv / h.source_detector_dist);
+        const double t0 = tref - W * g.kref, t1 = tref + W * (g.Nz - 1 - g.kref);
+        mlo = max(0, (int)rint(t0) - RAD);
+        mhi = min(g.R - 1, (int)rint(t1) + RAD);
+    } else {
+        r[6] = 0.0f;
+        r[7] = 1.0f;
+        r[8] = 0.0f;
+        r[9] = 1.0f;
+        r[10] = 1.0f;
+        r[11] = 1.0f;
+        r[12] = 0.0f;
+        r[13] = 0.0f;
+    }
+    r[14] = __int_as_float(ai);
+    r[15] = __int_as_float(mlo | (mhi << 16));
     float4 *dst = (float4 *)(T + ((size_t)vi * npix + s) * kRec);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) dst[i] = make_float4(r[4 * i], r[4 * i + 1], r[4 * i + 2], r[4 * i + 3]);
+    for (int q = 0; q < 4; ++q) dst[q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
     NC[(size_t)vi * npix + s] = nc;
 }
+
+// Rows [first, last] packed in record word 15.
+__device__ __forceinline__ int rec_mlo(float w) { return __float_as_int(w) & 0xffff; }
+__device__ __forceinline__ int rec_mhi(float w) { return __float_as_int(w) >> 16; }
 
 int launch_table(const DevGeom &g, const float *angles, const int32_t *pix, int npix, const int32_t *views, int nviews,
                  float *T, int *NC, hipStream_t st) {
@@ -202,9 +263,9 @@ BackPlan plan_back(const DevGeom &g, int npix, int nviews) {
     p.nsplit = (waves <= 16384 && nviews >= 8) ? 8 : 1;
     // LDS detector-column capacity per pixel: the rows one slice chunk can
     // reach, from the largest pixel magnification in the recon disc (plus
-    // taps), padded by one load round; at least the detector height + 2 plus a
-    // round (the clamped fallback layout).
-    const int round = kWave * kFill;
+    // taps and margins), padded to whole 64-row chunks; at least the detector
+    // height + 2 plus a chunk (the clamped fallback layout).
+    const int round = kWave;
     p.cap = g.R + 2 + round;
     if (g.kind == MBX_GEOM_CONE) {
         const double rmax = g.dv * std::sqrt((double)g.img_row_center * g.img_row_center +
@@ -247,27 +308,37 @@ struct ColumnSource {
 
 // Vertical fan of one pixel at one view (mbirjax/cone_beam.py:480-541): every
 // lane resamples the LDS detector column onto slices k0 + lane + 64 i.
-template <int KPL, int RAD, int POW, bool KAHAN, bool TAIL>
-__device__ __forceinline__ void back_vertical(const float *col, int shift, bool fit, int n_rows, float a, float W,
-                                              float hv, float Lv, float c0, float c1, float kf0, float klast,
+// FIT: the column holds every row the chunk's taps reach (plus a zero or
+// don't-care margin row each side), so taps index it directly; otherwise the
+// indices are clamped to the stored rows.
+template <int KPL, int RAD, int POW, bool KAHAN, bool TAIL, bool FIT>
+__device__ __forceinline__ void back_vertical(const float *col, int shift, int n_rows, double acd, double Wd,
+                                              float hv, float Lv, float c0, float c1, float kr0, float krlast,
                                               float (&acc)[KPL], float (&cmp)[KPL]) {
+    // Slice k sits at row ai + t with t = acr + W*(k - kref), formed in FP64 so
+    // that the tap distances d = t - rint(t) carry no float32 rounding at the
+    // magnitude of a row index; rint is the 1.5*2^52 shift, whose low word is
+    // the rounded integer.
+    constexpr double kMagic = 6755399441055744.0;
+    const float *colp = col - shift;  // colp[m - RAD] is row ai + m (32-bit LDS address arithmetic)
     const float hm1 = hv - 1.0f;
-    const float t0 = fmaf(W, kf0, a), u0 = fmaf(c1, kf0, c0);
-    const float W64 = W * (float)kWave, c164 = c1 * (float)kWave;
+    const double t0 = fma(Wd, (double)kr0, acd), W64 = Wd * (double)kWave;
+    const float u0 = fmaf(c1, kr0, c0), c164 = c1 * (float)kWave;
 #pragma unroll
     for (int i = 0; i < KPL; ++i) {
-        float t, u;
+        double t;
+        float u;
         if (TAIL) {  // lanes past the last slice repeat it (their sums are never stored)
-            const float kf = fminf(kf0 + (float)(kWave * i), klast);
-            t = fmaf(W, kf, a);
-            u = fmaf(c1, kf, c0);
+            const float kr = fminf(kr0 + (float)(kWave * i), krlast);
+            t = fma(Wd, (double)kr, acd);
+            u = fmaf(c1, kr, c0);
         } else {
-            t = fmaf(W64, (float)i, t0);
+            t = fma(W64, (double)i, t0);
             u = fmaf(c164, (float)i, u0);
         }
-        const float mc = rintf(t);
-        const float d = t - mc;            // |d| <= 1/2
-        const int base = (int)mc - shift;  // buffer index of row mc - RAD
+        const double sh = t + kMagic;
+        const int mi = (int)(uint32_t)__builtin_bit_cast(uint64_t, sh);
+        const float d = (float)(t - (sh - kMagic));  // |d| <= 1/2
         float sum;
         if (RAD == 1) {
             float wm = __builtin_amdgcn_fmed3f(hm1 - d, 0.0f, Lv);        // row mc-1
@@ -279,11 +350,12 @@ __device__ __forceinline__ void back_vertical(const float *col, int shift, bool 
                 wm *= wm;
             }
             float cm, cc, cp;
-            if (fit) {
-                cm = col[base];
-                cc = col[base + 1];
-                cp = col[base + 2];
+            if (FIT) {
+                cm = colp[mi];
+                cc = colp[mi + 1];
+                cp = colp[mi + 2];
             } else {
+                const int base = mi - shift;
                 cm = col[min(max(base, 0), n_rows - 1)];
                 cc = col[min(max(base + 1, 0), n_rows - 1)];
                 cp = col[min(max(base + 2, 0), n_rows - 1)];
@@ -297,8 +369,8 @@ __device__ __forceinline__ void back_vertical(const float *col, int shift, bool 
             for (int j = 0; j <= 2 * RAD; ++j) {
                 float wj = __builtin_amdgcn_fmed3f(hv - fabsf(d - (float)(j - RAD)), 0.0f, Lv);
                 if (POW == 2) wj *= wj;
-                const int idx = fit ? base + j : min(max(base + j, 0), n_rows - 1);
-                sum = fmaf(wj, col[idx], sum);
+                const float cj = FIT ? colp[mi + j] : col[min(max(mi - shift + j, 0), n_rows - 1)];
+                sum = fmaf(wj, cj, sum);
             }
         }
         const float s2 = fmaf(u, u, 1.0f);
@@ -314,10 +386,10 @@ __device__ __forceinline__ void back_vertical(const float *col, int shift, bool 
     }
 }
 
-template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN, bool WGT>
+template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN, bool WGT, bool TAIL>
 __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *__restrict__ rec,
                                                 const ET *__restrict__ sino, const float *__restrict__ w, int vd,
-                                                int k0, int kend, int cap, float *col, int lane, float kf0,
+                                                int k0, int kend, int cap, float *col, int lane,
                                                 float (&acc)[KPL], float (&cmp)[KPL]) {
     const int nc = __float_as_int(rec[0]);
     float Aw[2 * RAD + 1];
@@ -337,49 +409,62 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *_
         src.soff[j] = (src.cv[j] ? n : 0) * g.R * (int)sizeof(ET);
     }
     if (GEOM == MBX_GEOM_CONE) {
-        const float a = rec[6], W = rec[7], hv = rec[8], Lv = rec[9], c0 = rec[11], c1 = rec[12];
-        const int mfirst = (int)rintf(fmaf(W, (float)k0, a)) - RAD;
-        const int mlast = (int)rintf(fmaf(W, (float)(kend - 1), a)) + RAD;
+        const float hv = rec[9], Lv = rec[10], c0 = rec[12], c1 = rec[13];
+        const int ai = __float_as_int(rec[14]);
+        const double Wd = (double)rec[7] + (double)rec[8], acd = (double)rec[6];
+        const int mfirst = ai + (int)rint(fma(Wd, (double)(k0 - g.kref), acd)) - RAD;
+        const int mlast = ai + (int)rint(fma(Wd, (double)(kend - 1 - g.kref), acd)) + RAD;
         const int mlo = max(mfirst, 0), mhi = min(mlast, g.R - 1);
         if (mlo > mhi) return;  // wave-uniform: this view misses the detector
         // One zero row of margin on each side: a lane's slice position may
         // round to one row beyond the chunk's first or last centre row.
         const int count = mlast - mfirst + 3;
-        const bool fit = count + kWave * kFill <= cap;
+        const bool fit = count + kWave <= cap;
         const int mstart = fit ? mfirst - 1 : mlo - 1;
         const int n_rows = fit ? count : mhi - mlo + 3;
-        // ---- detector column col[i] = row mstart + i (zero off the detector)
-        for (int i0 = 0; i0 < n_rows; i0 += kWave * kFill) {
-            float ev[kFill][2 * RAD + 1];
-            bool ok[kFill];
+        // ---- detector column col[i] = row mstart + i, gathered in 64-row
+        // chunks (loads of up to kFill chunks in flight).  Rows are clamped
+        // to the detector for the loads; rows off it are zeroed afterwards.
+        constexpr int FILL = KAHAN ? 2 : kBackFill;  // Kahan needs the registers for its compensation
+        const int nch = (n_rows + kWave - 1) / kWave;
+        for (int c0 = 0; c0 < nch; c0 += FILL) {
+            float ev[FILL][2 * RAD + 1];
 #pragma unroll
-            for (int u = 0; u < kFill; ++u) {
-                const int r = mstart + i0 + u * kWave + lane;
-                ok[u] = (r >= mlo) && (r <= mhi);
-                const int rc = min(max(r, mlo), mhi);
+            for (int u = 0; u < FILL; ++u) {
+                if (c0 + u < nch) {  // wave-uniform
+                    const int r = min(max(mstart + (c0 + u) * kWave + lane, 0), g.R - 1);
+                    const int vo = r * (int)sizeof(ET);
 #pragma unroll
-                for (int j = 0; j <= 2 * RAD; ++j) {
-                    float e = buf_load<ET>(src.rs, rc * (int)sizeof(ET), src.soff[j]);
-                    if (WGT) e *= buf_load<float>(src.rw, rc * 4, src.soff[j] * (int)(4 / sizeof(ET)));
-                    ev[u][j] = e;
+                    for (int j = 0; j <= 2 * RAD; ++j) {
+                        float e = buf_load<ET>(src.rs, vo, src.soff[j]);
+                        if (WGT) e *= buf_load<float>(src.rw, r * 4, src.soff[j] * (int)(4 / sizeof(ET)));
+                        ev[u][j] = e;
+                    }
                 }
             }
 #pragma unroll
-            for (int u = 0; u < kFill; ++u) {
-                float c = 0.0f;
+            for (int u = 0; u < FILL; ++u) {
+                if (c0 + u < nch) {
+                    float c = 0.0f;
 #pragma unroll
-                for (int j = 0; j <= 2 * RAD; ++j) c = fmaf(src.cv[j] ? Aw[j] : 0.0f, ev[u][j], c);
-                col[i0 + u * kWave + lane] = ok[u] ? c : 0.0f;  // cap holds a whole round past the rows
+                    for (int j = 0; j <= 2 * RAD; ++j) c = fmaf(src.cv[j] ? Aw[j] : 0.0f, ev[u][j], c);
+                    col[(c0 + u) * kWave + lane] = c;  // cap holds whole chunks
+                }
             }
         }
+        if (mstart < 0)  // rows above the detector
+            for (int i = lane; i < -mstart; i += kWave) col[i] = 0.0f;
+        if (mstart + n_rows > g.R)  // rows below it
+            for (int i = g.R - mstart + lane; i < n_rows; i += kWave) col[i] = 0.0f;
         wave_sync();
-        const int shift = RAD + mstart;
-        if (kend - k0 == kWave * KPL)  // full chunk: no tail lanes
-            back_vertical<KPL, RAD, POW, KAHAN, false>(col, shift, fit, n_rows, a, W, hv, Lv, c0, c1, kf0,
-                                                       (float)(kend - 1), acc, cmp);
+        const int shift = RAD + mstart - ai;
+        const float kr0 = (float)(k0 + lane - g.kref), krl = (float)(kend - 1 - g.kref);
+        if (fit)
+            back_vertical<KPL, RAD, POW, KAHAN, TAIL, true>(col, shift, n_rows, acd, Wd, hv, Lv, c0, c1, kr0, krl, acc,
+                                                            cmp);
         else
-            back_vertical<KPL, RAD, POW, KAHAN, true>(col, shift, fit, n_rows, a, W, hv, Lv, c0, c1, kf0,
-                                                      (float)(kend - 1), acc, cmp);
+            back_vertical<KPL, RAD, POW, KAHAN, TAIL, false>(col, shift, n_rows, acd, Wd, hv, Lv, c0, c1, kr0, krl,
+                                                             acc, cmp);
         wave_sync();
     } else {
 #pragma unroll
@@ -404,18 +489,19 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *_
     }
 }
 
-template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN, bool WGT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void back_kernel(DevGeom g, const float *__restrict__ T, const ET *__restrict__ sino,
-                                                   const float *__restrict__ w, int npix,
-                                                   const int32_t *__restrict__ views, int nviews, int nsplit,
-                                                   int nchunk, int cap, float scale, int accumulate,
-                                                   float *__restrict__ out, float *__restrict__ part) {
+// TAIL: the launch covers only the last, partial slice chunk (lanes past the
+// last slice repeat it); otherwise every chunk of the launch is full.
+template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN, bool WGT, bool TAIL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void back_kernel(
+    DevGeom g, const float *__restrict__ T, const ET *__restrict__ sino, const float *__restrict__ w, int npix,
+    const int32_t *__restrict__ views, int nviews, int nsplit, int chunk0, int nchunk, int cap, float scale,
+    int accumulate, float *__restrict__ out, float *__restrict__ part) {
     extern __shared__ float lds[];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int split = blockIdx.x % nsplit;
     const int grp = blockIdx.x / nsplit;
-    const int chunk = grp % nchunk;
+    const int chunk = chunk0 + grp % nchunk;
     const int q = __builtin_amdgcn_readfirstlane((grp / nchunk) * 4 + wave);
     const int sA = 2 * q;
     if (sA >= npix) return;  // whole wavefront leaves; no block barriers below
@@ -431,16 +517,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
     float accA[KPL], cmpA[KPL], accB[KPL], cmpB[KPL];
 #pragma unroll
     for (int i = 0; i < KPL; ++i) accA[i] = cmpA[i] = accB[i] = cmpB[i] = 0.0f;
-    const float kf0 = (float)(k0 + lane);
     for (int vi = vb; vi < ve; ++vi) {
         const int vd = views ? views[vi] : vi;
         const float *recA = T + ((size_t)vi * npix + sA) * kRec;
         const float *recB = T + ((size_t)vi * npix + sB) * kRec;
-        back_pixel_view<GEOM, KPL, RAD, ET, POW, KAHAN, WGT>(g, recA, sino, w, vd, k0, kend, cap, colA, lane,
-                                                             kf0, accA, cmpA);
+        back_pixel_view<GEOM, KPL, RAD, ET, POW, KAHAN, WGT, TAIL>(g, recA, sino, w, vd, k0, kend, cap, colA, lane,
+                                                             accA, cmpA);
         if (hasB)
-            back_pixel_view<GEOM, KPL, RAD, ET, POW, KAHAN, WGT>(g, recB, sino, w, vd, k0, kend, cap, colB,
-                                                                 lane, kf0, accB, cmpB);
+            back_pixel_view<GEOM, KPL, RAD, ET, POW, KAHAN, WGT, TAIL>(g, recB, sino, w, vd, k0, kend, cap, colB,
+                                                                 lane, accB, cmpB);
     }
 #pragma unroll
     for (int i = 0; i < KPL; ++i) {
@@ -474,21 +559,42 @@ __global__ __launch_bounds__(256) void back_reduce_kernel(const float *__restric
     }
 }
 
+template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN, bool WGT, bool TAIL>
+static void launch_back_range(const DevGeom &g, const BackPlan &pl, const float *T, const ET *sino, const float *w,
+                              int npix, const int32_t *views, int nviews, int chunk0, int nch, float scale,
+                              int accumulate, float *out, float *part, hipStream_t st) {
+    const int64_t octets = ((int64_t)npix + 7) / 8;  // 4 waves x 2 pixels per block
+    const int64_t blocks = octets * nch * pl.nsplit;
+    const size_t lds = GEOM == MBX_GEOM_CONE ? 4 * 2 * (size_t)pl.cap * sizeof(float) : 0;
+    hipLaunchKernelGGL((back_kernel<GEOM, KPL, RAD, ET, POW, KAHAN, WGT, TAIL>), dim3((unsigned)blocks), dim3(256), lds,
+                       st, g, T, sino, w, npix, views, nviews, pl.nsplit, chunk0, nch, pl.cap, scale, accumulate, out,
+                       part);
+}
+
+// Full slice chunks in one launch, a partial last chunk in a second one.
+template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN, bool WGT>
+static void launch_back_w(const DevGeom &g, const BackPlan &pl, const float *T, const ET *sino, const float *w,
+                          int npix, const int32_t *views, int nviews, float scale, int accumulate, float *out,
+                          float *part, hipStream_t st) {
+    const int full = g.Nz / (kWave * KPL);
+    if (full > 0)
+        launch_back_range<GEOM, KPL, RAD, ET, POW, KAHAN, WGT, false>(g, pl, T, sino, w, npix, views, nviews, 0, full,
+                                                                      scale, accumulate, out, part, st);
+    if (full < pl.nchunk)
+        launch_back_range<GEOM, KPL, RAD, ET, POW, KAHAN, WGT, true>(g, pl, T, sino, w, npix, views, nviews, full, 1,
+                                                                     scale, accumulate, out, part, st);
+}
+
 template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN>
 static void launch_back_t(const DevGeom &g, const BackPlan &pl, const float *T, const ET *sino, const float *w,
                           int npix, const int32_t *views, int nviews, float scale, int accumulate, float *out,
                           float *part, hipStream_t st) {
-    const int64_t octets = ((int64_t)npix + 7) / 8;  // 4 waves x 2 pixels per block
-    const int64_t blocks = octets * pl.nchunk * pl.nsplit;
-    const size_t lds = GEOM == MBX_GEOM_CONE ? 4 * 2 * (size_t)pl.cap * sizeof(float) : 0;
     if (w)
-        hipLaunchKernelGGL((back_kernel<GEOM, KPL, RAD, ET, POW, KAHAN, true>), dim3((unsigned)blocks), dim3(256), lds,
-                           st, g, T, sino, w, npix, views, nviews, pl.nsplit, pl.nchunk, pl.cap, scale, accumulate,
-                           out, part);
+        launch_back_w<GEOM, KPL, RAD, ET, POW, KAHAN, true>(g, pl, T, sino, w, npix, views, nviews, scale, accumulate,
+                                                            out, part, st);
     else
-        hipLaunchKernelGGL((back_kernel<GEOM, KPL, RAD, ET, POW, KAHAN, false>), dim3((unsigned)blocks), dim3(256),
-                           lds, st, g, T, sino, w, npix, views, nviews, pl.nsplit, pl.nchunk, pl.cap, scale,
-                           accumulate, out, part);
+        launch_back_w<GEOM, KPL, RAD, ET, POW, KAHAN, false>(g, pl, T, sino, w, npix, views, nviews, scale, accumulate,
+                                                             out, part, st);
 }
 
 template <int GEOM, int KPL, int RAD>
@@ -699,7 +805,7 @@ __global__ __launch_bounds__(512) void fwd_kernel(DevGeom g, const float *__rest
             flag = (nc + RAD >= n0) && (nc - RAD <= n1 - 1);
             if (GEOM == MBX_GEOM_CONE && flag && nrb > 1) {
                 const float *rec = Tv + (size_t)s * kRec;
-                flag = (__float_as_int(rec[13]) <= m1 - 1) && (__float_as_int(rec[14]) >= m0);
+                flag = (rec_mlo(rec[15]) <= m1 - 1) && (rec_mhi(rec[15]) >= m0);
             }
         }
         const unsigned long long bal = __ballot(flag);
@@ -728,13 +834,14 @@ __global__ __launch_bounds__(512) void fwd_kernel(DevGeom g, const float *__rest
                 int kf, span;
                 float c0 = 0.0f, c1 = 0.0f;
                 if (GEOM == MBX_GEOM_CONE) {
-                    const float a = rec[6], invW = rec[10];
-                    c0 = rec[11];
-                    c1 = rec[12];
-                    const int rlo = max(m0, __float_as_int(rec[13]));
-                    const int rhi = min(m1 - 1, __float_as_int(rec[14]));
-                    kf = max((int)rintf(((float)rlo - a) * invW) - RB, 0);
-                    const int kl = min((int)rintf(((float)rhi - a) * invW) + RB, g.Nz - 1);
+                    const float acr = rec[6], invW = rec[11];
+                    c0 = rec[12];
+                    c1 = rec[13];
+                    const int ai = __float_as_int(rec[14]);
+                    const int rlo = max(m0, rec_mlo(rec[15]));
+                    const int rhi = min(m1 - 1, rec_mhi(rec[15]));
+                    kf = max(g.kref + (int)rintf(((float)(rlo - ai) - acr) * invW) - RB, 0);
+                    const int kl = min(g.kref + (int)rintf(((float)(rhi - ai) - acr) * invW) + RB, g.Nz - 1);
                     span = (rlo <= rhi && kf <= kl) ? kl - kf + 1 : 0;
                 } else {
                     kf = m0;
@@ -763,7 +870,7 @@ __global__ __launch_bounds__(512) void fwd_kernel(DevGeom g, const float *__rest
                             const int kk = q0 + u * kWave + lane;  // a round past the slices fits in cstride
                             float v = xv[u];
                             if (GEOM == MBX_GEOM_CONE) {
-                                const float uu = fmaf(c1, (float)(kf + kk), c0);
+                                const float uu = fmaf(c1, (float)(kf + kk - g.kref), c0);
                                 v *= __builtin_amdgcn_sqrtf(fmaf(uu, uu, 1.0f));
                             }
                             cyl[P + kk] = kk < span ? v : 0.0f;
@@ -789,8 +896,7 @@ __global__ __launch_bounds__(512) void fwd_kernel(DevGeom g, const float *__rest
                     const int sc = __builtin_amdgcn_readlane(l_sc, slot);
                     const float4 *r4 = (const float4 *)(Tv + (size_t)sc * kRec);
                     const float4 R1 = r4[1], R2 = r4[2], R3 = r4[3];
-                    if (GEOM == MBX_GEOM_CONE &&
-                        (__float_as_int(R3.y) > wm0 + kWave - 1 || __float_as_int(R3.z) < wm0))
+                    if (GEOM == MBX_GEOM_CONE && (rec_mlo(R3.w) > wm0 + kWave - 1 || rec_mhi(R3.w) < wm0))
                         continue;  // no row of this slab is reached
                     const float4 wA = *(const float4 *)(slot_w + slot * CB);
                     const float4 wB = *(const float4 *)(slot_w + slot * CB + 4);
@@ -798,11 +904,13 @@ __global__ __launch_bounds__(512) void fwd_kernel(DevGeom g, const float *__rest
                     const float *cyl = cyl_all + (size_t)slot * cstride;
                     float v;
                     if (GEOM == MBX_GEOM_CONE) {
-                        const float a = R1.z, W = R1.w, hv = R2.x, Lv = R2.y, invW = R2.z;
-                        const float kc = rintf(fmaf(mf, invW, -(a * invW)));
-                        const float dm = fmaf(W, kc, a) - mf;  // t_kc - m
+                        const float acr = R1.z, W = R1.w, Wlo = R2.x, hv = R2.y, Lv = R2.z, invW = R2.w;
+                        const float mp = mf - (float)__float_as_int(R3.z);  // m - ai, exact
+                        const float kr = rintf(fmaf(mp, invW, -(acr * invW)));  // nearest slice - kref
+                        // t - m for slice kref + kr: the first fma is rounded once, at |t - m| < 1
+                        const float dm = fmaf(W, kr, -mp) + fmaf(Wlo, kr, acr);
                         // taps past the staged slices read the zero pads
-                        const int off = min(max((int)kc + (P - kf), P - RB - 1), P + span + RB);
+                        const int off = min(max((int)kr + (g.kref + P - kf), P - RB - 1), P + span + RB);
                         v = 0.0f;
 #pragma unroll
                         for (int j = -RB; j <= RB; ++j) {

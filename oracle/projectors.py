@@ -73,7 +73,10 @@ class Geometry:
                  det_row_offset=0.0, delta_voxel=None, recon_slice_offset=0.0):
         self.kind = kind
         self.sinogram_shape = tuple(int(v) for v in sinogram_shape)
-        self.angles = np.asarray(angles, dtype=np.float64)
+        # The reference holds the angles as a float32 JAX array (jnp.asarray,
+        # mbirjax/cone_beam.py:56, parallel_beam.py:50): evaluate the geometry
+        # in FP64 at those float32 values.
+        self.angles = np.asarray(angles, dtype=np.float64).astype(np.float32).astype(np.float64)
         self.delta_det_channel = float(delta_det_channel)
         self.delta_det_row = float(delta_det_row)
         self.det_channel_offset = float(det_channel_offset)

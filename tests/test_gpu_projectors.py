@@ -53,6 +53,10 @@ CASES = {
     "cone_tall": lambda: make_cone(8, 150, 20, 160.0, 80.0),
     "parallel": lambda: make_par(20, 12, 30),
     "parallel_offset_tall": lambda: make_par(9, 130, 17, det_channel_offset=0.6),
+    # more than 1024 slices: several slice chunks plus a partial last one in
+    # the back projector, several row blocks in the forward projector
+    "parallel_multichunk": lambda: make_par(6, 1100, 10),
+    "cone_multichunk": lambda: make_cone(6, 1300, 12, 2600.0, 1300.0, recon_shape=(8, 8, 1200)),
 }
 
 
@@ -108,6 +112,29 @@ def test_benchmark_c2_full_projections(gpu):
     b = m.sparse_back_project(ref.astype(np.float32), full)
     rb = rel(b, op.back(g, ref.astype(np.float32), full))
     print("C2 back rel-L2 %.3e" % rb, flush=True)
+    assert rb <= TOL
+
+
+def test_benchmark_c4_subset_parity(gpu):
+    """The 512^3 benchmark geometry (720 views x 512 x 768, sdd 3072, sid 1536,
+    recon 512^3) restricted to every 10th view: forward and back projection of
+    one K=128 subset (1602 pixels) against the FP64 oracle."""
+    ang = np.linspace(0, 2 * np.pi, 720, endpoint=False)[::10]
+    m = mj.ConeBeamModel((72, 512, 768), ang, source_detector_dist=3072.0, source_iso_dist=1536.0)
+    m.set_params(recon_shape=(512, 512, 512))
+    g = op.Geometry("cone", (72, 512, 768), ang, 3072.0, 1536.0, recon_shape=(512, 512, 512),
+                    delta_voxel=m.get_params('delta_voxel'))
+    np.random.seed(0)
+    pix = mj.gen_pixel_partition((512, 512, 512), 128)[0]
+    assert pix.size == 1602
+    rng = np.random.default_rng(21)
+    x = rng.random((pix.size, 512)).astype(np.float32)
+    r = rel(m.sparse_forward_project(x, pix), op.forward(g, x, pix))
+    print("C4 subset forward rel-L2 %.3e" % r, flush=True)
+    assert r <= TOL
+    y = rng.random((72, 512, 768)).astype(np.float32)
+    rb = rel(m.sparse_back_project(y, pix), op.back(g, y, pix))
+    print("C4 subset back rel-L2 %.3e" % rb, flush=True)
     assert rb <= TOL
 
 

@@ -135,8 +135,9 @@ def test_parallel_closed_form_axis_sums():
     s = op.forward(geom, x.reshape(-1, nz), np.arange(rows * cols))
     # view 0: channel n = column j, sum over rows i
     assert np.allclose(s[0], x.sum(axis=0).T * geom.delta_voxel, rtol=1e-12)
-    # view 90 deg: channel n = C-1-i, sum over columns j
-    assert np.allclose(s[1], x.sum(axis=1)[::-1].T * geom.delta_voxel, rtol=1e-9)
+    # view 90 deg: channel n = C-1-i, sum over columns j (the float32 angle of
+    # pi/2 is off by 4.4e-8 rad, which moves the taps by ~1e-7)
+    assert np.allclose(s[1], x.sum(axis=1)[::-1].T * geom.delta_voxel, rtol=1e-6)
 
 
 def test_cone_far_source_equals_parallel():
