@@ -70,6 +70,8 @@ struct FwdPlan {
 };
 FwdPlan plan_fwd(const DevGeom &g, int nviews);
 int64_t fwd_workspace_bytes(const DevGeom &g, int npix, int nviews);
+// Forward scratch besides the geometry table (partials, binned pixel lists).
+int64_t fwd_aux_bytes(const DevGeom &g, int chunk, int nviews);
 
 // Launchers used by the C ABI and by the VCD subset update.  All arguments are
 // already validated.  launch_back / launch_fwd build the geometry table in
@@ -85,8 +87,8 @@ int launch_back_tab(const DevGeom &g, const float *T, int npix, const void *sino
                     const float *weights, const int32_t *views, int nviews, int coeff_power, int kahan, float scale,
                     int accumulate, float *out, float *part, hipStream_t st);
 int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float *vox, int npix, int nviews, void *sino,
-                   int sino_dtype, int layout, int accumulate, const float *weights, double *part, double *sumsq,
-                   hipStream_t st);
+                   int sino_dtype, int layout, int accumulate, const float *weights, void *aux, int chunk,
+                   double *sumsq, hipStream_t st);
 // Sum `n` doubles in fixed order into out[0] (one block).
 int launch_sum_partials(const double *partials, int64_t n, double *out, int accumulate, hipStream_t st);
 

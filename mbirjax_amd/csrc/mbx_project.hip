@@ -703,36 +703,48 @@ int launch_back(const DevGeom &g, const float *angles, const void *sino, int sin
 // Forward projection
 // ======================================================================
 
-constexpr int kFwdCB = 8;         // channels per block: the per-lane register tile
+constexpr int kFwdCB = 16;        // channels per block: the per-lane register tile
+constexpr int kFwdRPL = 2;        // detector rows per lane (64 apart)
 constexpr int kFwdPad = 5;        // zero slices either side of a staged cylinder (>= 2*bp_psf_radius + 1)
 constexpr int kFwdMaxBatch = 16;  // cylinders staged per batch
-constexpr int kFwdList = 1024;    // candidate list capacity
-static_assert(kFwdCB == 8, "the forward register tile is read as two float4 weight vectors");
+constexpr size_t kFwdLds = 36 * 1024;  // both staging buffers of a block
+constexpr int kBinMax = 16384;    // channel buckets the binning kernel holds in LDS
 
 FwdPlan plan_fwd(const DevGeom &g, int nviews) {
     FwdPlan p;
-    p.nw = (std::min(g.R, 512) + kWave - 1) / kWave;
-    p.rows_pb = p.nw * kWave;
+    const int slab = kWave * kFwdRPL;  // rows per wave
+    p.nw = (std::min(g.R, 4 * slab) + slab - 1) / slab;
+    p.rows_pb = p.nw * slab;
     p.nrb = (g.R + p.rows_pb - 1) / p.rows_pb;
     p.cb = kFwdCB;
     p.ncb = (g.C + p.cb - 1) / p.cb;
-    p.list = kFwdList;
+    p.list = 0;
     // A staged cylinder holds the slices the block's rows reach (at most all
-    // slices), one load round past them and the zero pads.
+    // slices) in whole 64-slice chunks, and the zero pads.
     const int span_max = g.kind == MBX_GEOM_CONE ? g.Nz : p.rows_pb;
-    p.cstride = ((span_max + kWave * kFill - 1) / (kWave * kFill)) * (kWave * kFill) + 2 * kFwdPad + 6;
-    const size_t budget = 40 * 1024;
-    p.nb = (int)std::max<size_t>(1, std::min<size_t>(kFwdMaxBatch, budget / ((size_t)p.cstride * sizeof(float))));
-    p.lds = (size_t)p.nb * p.cstride * sizeof(float) +
-            (size_t)(p.list + (2 + kFwdCB) * kFwdMaxBatch + 32) * sizeof(int);
+    p.cstride = ((span_max + kWave - 1) / kWave) * kWave + 2 * kFwdPad + 6;
+    p.nb = (int)std::max<size_t>(1, std::min<size_t>(kFwdMaxBatch, kFwdLds / (2 * (size_t)p.cstride * sizeof(float))));
+    p.lds = 2 * (size_t)p.nb * p.cstride * sizeof(float) + 64 * sizeof(int);
     p.nblocks = (int64_t)nviews * p.ncb * p.nrb;
     return p;
 }
 
+static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+static int fwd_nbk(const DevGeom &g) { return g.C + 2 * g.rh; }  // centre-channel buckets n_c + r
+
+// Forward scratch after the geometry table: per-block FP64 partials of the
+// squared norm, then per view the pixel list sorted by centre channel and
+// the bucket starts.
+int64_t fwd_aux_bytes(const DevGeom &g, int chunk, int nviews) {
+    return (int64_t)(al256((size_t)plan_fwd(g, nviews).nblocks * sizeof(double)) +
+                     al256((size_t)nviews * chunk * sizeof(int)) +
+                     al256((size_t)nviews * (fwd_nbk(g) + 1) * sizeof(int)));
+}
+
 int64_t fwd_workspace_bytes(const DevGeom &g, int npix, int nviews) {
-    if (npix == 0 || nviews == 0) return 0;
-    const int chunk = table_chunk(g, npix, nviews);
-    return table_bytes(chunk, nviews) + ((plan_fwd(g, nviews).nblocks * (int64_t)sizeof(double) + 255) & ~255ll);
+    if (nviews == 0) return 0;  // an empty pixel set still writes the views (zeros or unchanged)
+    const int chunk = table_chunk(g, std::max(npix, 1), nviews);
+    return table_bytes(chunk, nviews) + fwd_aux_bytes(g, chunk, nviews);
 }
 
 template <typename T>
@@ -748,34 +760,126 @@ __device__ __forceinline__ T block_sum(T v, T *scratch) {
     return r;  // valid in thread 0
 }
 
-// One workgroup per (view, 8-channel block, row block); wave w owns the 64
-// detector rows m0 + 64w + lane and keeps their 8 channels in registers.  The
-// subset pixels whose channel taps meet the block are compacted in pixel
-// order and handled in batches:
-//   staging: one wave per pixel loads the slices of its cylinder that the
-//            block's rows reach, scaled by 1/cos(phi_k), into LDS with zero
-//            pads;
-//   update:  every wave resamples each staged cylinder onto its rows
-//            (row-driven vertical fan, mbirjax/cone_beam.py:347-372) and adds
-//            the horizontal taps (mbirjax/cone_beam.py:276-282) to its
-//            register tile, in batch order.
+// Per view: the subset pixels sorted by centre channel n_c (bucket n_c + r),
+// ascending pixel slot within a bucket, and the bucket starts.  Pixels whose
+// taps miss the detector are dropped.  Counting sort with LDS counters, then
+// every bucket is put in pixel order, so the result (and the forward
+// summation order) does not depend on atomic ordering.
+__global__ __launch_bounds__(256) void fwd_bin_kernel(const int *__restrict__ NC, int npix, int rad, int nbk,
+                                                      int *__restrict__ list, int *__restrict__ cstart) {
+    extern __shared__ int cnt[];  // [nbk]
+    const int v = blockIdx.x, tid = threadIdx.x;
+    const int *ncv = NC + (size_t)v * npix;
+    int *lv = list + (size_t)v * npix;
+    int *cs = cstart + (size_t)v * (nbk + 1);
+    for (int b = tid; b < nbk; b += blockDim.x) cnt[b] = 0;
+    __syncthreads();
+    for (int s = tid; s < npix; s += blockDim.x) {
+        const int b = ncv[s] + rad;
+        if (b >= 0 && b < nbk) atomicAdd(&cnt[b], 1);
+    }
+    __syncthreads();
+    // exclusive scan: each thread owns a contiguous run of buckets
+    const int per = (nbk + blockDim.x - 1) / blockDim.x;
+    const int b0 = tid * per, b1 = min(nbk, b0 + per);
+    int run = 0;
+    for (int b = b0; b < b1; ++b) run += cnt[b];
+    __shared__ int wsum[4];
+    int incl = run;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if ((tid & 63) >= o) incl += y;
+    }
+    if ((tid & 63) == 63) wsum[tid >> 6] = incl;
+    __syncthreads();
+    int base = incl - run;
+    for (int w = 0; w < (tid >> 6); ++w) base += wsum[w];
+    for (int b = b0; b < b1; ++b) {
+        const int c = cnt[b];
+        cs[b] = base;
+        cnt[b] = base;  // placement cursor
+        base += c;
+    }
+    if (tid == blockDim.x - 1) cs[nbk] = base;
+    __syncthreads();
+    for (int s = tid; s < npix; s += blockDim.x) {
+        const int b = ncv[s] + rad;
+        if (b >= 0 && b < nbk) lv[atomicAdd(&cnt[b], 1)] = s;
+    }
+    __syncthreads();
+    __threadfence_block();
+    for (int b = tid; b < nbk; b += blockDim.x) {  // insertion sort of each bucket
+        const int lo = cs[b], hi = cs[b + 1];
+        for (int i = lo + 1; i < hi; ++i) {
+            const int x = lv[i];
+            int j = i - 1;
+            while (j >= lo && lv[j] > x) {
+                lv[j + 1] = lv[j];
+                --j;
+            }
+            lv[j + 1] = x;
+        }
+    }
+}
+
+// Staging metadata of one candidate pixel for this block.
+struct FwdSlot {
+    int sc, kf, span;
+    float c0, c1;
+};
+
+template <int GEOM, int RB>
+__device__ __forceinline__ FwdSlot fwd_slot(const DevGeom &g, const float *__restrict__ Tv, int sc, int m0, int m1) {
+    FwdSlot q;
+    q.sc = sc;
+    q.c0 = 0.0f;
+    q.c1 = 0.0f;
+    if (GEOM == MBX_GEOM_CONE) {
+        const float *rec = Tv + (size_t)sc * kRec;
+        const float acr = rec[6], invW = rec[11];
+        q.c0 = rec[12];
+        q.c1 = rec[13];
+        const int ai = __float_as_int(rec[14]);
+        const int rlo = max(m0, rec_mlo(rec[15]));
+        const int rhi = min(m1 - 1, rec_mhi(rec[15]));
+        q.kf = max(g.kref + (int)rintf(((float)(rlo - ai) - acr) * invW) - RB, 0);
+        const int kl = min(g.kref + (int)rintf(((float)(rhi - ai) - acr) * invW) + RB, g.Nz - 1);
+        q.span = (rlo <= rhi && q.kf <= kl) ? kl - q.kf + 1 : 0;
+    } else {
+        q.kf = m0;
+        q.span = m1 - m0;
+    }
+    return q;
+}
+
+// One workgroup per (view, 16-channel block, row block); lane l of wave w owns
+// detector rows m0 + 128w + l and m0 + 128w + 64 + l and keeps their 16
+// channels in registers.  The block's candidate pixels are a contiguous run
+// of the view's binned list (sorted by centre channel, then pixel) and are
+// handled in batches, double-buffered in LDS:
+//   staging: the slices of each candidate's cylinder that the block's rows
+//            reach are copied global -> LDS asynchronously (buffer loads into
+//            LDS) while the previous batch is being used, then scaled in place
+//            by 1/cos(phi_k) with zero pads;
+//   update:  for each centre channel (a compile-time loop, so every tap lands
+//            in a fixed register) every wave resamples that channel's staged
+//            cylinders onto its rows (row-driven vertical fan,
+//            mbirjax/cone_beam.py:347-372), weights them with the horizontal
+//            taps (mbirjax/cone_beam.py:276-282) and adds them to its tile.
 // Each output element has one writer and a fixed summation order; the tile is
 // written to HBM once.
 template <int GEOM, int RAD, int RB, typename OT, int LAYOUT>
-__global__ __launch_bounds__(512) void fwd_kernel(DevGeom g, const float *__restrict__ T, const int *__restrict__ NC,
+__global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__restrict__ T,
+                                                  const int *__restrict__ list, const int *__restrict__ cstart,
                                                   const float *__restrict__ vox, int npix, int ncb, int nrb,
-                                                  int nbatch, int cstride, int list_cap, OT *__restrict__ sino,
-                                                  int accumulate, const float *__restrict__ w,
-                                                  double *__restrict__ part_sumsq) {
-    constexpr int CB = kFwdCB, P = kFwdPad;
+                                                  int nbatch, int cstride, OT *__restrict__ sino, int accumulate,
+                                                  const float *__restrict__ w, double *__restrict__ part_sumsq) {
+    constexpr int CB = kFwdCB, P = kFwdPad, RPL = kFwdRPL;
+    constexpr int ND = CB + 2 * RAD;  // centre-channel buckets n0 - RAD .. n0 + CB - 1 + RAD
     extern __shared__ float lds[];
     const int nw = blockDim.x >> 6;
-    float *cyl_all = lds;                                     // [nbatch][cstride]
-    int *l_s = (int *)(cyl_all + (size_t)nbatch * cstride);  // [list_cap]
-    int *slot_kf = l_s + list_cap;                            // [kFwdMaxBatch]
-    int *slot_span = slot_kf + kFwdMaxBatch;                  // [kFwdMaxBatch]
-    float *slot_w = (float *)(slot_span + kFwdMaxBatch);      // [kFwdMaxBatch][CB] channel weights
-    int *scan = (int *)(slot_w + kFwdMaxBatch * CB);          // [32]: wave counts; doubles at +8
+    float *bufs = lds;                                            // [2][nbatch][cstride]
+    double *dscratch = (double *)(lds + 2 * (size_t)nbatch * cstride);  // [8]
 
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
@@ -784,237 +888,270 @@ __global__ __launch_bounds__(512) void fwd_kernel(DevGeom g, const float *__rest
     const int rem = b - vi * (ncb * nrb);
     const int cbi = rem / nrb, rbi = rem - (rem / nrb) * nrb;
     const int n0 = cbi * CB, n1 = min(g.C, n0 + CB);
-    const int rows_pb = nw * kWave;
+    const int rows_pb = nw * kWave * RPL;
     const int m0 = rbi * rows_pb, m1 = min(g.R, m0 + rows_pb);
-    const int wm0 = m0 + wave * kWave;  // this wave's slab
-    const int m = wm0 + lane;           // this lane's row
-    const float mf = (float)m;
-    const int *ncv = NC + (size_t)vi * npix;
+    const int wm0 = m0 + wave * kWave * RPL;  // this wave's rows: wm0 + lane + 64 r
     const float *Tv = T + (size_t)vi * npix * kRec;
+    const int nbk = g.C + 2 * RAD;
+    const int *lv = list + (size_t)vi * npix;
+    const int *csv = cstart + (size_t)vi * (nbk + 1);
+    // bucket starts of this block: lane D holds the start of centre channel n0 + D - RAD
+    const int l_bs = lane <= ND ? csv[min(n0 + lane, nbk)] : 0;
+    const int lo = __builtin_amdgcn_readlane(l_bs, 0);
+    const int hi = __builtin_amdgcn_readlane(l_bs, ND);
+    const int nbat = (hi - lo + nbatch - 1) / nbatch;
 
-    float acc[CB];
+    float acc[RPL][CB];
 #pragma unroll
-    for (int c = 0; c < CB; ++c) acc[c] = 0.0f;
+    for (int r = 0; r < RPL; ++r)
+#pragma unroll
+        for (int c = 0; c < CB; ++c) acc[r][c] = 0.0f;
 
-    int count = 0;
-    for (int base = 0; base < npix; base += blockDim.x) {
-        const int s = base + tid;
-        bool flag = false;
-        if (s < npix) {
-            const int nc = ncv[s];
-            flag = (nc + RAD >= n0) && (nc - RAD <= n1 - 1);
-            if (GEOM == MBX_GEOM_CONE && flag && nrb > 1) {
-                const float *rec = Tv + (size_t)s * kRec;
-                flag = (rec_mlo(rec[15]) <= m1 - 1) && (rec_mhi(rec[15]) >= m0);
-            }
+    // lane i of every wave holds slot i's staging data for the batch in use
+    // (cur) and the one being loaded (nxt)
+    FwdSlot cur{0, 0, 0, 0.0f, 0.0f}, nxt{0, 0, 0, 0.0f, 0.0f};
+    auto load_slots = [&](int bi) {
+        FwdSlot q{0, 0, 0, 0.0f, 0.0f};
+        const int i = lo + bi * nbatch + lane;
+        if (lane < nbatch && i < hi) q = fwd_slot<GEOM, RB>(g, Tv, lv[i], m0, m1);
+        return q;
+    };
+    // asynchronous global -> LDS copy of the raw slices of this wave's slots
+    auto issue = [&](const FwdSlot &q, int bi) {
+        float *buf = bufs + (size_t)(bi & 1) * nbatch * cstride;
+        const int nb = min(nbatch, hi - lo - bi * nbatch);
+        for (int slot = wave; slot < nb; slot += nw) {
+            const int span = __builtin_amdgcn_readlane(q.span, slot);
+            if (span == 0) continue;
+            const int kf = __builtin_amdgcn_readlane(q.kf, slot);
+            const int sc = __builtin_amdgcn_readlane(q.sc, slot);
+            // slices past the cylinder end read 0 (num_records bounds the buffer)
+            const __amdgpu_buffer_rsrc_t rs =
+                __builtin_amdgcn_make_buffer_rsrc((void *)(vox + (size_t)sc * g.Nz + kf), 0, (g.Nz - kf) * 4,
+                                                  0x00020000);
+            float *dst = buf + (size_t)slot * cstride + P;
+            const int nch = (span + kWave - 1) / kWave;
+            for (int c = 0; c < nch; ++c)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rs, (__attribute__((address_space(3))) void *)(dst + c * kWave), 4, (c * kWave + lane) * 4, 0, 0,
+                    0);
         }
-        const unsigned long long bal = __ballot(flag);
-        const int wcount = __popcll(bal);
-        const int lpre = __popcll(bal & ((1ull << lane) - 1ull));
-        if (lane == 0) scan[wave] = wcount;
-        __syncthreads();
-        int woff = 0, total = 0;
-        for (int i = 0; i < nw; ++i) {
-            const int c = scan[i];
-            if (i < wave) woff += c;
-            total += c;
-        }
-        if (flag) l_s[count + woff + lpre] = s;
-        count += total;
-        __syncthreads();
-        if (!(count > list_cap - (int)blockDim.x || base + (int)blockDim.x >= npix)) continue;
-
-        for (int b0 = 0; b0 < count; b0 += nbatch) {
-            const int nb = min(nbatch, count - b0);
-            // ---- staging: one wave per cylinder
-            for (int slot = wave; slot < nb; slot += nw) {
-                const int sc = __builtin_amdgcn_readfirstlane(l_s[b0 + slot]);
-                const float *rec = Tv + (size_t)sc * kRec;
-                float *cyl = cyl_all + (size_t)slot * cstride;
-                int kf, span;
-                float c0 = 0.0f, c1 = 0.0f;
-                if (GEOM == MBX_GEOM_CONE) {
-                    const float acr = rec[6], invW = rec[11];
-                    c0 = rec[12];
-                    c1 = rec[13];
-                    const int ai = __float_as_int(rec[14]);
-                    const int rlo = max(m0, rec_mlo(rec[15]));
-                    const int rhi = min(m1 - 1, rec_mhi(rec[15]));
-                    kf = max(g.kref + (int)rintf(((float)(rlo - ai) - acr) * invW) - RB, 0);
-                    const int kl = min(g.kref + (int)rintf(((float)(rhi - ai) - acr) * invW) + RB, g.Nz - 1);
-                    span = (rlo <= rhi && kf <= kl) ? kl - kf + 1 : 0;
-                } else {
-                    kf = m0;
-                    span = m1 - m0;
+    };
+    // in-place scaling by 1/cos(phi_k), zero past the span, zero pads
+    auto finish = [&](const FwdSlot &q, int bi) {
+        float *buf = bufs + (size_t)(bi & 1) * nbatch * cstride;
+        const int nb = min(nbatch, hi - lo - bi * nbatch);
+        for (int slot = wave; slot < nb; slot += nw) {
+            const int span = __builtin_amdgcn_readlane(q.span, slot);
+            if (span == 0) continue;
+            float *cyl = buf + (size_t)slot * cstride;
+            if (GEOM == MBX_GEOM_CONE) {
+                const int kf = __builtin_amdgcn_readlane(q.kf, slot);
+                const float c0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, q.c0), slot));
+                const float c1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, q.c1), slot));
+                const int nch = (span + kWave - 1) / kWave;
+                for (int c = 0; c < nch; ++c) {
+                    const int kk = c * kWave + lane;
+                    const float uu = fmaf(c1, (float)(kf + kk - g.kref), c0);
+                    const float x = cyl[P + kk] * __builtin_amdgcn_sqrtf(fmaf(uu, uu, 1.0f));
+                    cyl[P + kk] = kk < span ? x : 0.0f;
                 }
-                if (lane == 0) {
-                    slot_kf[slot] = kf;
-                    slot_span[slot] = span;
-                }
-                if (lane < CB) {  // weight of channel n0 + lane: Ac(n_c + j) for |j| <= RAD, else 0
-                    const int j = n0 + lane - __float_as_int(rec[0]);
-                    const float aw = rec[3 + min(max(j, -RAD), RAD)];
-                    slot_w[slot * CB + lane] = (j >= -RAD && j <= RAD) ? aw : 0.0f;
-                }
-                if (span > 0) {  // wave-uniform
-                    const i32x4_t rs = make_rsrc(vox + (size_t)sc * g.Nz + kf);
-                    for (int q0 = 0; q0 < span; q0 += kWave * kFill) {
-                        float xv[kFill];  // all loads of the round in flight together
-#pragma unroll
-                        for (int u = 0; u < kFill; ++u) {
-                            const int kk = min(q0 + u * kWave + lane, span - 1);
-                            xv[u] = buf_load<float>(rs, kk * 4, 0);
-                        }
-#pragma unroll
-                        for (int u = 0; u < kFill; ++u) {
-                            const int kk = q0 + u * kWave + lane;  // a round past the slices fits in cstride
-                            float v = xv[u];
-                            if (GEOM == MBX_GEOM_CONE) {
-                                const float uu = fmaf(c1, (float)(kf + kk - g.kref), c0);
-                                v *= __builtin_amdgcn_sqrtf(fmaf(uu, uu, 1.0f));
-                            }
-                            cyl[P + kk] = kk < span ? v : 0.0f;
-                        }
-                    }
-                    if (lane < P) {
-                        cyl[lane] = 0.0f;
-                        cyl[P + span + lane] = 0.0f;
-                    }
+            } else {
+                const int nch = (span + kWave - 1) / kWave;
+                for (int c = 0; c < nch; ++c) {
+                    const int kk = c * kWave + lane;
+                    if (kk >= span) cyl[P + kk] = 0.0f;
                 }
             }
-            __syncthreads();
-            // ---- update: every wave adds the batch to its rows, batch order
-            if (wm0 < m1) {  // wave-uniform
-                // lane i holds slot i's staging data; the loop reads it with readlane
-                const int l_span = lane < nb ? slot_span[lane] : 0;
-                const int l_kf = lane < nb ? slot_kf[lane] : 0;
-                const int l_sc = lane < nb ? l_s[b0 + lane] : 0;
-                for (int slot = 0; slot < nb; ++slot) {
-                    const int span = __builtin_amdgcn_readlane(l_span, slot);
+            if (lane < P) {
+                cyl[lane] = 0.0f;
+                cyl[P + span + lane] = 0.0f;
+            }
+        }
+    };
+
+    if (nbat > 0) {
+        nxt = load_slots(0);
+        issue(nxt, 0);
+    }
+    for (int bi = 0; bi < nbat; ++bi) {
+        cur = nxt;
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of batch bi have landed
+        finish(cur, bi);
+        __syncthreads();  // batch bi staged by every wave; batch bi-1 no longer read
+        if (bi + 1 < nbat) {
+            nxt = load_slots(bi + 1);
+            issue(nxt, bi + 1);
+        }
+        const int b0 = lo + bi * nbatch;
+        const int nb = min(nbatch, hi - b0);
+        const float *buf = bufs + (size_t)(bi & 1) * nbatch * cstride;
+        if (wm0 < m1) {  // wave-uniform
+#pragma unroll
+            for (int D = 0; D < ND; ++D) {
+                const int slo = max(__builtin_amdgcn_readlane(l_bs, D), b0) - b0;
+                const int shi = min(__builtin_amdgcn_readlane(l_bs, D + 1), b0 + nb) - b0;
+                if (slo >= shi) continue;  // wave-uniform
+                float vs[RPL][2 * RAD + 1];
+#pragma unroll
+                for (int r = 0; r < RPL; ++r)
+#pragma unroll
+                    for (int t = 0; t <= 2 * RAD; ++t) vs[r][t] = 0.0f;
+                for (int slot = slo; slot < shi; ++slot) {
+                    const int span = __builtin_amdgcn_readlane(cur.span, slot);
                     if (span == 0) continue;
-                    const int kf = __builtin_amdgcn_readlane(l_kf, slot);
-                    const int sc = __builtin_amdgcn_readlane(l_sc, slot);
+                    const int kf = __builtin_amdgcn_readlane(cur.kf, slot);
+                    const int sc = __builtin_amdgcn_readlane(cur.sc, slot);
                     const float4 *r4 = (const float4 *)(Tv + (size_t)sc * kRec);
-                    const float4 R1 = r4[1], R2 = r4[2], R3 = r4[3];
-                    if (GEOM == MBX_GEOM_CONE && (rec_mlo(R3.w) > wm0 + kWave - 1 || rec_mhi(R3.w) < wm0))
-                        continue;  // no row of this slab is reached
-                    const float4 wA = *(const float4 *)(slot_w + slot * CB);
-                    const float4 wB = *(const float4 *)(slot_w + slot * CB + 4);
-                    const float wc[CB] = {wA.x, wA.y, wA.z, wA.w, wB.x, wB.y, wB.z, wB.w};
-                    const float *cyl = cyl_all + (size_t)slot * cstride;
-                    float v;
-                    if (GEOM == MBX_GEOM_CONE) {
-                        const float acr = R1.z, W = R1.w, Wlo = R2.x, hv = R2.y, Lv = R2.z, invW = R2.w;
-                        const float mp = mf - (float)__float_as_int(R3.z);  // m - ai, exact
-                        const float kr = rintf(fmaf(mp, invW, -(acr * invW)));  // nearest slice - kref
-                        // t - m for slice kref + kr: the first fma is rounded once, at |t - m| < 1
-                        const float dm = fmaf(W, kr, -mp) + fmaf(Wlo, kr, acr);
-                        // taps past the staged slices read the zero pads
-                        const int off = min(max((int)kr + (g.kref + P - kf), P - RB - 1), P + span + RB);
-                        v = 0.0f;
+                    const float4 R0 = r4[0], R1 = r4[1], R2 = r4[2], R3 = r4[3];
+                    if (GEOM == MBX_GEOM_CONE &&
+                        (rec_mlo(R3.w) > wm0 + kWave * RPL - 1 || rec_mhi(R3.w) < wm0))
+                        continue;  // no row of this wave is reached
+                    const float Aall[5] = {R0.y, R0.z, R0.w, R1.x, R1.y};  // Ac(n_c - 2 .. n_c + 2)
+                    const float *cyl = buf + (size_t)slot * cstride;
+                    const float acr = R1.z, W = R1.w, Wlo = R2.x, hv = R2.y, Lv = R2.z, invW = R2.w;
+                    const float aif = (float)__float_as_int(R3.z), nai = -(acr * invW);
+                    const float q1 = hv - W, q2 = hv - 2.0f * W;  // taps one and two slices off the nearest
+                    const int kbase = g.kref + P - kf;
 #pragma unroll
-                        for (int j = -RB; j <= RB; ++j) {
-                            const float wt = __builtin_amdgcn_fmed3f(hv - fabsf(fmaf(W, (float)j, dm)), 0.0f, Lv);
-                            v = fmaf(wt, cyl[off + j], v);
+                    for (int r = 0; r < RPL; ++r) {
+                        const float mf = (float)(wm0 + lane + kWave * r);
+                        float v;
+                        if (GEOM == MBX_GEOM_CONE) {
+                            const float mp = mf - aif;                      // m - ai, exact
+                            const float kr = rintf(fmaf(mp, invW, nai));    // nearest slice - kref
+                            // t - m for slice kref + kr: the first fma rounds once, at |t - m| < 1
+                            const float dm = fmaf(W, kr, -mp) + fmaf(Wlo, kr, acr);
+                            // taps past the staged slices read the zero pads
+                            const int off = min(max((int)kr + kbase, P - RB - 1), P + span + RB);
+                            // |dm| <= W/2, so |dm -+ jW| = jW -+ dm
+                            v = __builtin_amdgcn_fmed3f(hv - fabsf(dm), 0.0f, Lv) * cyl[off];
+                            v = fmaf(__builtin_amdgcn_fmed3f(q1 + dm, 0.0f, Lv), cyl[off - 1], v);
+                            v = fmaf(__builtin_amdgcn_fmed3f(q1 - dm, 0.0f, Lv), cyl[off + 1], v);
+                            if (RB == 2) {
+                                v = fmaf(__builtin_amdgcn_fmed3f(q2 + dm, 0.0f, Lv), cyl[off - 2], v);
+                                v = fmaf(__builtin_amdgcn_fmed3f(q2 - dm, 0.0f, Lv), cyl[off + 2], v);
+                            }
+                        } else {
+                            v = cyl[P + (int)mf - m0];  // row m sees slice m (rows past the block read don't-cares)
                         }
-                    } else {
-                        v = cyl[P + (m - m0)];  // row m sees slice m
-                    }
 #pragma unroll
-                    for (int c = 0; c < CB; ++c) acc[c] = fmaf(wc[c], v, acc[c]);
+                        for (int t = 0; t <= 2 * RAD; ++t) vs[r][t] = fmaf(Aall[2 - RAD + t], v, vs[r][t]);
+                    }
                 }
+                // bucket D is centre channel n0 + D - RAD: tap t lands on channel D - 2*RAD + t
+#pragma unroll
+                for (int r = 0; r < RPL; ++r)
+#pragma unroll
+                    for (int t = 0; t <= 2 * RAD; ++t) {
+                        const int c = D - 2 * RAD + t;
+                        if (c >= 0 && c < CB) acc[r][c] += vs[r][t];
+                    }
             }
-            __syncthreads();
         }
-        count = 0;
     }
 
     // ---- write the register tile once (optionally adding to what is there)
     double ss = 0.0;
-    if (m < m1) {
 #pragma unroll
-        for (int c = 0; c < CB; ++c) {
-            const int n = n0 + c;
-            if (n < n1) {
-                const size_t o = LAYOUT == MBX_LAYOUT_VCR ? ((size_t)vi * g.C + n) * (size_t)g.R + m
-                                                          : ((size_t)vi * g.R + m) * (size_t)g.C + n;
-                float v = acc[c];
-                if (accumulate) v += to_float(sino[o]);
-                const OT st = from_float<OT>(v);
-                sino[o] = st;
-                if (part_sumsq) {
-                    const float vs = to_float(st);
-                    ss += (double)vs * (double)vs * (w ? (double)w[o] : 1.0);
+    for (int r = 0; r < RPL; ++r) {
+        const int m = wm0 + lane + kWave * r;
+        if (m < m1) {
+#pragma unroll
+            for (int c = 0; c < CB; ++c) {
+                const int n = n0 + c;
+                if (n < n1) {
+                    const size_t o = LAYOUT == MBX_LAYOUT_VCR ? ((size_t)vi * g.C + n) * (size_t)g.R + m
+                                                              : ((size_t)vi * g.R + m) * (size_t)g.C + n;
+                    float v = acc[r][c];
+                    if (accumulate) v += to_float(sino[o]);
+                    const OT st = from_float<OT>(v);
+                    sino[o] = st;
+                    if (part_sumsq) {
+                        const float vs = to_float(st);
+                        ss += (double)vs * (double)vs * (w ? (double)w[o] : 1.0);
+                    }
                 }
             }
         }
     }
     if (part_sumsq) {
-        double *dscratch = (double *)(scan + 8);
         const double r = block_sum<double>(ss, dscratch);
         if (tid == 0) part_sumsq[b] = r;
     }
 }
 
 template <int GEOM, int RAD, int RB, typename OT, int LAYOUT>
-static void launch_fwd_t(const DevGeom &g, const FwdPlan &pl, const float *T, const int *NC, const float *vox,
-                         int npix, OT *sino, int accumulate, const float *w, double *part, hipStream_t st) {
+static void launch_fwd_t(const DevGeom &g, const FwdPlan &pl, const float *T, const int *list, const int *cstart,
+                         const float *vox, int npix, OT *sino, int accumulate, const float *w, double *part,
+                         hipStream_t st) {
     hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, OT, LAYOUT>), dim3((unsigned)pl.nblocks), dim3(64 * pl.nw), pl.lds,
-                       st, g, T, NC, vox, npix, pl.ncb, pl.nrb, pl.nb, pl.cstride, pl.list, sino, accumulate, w,
+                       st, g, T, list, cstart, vox, npix, pl.ncb, pl.nrb, pl.nb, pl.cstride, sino, accumulate, w,
                        part);
 }
 
 template <int GEOM, int RAD, int RB>
-static void dispatch_fwd_2(const DevGeom &g, const FwdPlan &pl, const float *T, const int *NC, const float *vox,
-                           int npix, void *sino, int dt, int layout, int accumulate, const float *w, double *part,
-                           hipStream_t st) {
+static void dispatch_fwd_2(const DevGeom &g, const FwdPlan &pl, const float *T, const int *list, const int *cstart,
+                           const float *vox, int npix, void *sino, int dt, int layout, int accumulate, const float *w,
+                           double *part, hipStream_t st) {
     if (dt == MBX_DTYPE_F32) {
         if (layout == MBX_LAYOUT_VCR)
-            launch_fwd_t<GEOM, RAD, RB, float, MBX_LAYOUT_VCR>(g, pl, T, NC, vox, npix, (float *)sino, accumulate, w,
-                                                               part, st);
+            launch_fwd_t<GEOM, RAD, RB, float, MBX_LAYOUT_VCR>(g, pl, T, list, cstart, vox, npix, (float *)sino,
+                                                               accumulate, w, part, st);
         else
-            launch_fwd_t<GEOM, RAD, RB, float, MBX_LAYOUT_VRC>(g, pl, T, NC, vox, npix, (float *)sino, accumulate, w,
-                                                               part, st);
+            launch_fwd_t<GEOM, RAD, RB, float, MBX_LAYOUT_VRC>(g, pl, T, list, cstart, vox, npix, (float *)sino,
+                                                               accumulate, w, part, st);
     } else {
         if (layout == MBX_LAYOUT_VCR)
-            launch_fwd_t<GEOM, RAD, RB, bf16, MBX_LAYOUT_VCR>(g, pl, T, NC, vox, npix, (bf16 *)sino, accumulate, w,
-                                                              part, st);
+            launch_fwd_t<GEOM, RAD, RB, bf16, MBX_LAYOUT_VCR>(g, pl, T, list, cstart, vox, npix, (bf16 *)sino,
+                                                              accumulate, w, part, st);
         else
-            launch_fwd_t<GEOM, RAD, RB, bf16, MBX_LAYOUT_VRC>(g, pl, T, NC, vox, npix, (bf16 *)sino, accumulate, w,
-                                                              part, st);
+            launch_fwd_t<GEOM, RAD, RB, bf16, MBX_LAYOUT_VRC>(g, pl, T, list, cstart, vox, npix, (bf16 *)sino,
+                                                              accumulate, w, part, st);
     }
 }
 
-// Forward projection of pixels whose table is built; part (nblocks doubles) and
-// sumsq may be NULL.
+// Forward projection of pixels whose table is built.  aux holds
+// fwd_aux_bytes(g, chunk >= npix, nviews) bytes; sumsq may be NULL.
 int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float *vox, int npix, int nviews, void *sino,
-                   int sino_dtype, int layout, int accumulate, const float *weights, double *part, double *sumsq,
-                   hipStream_t st) {
+                   int sino_dtype, int layout, int accumulate, const float *weights, void *aux, int chunk,
+                   double *sumsq, hipStream_t st) {
     const FwdPlan pl = plan_fwd(g, nviews);
     if (pl.lds > 160 * 1024)
         return fail(MBX_EUNSUPPORTED, "forward projection LDS footprint too large (num_recon_slices too big)");
-    if (sumsq && !part) return fail(MBX_EWORKSPACE, "forward sumsq needs a partials workspace");
+    const int nbk = fwd_nbk(g);
+    if (nbk > kBinMax) return fail(MBX_EUNSUPPORTED, "num_det_channels too large for the forward binning");
+    if (!aux) return fail(MBX_EWORKSPACE, "forward projection needs a workspace");
+    char *a8 = (char *)aux;
+    double *part = (double *)a8;
+    int *list = (int *)(a8 + al256((size_t)pl.nblocks * sizeof(double)));
+    int *cstart = (int *)((char *)list + al256((size_t)nviews * chunk * sizeof(int)));
+    if (nviews > 0) {
+        hipLaunchKernelGGL(fwd_bin_kernel, dim3(nviews), dim3(256), nbk * sizeof(int), st, NC, npix, g.rh, nbk, list,
+                           cstart);
+        MBX_LAUNCH_CHECK();
+    }
     double *p = sumsq ? part : nullptr;
     if (g.kind == MBX_GEOM_CONE) {
         if (g.rh == 1 && g.rb == 1)
-            dispatch_fwd_2<MBX_GEOM_CONE, 1, 1>(g, pl, T, NC, vox, npix, sino, sino_dtype, layout, accumulate, weights,
-                                                p, st);
+            dispatch_fwd_2<MBX_GEOM_CONE, 1, 1>(g, pl, T, list, cstart, vox, npix, sino, sino_dtype, layout, accumulate,
+                                                weights, p, st);
         else if (g.rh == 1)
-            dispatch_fwd_2<MBX_GEOM_CONE, 1, 2>(g, pl, T, NC, vox, npix, sino, sino_dtype, layout, accumulate, weights,
-                                                p, st);
+            dispatch_fwd_2<MBX_GEOM_CONE, 1, 2>(g, pl, T, list, cstart, vox, npix, sino, sino_dtype, layout, accumulate,
+                                                weights, p, st);
         else if (g.rb == 1)
-            dispatch_fwd_2<MBX_GEOM_CONE, 2, 1>(g, pl, T, NC, vox, npix, sino, sino_dtype, layout, accumulate, weights,
-                                                p, st);
+            dispatch_fwd_2<MBX_GEOM_CONE, 2, 1>(g, pl, T, list, cstart, vox, npix, sino, sino_dtype, layout, accumulate,
+                                                weights, p, st);
         else
-            dispatch_fwd_2<MBX_GEOM_CONE, 2, 2>(g, pl, T, NC, vox, npix, sino, sino_dtype, layout, accumulate, weights,
-                                                p, st);
+            dispatch_fwd_2<MBX_GEOM_CONE, 2, 2>(g, pl, T, list, cstart, vox, npix, sino, sino_dtype, layout, accumulate,
+                                                weights, p, st);
     } else {
         if (g.rh == 1)
-            dispatch_fwd_2<MBX_GEOM_PARALLEL, 1, 1>(g, pl, T, NC, vox, npix, sino, sino_dtype, layout, accumulate,
-                                                    weights, p, st);
+            dispatch_fwd_2<MBX_GEOM_PARALLEL, 1, 1>(g, pl, T, list, cstart, vox, npix, sino, sino_dtype, layout,
+                                                    accumulate, weights, p, st);
         else
-            dispatch_fwd_2<MBX_GEOM_PARALLEL, 2, 1>(g, pl, T, NC, vox, npix, sino, sino_dtype, layout, accumulate,
-                                                    weights, p, st);
+            dispatch_fwd_2<MBX_GEOM_PARALLEL, 2, 1>(g, pl, T, list, cstart, vox, npix, sino, sino_dtype, layout,
+                                                    accumulate, weights, p, st);
     }
     MBX_LAUNCH_CHECK();
     if (sumsq) return launch_sum_partials(part, pl.nblocks, sumsq, 0, st);
@@ -1025,7 +1162,7 @@ int launch_fwd(const DevGeom &g, const float *angles, const float *vox, const in
                const int32_t *views, int nviews, void *sino, int sino_dtype, int layout, int accumulate,
                const float *weights, double *sumsq, void *ws, hipStream_t st) {
     if (nviews == 0) {
-        if (sumsq) MBX_HIP_CHECK(hipMemsetAsync(sumsq, 0, sizeof(double), st));
+        if (sumsq) return launch_sum_partials(nullptr, 0, sumsq, 0, st);  // writes 0
         return MBX_OK;
     }
     if (!ws) return fail(MBX_EWORKSPACE, "forward projection needs a workspace");
@@ -1033,11 +1170,11 @@ int launch_fwd(const DevGeom &g, const float *angles, const float *vox, const in
     char *w8 = (char *)ws;
     float *T = (float *)w8;
     int *NC = table_nc(T, chunk, nviews);
-    double *part = (double *)(w8 + table_bytes(chunk, nviews));
+    void *aux = w8 + table_bytes(chunk, nviews);
     if (npix == 0) {
         // nothing to project: the kernel still writes (zeros or the unchanged sinogram)
-        return launch_fwd_tab(g, T, NC, vox, 0, nviews, sino, sino_dtype, layout, accumulate, weights, part, sumsq,
-                              st);
+        return launch_fwd_tab(g, T, NC, vox, 0, nviews, sino, sino_dtype, layout, accumulate, weights, aux, chunk,
+                              sumsq, st);
     }
     for (int c0 = 0; c0 < npix; c0 += chunk) {
         const int n = std::min(chunk, npix - c0);
@@ -1045,7 +1182,7 @@ int launch_fwd(const DevGeom &g, const float *angles, const float *vox, const in
         int rc = launch_table(g, angles, pix + c0, n, views, nviews, T, NC, st);
         if (rc != MBX_OK) return rc;
         rc = launch_fwd_tab(g, T, NC, vox + (size_t)c0 * g.Nz, n, nviews, sino, sino_dtype, layout,
-                            c0 == 0 ? accumulate : 1, weights, part, last ? sumsq : nullptr, st);
+                            c0 == 0 ? accumulate : 1, weights, aux, chunk, last ? sumsq : nullptr, st);
         if (rc != MBX_OK) return rc;
     }
     return MBX_OK;

@@ -335,7 +335,7 @@ static VcdLayout vcd_layout(const DevGeom &g, int npix, int ddt) {
     L.bpart = off;
     off += align256(bpl.nsplit > 1 ? (size_t)bpl.nsplit * L.chunk * g.Nz * sizeof(float) : 0);
     L.fpart = off;
-    off += align256((size_t)plan_fwd(g, g.V).nblocks * sizeof(double));
+    off += align256((size_t)fwd_aux_bytes(g, L.chunk, g.V));
     L.bp = off;
     off += align256(cyl);
     L.pg = off;
@@ -462,7 +462,7 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
     float *T = (float *)(ws + L.table);
     int *NC = table_nc(T, L.chunk, d.V);
     float *bpart = (float *)(ws + L.bpart);
-    double *fpart = (double *)(ws + L.fpart);
+    void *fpart = ws + L.fpart;
     const bool one_table = npix <= L.chunk;  // build the table once for all projections
 
     // (c) back projection of the weighted error sinogram
@@ -484,7 +484,7 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
                 if (r != MBX_OK) return r;
             }
             int r = launch_fwd_tab(d, T, NC, delta + (size_t)c0 * d.Nz, n, d.V, dsino, delta_dtype, MBX_LAYOUT_VCR,
-                                   c0 > 0, w, fpart, last ? sumsq : nullptr, st);
+                                   c0 > 0, w, fpart, L.chunk, last ? sumsq : nullptr, st);
             if (r != MBX_OK) return r;
         }
         return MBX_OK;
