@@ -16,8 +16,9 @@ from .vcd_utils import (get_2d_ror_mask, gen_pixel_partition, gen_set_of_pixel_p
                         gen_full_indices)
 from .tomography_utils import generate_direct_recon_filter
 from . import utilities
+from . import autograd
 
 __all__ = ['Param', 'ParameterHandler', 'TomographyModel', 'ReconParams', 'ParallelBeamModel', 'ConeBeamModel',
            'get_b_from_nbr_wts', 'qggmrf_gradient_and_hessian_at_indices', 'qggmrf_loss',
            'prox_gradient_at_indices', 'get_2d_ror_mask', 'gen_pixel_partition', 'gen_set_of_pixel_partitions',
-           'gen_partition_sequence', 'gen_full_indices', 'generate_direct_recon_filter', 'utilities']
+           'gen_partition_sequence', 'gen_full_indices', 'generate_direct_recon_filter', 'utilities', 'autograd']
