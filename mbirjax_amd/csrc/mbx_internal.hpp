@@ -48,12 +48,14 @@ inline int *table_nc(float *T, int chunk, int nviews) {
 
 // Back projection planning: number of view splits (partials) and slice chunking.
 struct BackPlan {
+    int ppw;     // pixels per wavefront (2; 1 with Kahan compensation)
     int nsplit;  // view groups reduced at the end (1 = write directly)
     int kpl;     // slices per lane (chunk = 64*kpl slices)
     int nchunk;  // slice chunks
     int cap;     // LDS floats per wave for the detector column
 };
-BackPlan plan_back(const DevGeom &g, int npix, int nviews);
+BackPlan plan_back(const DevGeom &g, int npix, int nviews, int ppw);
+int64_t back_part_bytes(const DevGeom &g, int chunk, int nviews);  // view-group partials
 int64_t back_workspace_bytes(const DevGeom &g, int npix, int nviews);
 
 struct FwdPlan {

@@ -252,15 +252,23 @@ int launch_table(const DevGeom &g, const float *angles, const int32_t *pix, int 
 // Back projection
 // ======================================================================
 
-BackPlan plan_back(const DevGeom &g, int npix, int nviews) {
+BackPlan plan_back(const DevGeom &g, int npix, int nviews, int ppw) {
     BackPlan p;
     const int per_lane = (g.Nz + kWave - 1) / kWave;
     p.kpl = per_lane <= 4 ? 4 : per_lane <= 8 ? 8 : 16;
     p.nchunk = (g.Nz + kWave * p.kpl - 1) / (kWave * p.kpl);
-    const int64_t waves = ((int64_t)npix + 1) / 2 * p.nchunk;
-    // Small subsets: split the views over the 8 XCDs so that each XCD's L2
-    // streams only its eighth of the sinogram; large passes are VALU-bound.
-    p.nsplit = (waves <= 16384 && nviews >= 8) ? 8 : 1;
+    p.ppw = ppw;
+    const int64_t units = ((int64_t)npix + ppw - 1) / ppw * p.nchunk;  // waves per view group
+    // Small subsets: split the views over the XCDs so that each XCD's L2
+    // streams only its share of the sinogram, keeping every wave of the pass
+    // resident at once (7 waves per SIMD); large passes are VALU-bound.
+    const int64_t resident = 7LL * 4 * 256;
+    if (nviews < 8 || units > 16384)
+        p.nsplit = 1;
+    else if (ppw == 2)
+        p.nsplit = 8;
+    else
+        p.nsplit = units * 8 <= resident ? 8 : units * 4 <= resident ? 4 : 2;
     // LDS detector-column capacity per pixel: the rows one slice chunk can
     // reach, from the largest pixel magnification in the recon disc (plus
     // taps and margins), padded to whole 64-row chunks; at least the detector
@@ -281,12 +289,20 @@ BackPlan plan_back(const DevGeom &g, int npix, int nviews) {
     return p;
 }
 
+// View-group partials of either pixels-per-wave plan.
+int64_t back_part_bytes(const DevGeom &g, int chunk, int nviews) {
+    int64_t part = 0;
+    for (int ppw = 1; ppw <= 2; ++ppw) {
+        const BackPlan p = plan_back(g, chunk, nviews, ppw);
+        if (p.nsplit > 1) part = std::max<int64_t>(part, (int64_t)p.nsplit * chunk * g.Nz * (int64_t)sizeof(float));
+    }
+    return (part + 255) & ~255ll;
+}
+
 int64_t back_workspace_bytes(const DevGeom &g, int npix, int nviews) {
     if (npix == 0 || nviews == 0) return 0;
     const int chunk = table_chunk(g, npix, nviews);
-    BackPlan p = plan_back(g, chunk, nviews);
-    int64_t part = p.nsplit == 1 ? 0 : (int64_t)p.nsplit * chunk * g.Nz * (int64_t)sizeof(float);
-    return table_bytes(chunk, nviews) + ((part + 255) & ~255ll);
+    return table_bytes(chunk, nviews) + back_part_bytes(g, chunk, nviews);
 }
 
 // One wavefront per (pixel pair, slice chunk, view group).  For every view the
@@ -496,6 +512,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
     DevGeom g, const float *__restrict__ T, const ET *__restrict__ sino, const float *__restrict__ w, int npix,
     const int32_t *__restrict__ views, int nviews, int nsplit, int chunk0, int nchunk, int cap, float scale,
     int accumulate, float *__restrict__ out, float *__restrict__ part) {
+    constexpr int PPW = KAHAN ? 1 : 2;  // pixels per wave
     extern __shared__ float lds[];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -503,16 +520,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
     const int grp = blockIdx.x / nsplit;
     const int chunk = chunk0 + grp % nchunk;
     const int q = __builtin_amdgcn_readfirstlane((grp / nchunk) * 4 + wave);
-    const int sA = 2 * q;
+    const int sA = PPW * q;
     if (sA >= npix) return;  // whole wavefront leaves; no block barriers below
-    const bool hasB = sA + 1 < npix;
+    const bool hasB = PPW == 2 && sA + 1 < npix;
     const int sB = hasB ? sA + 1 : sA;
     const int k0 = chunk * kWave * KPL;
     const int kend = min(g.Nz, k0 + kWave * KPL);
     const int vb = (int)(((int64_t)nviews * split) / nsplit);
     const int ve = (int)(((int64_t)nviews * (split + 1)) / nsplit);
-    float *colA = lds + wave * 2 * cap;
-    float *colB = colA + cap;
+    float *colA = lds + wave * PPW * cap;
+    float *colB = colA + cap;  // PPW == 2 only
 
     float accA[KPL], cmpA[KPL], accB[KPL], cmpB[KPL];
 #pragma unroll
@@ -563,9 +580,10 @@ template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN, bool WGT
 static void launch_back_range(const DevGeom &g, const BackPlan &pl, const float *T, const ET *sino, const float *w,
                               int npix, const int32_t *views, int nviews, int chunk0, int nch, float scale,
                               int accumulate, float *out, float *part, hipStream_t st) {
-    const int64_t octets = ((int64_t)npix + 7) / 8;  // 4 waves x 2 pixels per block
-    const int64_t blocks = octets * nch * pl.nsplit;
-    const size_t lds = GEOM == MBX_GEOM_CONE ? 4 * 2 * (size_t)pl.cap * sizeof(float) : 0;
+    constexpr int PPW = KAHAN ? 1 : 2;  // pixels per wave, as in back_kernel
+    const int64_t groups = ((int64_t)npix + 4 * PPW - 1) / (4 * PPW);  // 4 waves per block
+    const int64_t blocks = groups * nch * pl.nsplit;
+    const size_t lds = GEOM == MBX_GEOM_CONE ? 4 * PPW * (size_t)pl.cap * sizeof(float) : 0;
     hipLaunchKernelGGL((back_kernel<GEOM, KPL, RAD, ET, POW, KAHAN, WGT, TAIL>), dim3((unsigned)blocks), dim3(256), lds,
                        st, g, T, sino, w, npix, views, nviews, pl.nsplit, chunk0, nch, pl.cap, scale, accumulate, out,
                        part);
@@ -657,7 +675,8 @@ int launch_back_tab(const DevGeom &g, const float *T, int npix, const void *sino
                     const float *weights, const int32_t *views, int nviews, int coeff_power, int kahan, float scale,
                     int accumulate, float *out, float *part, hipStream_t st) {
     if (npix == 0) return MBX_OK;
-    const BackPlan pl = plan_back(g, npix, nviews);
+    // Kahan keeps a compensation per slice: one pixel per wave keeps it in registers
+    const BackPlan pl = plan_back(g, npix, nviews, kahan ? 1 : 2);
     if (pl.nsplit > 1 && !part) return fail(MBX_EWORKSPACE, "back projection needs a partials workspace");
     int rc;
     if (g.kind == MBX_GEOM_CONE)

@@ -327,13 +327,13 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 static VcdLayout vcd_layout(const DevGeom &g, int npix, int ddt) {
     VcdLayout L;
     L.chunk = table_chunk(g, std::max(npix, 1), g.V);
-    const BackPlan bpl = plan_back(g, L.chunk, g.V);
+
     size_t off = 0;
     const size_t cyl = (size_t)npix * g.Nz * sizeof(float);
     L.table = off;
     off += align256((size_t)table_bytes(L.chunk, g.V));
     L.bpart = off;
-    off += align256(bpl.nsplit > 1 ? (size_t)bpl.nsplit * L.chunk * g.Nz * sizeof(float) : 0);
+    off += align256((size_t)back_part_bytes(g, L.chunk, g.V));
     L.fpart = off;
     off += align256((size_t)fwd_aux_bytes(g, L.chunk, g.V));
     L.bp = off;

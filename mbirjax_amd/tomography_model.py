@@ -24,6 +24,7 @@ import torch
 
 from . import _lib
 from . import qggmrf as _qggmrf
+from . import utilities
 from . import vcd_utils
 from .parameter_handler import ParameterHandler
 
@@ -362,8 +363,11 @@ class TomographyModel(ParameterHandler):
               print_logs=True, precision='fp32'):
         """MBIR reconstruction with multi-granular VCD (mbirjax/tomography_model.py:966-1084).
 
-        precision: 'fp32' (plain) or 'mixed' (bfloat16 error/delta sinograms,
-        Kahan-compensated back projection, FP64 step-size and cost reductions).
+        precision: 'fp32' (plain); 'mixed' (error sinogram accumulated in float32,
+        delta sinogram A*delta stored in bfloat16, Kahan-compensated back
+        projection, FP64 step-size and cost reductions); or 'bf16' (as 'mixed'
+        with the error sinogram also stored in bfloat16: least memory traffic,
+        but round-to-nearest drops updates below half a bfloat16 ulp).
         """
         if num_iterations is not None:
             warnings.warn('num_iterations has been deprecated and will be removed in a future release.\n'
@@ -420,8 +424,8 @@ class TomographyModel(ParameterHandler):
         the subset-update workspace (mbirjax/tomography_model.py:1110-1192).
         With channel_major=True, sinogram (and weights) are already (views, C, R)
         float32 device tensors."""
-        if precision not in ('fp32', 'mixed'):
-            raise ValueError("precision must be 'fp32' or 'mixed'")
+        if precision not in ('fp32', 'mixed', 'bf16'):
+            raise ValueError("precision must be 'fp32', 'mixed' or 'bf16'")
         dev = self._device()
         recon_shape = self.get_params('recon_shape')
         rows, cols, nz = recon_shape
@@ -452,7 +456,7 @@ class TomographyModel(ParameterHandler):
         Ax = self._fwd(init_recon.reshape(-1, nz)[full_d.long()].contiguous(), full_d, layout=_lib.LAYOUT_VCR)
         nrm = self._dot(Ax, None, w)
         alpha0 = self._dot(Ax, y, w) / nrm if nrm > 0 else 1.0
-        e_dtype = _BF16 if precision == 'mixed' else _F32
+        e_dtype = _BF16 if precision == 'bf16' else _F32
         alpha_dev = torch.tensor([float(np.float32(alpha0))], dtype=torch.float64, device=dev)
         e32 = y.clone()
         _lib.check(_lib.lib().mbx_sino_axpy(_lib.ptr(e32), _F32, _lib.ptr(Ax), _F32, _lib.ptr(alpha_dev),
@@ -572,6 +576,11 @@ class TomographyModel(ParameterHandler):
             return 1.0 / (torch.abs(s) + 0.1)
         raise Exception("gen_weights: undefined weight_type {}".format(weight_type))
 
+    def gen_modified_3d_sl_phantom(self):
+        """Low-dynamic-range 3D Shepp-Logan phantom of the model's recon shape
+        (mbirjax/tomography_model.py:1766-1775)."""
+        return utilities.generate_3d_shepp_logan_low_dynamic_range(self.get_params('recon_shape'))
+
     def reshape_recon(self, recon):
         return recon.reshape(self.get_params('recon_shape'))
 
@@ -599,7 +608,7 @@ class TomographyModel(ParameterHandler):
 class VCDState:
     """Device state of one VCD reconstruction: the recon (rows*cols, Nz), the
     error sinogram e = y - A x in channel-major layout (float32, or bfloat16
-    for precision='mixed'), weights, Hessian diagonal and workspace.  iterate()
+    for precision='bf16'), weights, Hessian diagonal and workspace.  iterate()
     runs one partition through mbx_vcd_subset_update with no host syncs."""
 
     def __init__(self, model, e, e_dtype, w, flat_recon, fm_hessian, prox_input, precision, max_subset_size):
@@ -617,8 +626,9 @@ class VCDState:
         self.qstruct = _lib.qggmrf_struct(*self.qp)
         self.sigma_prox = float(model.get_params('sigma_prox'))
         self.positivity = bool(model.get_params('positivity_flag'))
-        self.kahan = precision == 'mixed'
-        self.wsb = int(_lib.lib().mbx_vcd_workspace_bytes(model._geom, int(max_subset_size), e_dtype))
+        self.kahan = precision in ('mixed', 'bf16')
+        self.delta_dtype = _F32 if precision == 'fp32' else _BF16
+        self.wsb = int(_lib.lib().mbx_vcd_workspace_bytes(model._geom, int(max_subset_size), self.delta_dtype))
         self.ws = torch.empty(self.wsb, dtype=torch.uint8, device=e.device)
         self.stats = torch.zeros(4, dtype=torch.float64, device=e.device)
         self.avg_w = 1.0 if w is None else float(torch.mean(w, dtype=torch.float64).item())
@@ -629,7 +639,7 @@ class VCDState:
         _lib.check(_lib.lib().mbx_vcd_subset_update(
             m._geom, _lib.ptr(m._angles()), _lib.ptr(self.flat_recon), _lib.ptr(self.e), self.e_dtype,
             _lib.ptr(self.w), _lib.ptr(self.fm_hessian), _lib.ptr(subset), int(subset.numel()), self.fm_constant,
-            self.qstruct, _lib.ptr(self.prox_input), self.sigma_prox, int(self.positivity), self.e_dtype,
+            self.qstruct, _lib.ptr(self.prox_input), self.sigma_prox, int(self.positivity), self.delta_dtype,
             int(self.kahan), _lib.ptr(self.stats), _lib.ptr(self.ws), self.wsb, _lib.stream()))
 
     def iterate(self, part, order):
