@@ -123,6 +123,7 @@ int make_dev_geom(const mbx_geom *g, DevGeom *d) {
 constexpr int kRec = 16;
 constexpr int kFill = 4;  // 64-element groups loaded per round (forward staging)
 constexpr int kBackFill = 3;  // 64-row chunks loaded per round (back projection)
+constexpr int kKahanViews = 16;  // views summed plainly between Kahan steps
 constexpr int64_t kTableBudget = 512ll << 20;
 
 int table_chunk(const DevGeom &g, int npix, int nviews) {
@@ -327,10 +328,10 @@ struct ColumnSource {
 // FIT: the column holds every row the chunk's taps reach (plus a zero or
 // don't-care margin row each side), so taps index it directly; otherwise the
 // indices are clamped to the stored rows.
-template <int KPL, int RAD, int POW, bool KAHAN, bool TAIL, bool FIT>
+template <int KPL, int RAD, int POW, bool TAIL, bool FIT>
 __device__ __forceinline__ void back_vertical(const float *col, int shift, int n_rows, double acd, double Wd,
                                               float hv, float Lv, float c0, float c1, float kr0, float krlast,
-                                              float (&acc)[KPL], float (&cmp)[KPL]) {
+                                              float (&acc)[KPL]) {
     // Slice k sits at row ai + t with t = acr + W*(k - kref), formed in FP64 so
     // that the tap distances d = t - rint(t) carry no float32 rounding at the
     // magnitude of a row index; rint is the 1.5*2^52 shift, whose low word is
@@ -391,22 +392,15 @@ __device__ __forceinline__ void back_vertical(const float *col, int shift, int n
         }
         const float s2 = fmaf(u, u, 1.0f);
         const float f = (POW == 2) ? s2 : __builtin_amdgcn_sqrtf(s2);
-        if (KAHAN) {
-            const float y = sum * f - cmp[i];
-            const float tt = acc[i] + y;
-            cmp[i] = (tt - acc[i]) - y;
-            acc[i] = tt;
-        } else {
-            acc[i] = fmaf(sum, f, acc[i]);
-        }
+        acc[i] = fmaf(sum, f, acc[i]);
     }
 }
 
-template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN, bool WGT, bool TAIL>
+template <int GEOM, int KPL, int RAD, typename ET, int POW, bool WGT, bool TAIL>
 __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *__restrict__ rec,
                                                 const ET *__restrict__ sino, const float *__restrict__ w, int vd,
                                                 int k0, int kend, int cap, float *col, int lane,
-                                                float (&acc)[KPL], float (&cmp)[KPL]) {
+                                                float (&acc)[KPL]) {
     const int nc = __float_as_int(rec[0]);
     float Aw[2 * RAD + 1];
 #pragma unroll
@@ -441,7 +435,7 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *_
         // ---- detector column col[i] = row mstart + i, gathered in 64-row
         // chunks (loads of up to kFill chunks in flight).  Rows are clamped
         // to the detector for the loads; rows off it are zeroed afterwards.
-        constexpr int FILL = KAHAN ? 2 : kBackFill;  // Kahan needs the registers for its compensation
+        constexpr int FILL = kBackFill;
         const int nch = (n_rows + kWave - 1) / kWave;
         for (int c0 = 0; c0 < nch; c0 += FILL) {
             float ev[FILL][2 * RAD + 1];
@@ -476,11 +470,9 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *_
         const int shift = RAD + mstart - ai;
         const float kr0 = (float)(k0 + lane - g.kref), krl = (float)(kend - 1 - g.kref);
         if (fit)
-            back_vertical<KPL, RAD, POW, KAHAN, TAIL, true>(col, shift, n_rows, acd, Wd, hv, Lv, c0, c1, kr0, krl, acc,
-                                                            cmp);
+            back_vertical<KPL, RAD, POW, TAIL, true>(col, shift, n_rows, acd, Wd, hv, Lv, c0, c1, kr0, krl, acc);
         else
-            back_vertical<KPL, RAD, POW, KAHAN, TAIL, false>(col, shift, n_rows, acd, Wd, hv, Lv, c0, c1, kr0, krl,
-                                                             acc, cmp);
+            back_vertical<KPL, RAD, POW, TAIL, false>(col, shift, n_rows, acd, Wd, hv, Lv, c0, c1, kr0, krl, acc);
         wave_sync();
     } else {
 #pragma unroll
@@ -493,14 +485,7 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *_
                 if (WGT) e *= buf_load<float>(src.rw, k * 4, src.soff[j] * (int)(4 / sizeof(ET)));
                 c = fmaf(src.cv[j] ? Aw[j] : 0.0f, e, c);
             }
-            if (KAHAN) {
-                const float y = c - cmp[i];
-                const float tt = acc[i] + y;
-                cmp[i] = (tt - acc[i]) - y;
-                acc[i] = tt;
-            } else {
-                acc[i] += c;
-            }
+            acc[i] += c;
         }
     }
 }
@@ -531,18 +516,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
     float *colA = lds + wave * PPW * cap;
     float *colB = colA + cap;  // PPW == 2 only
 
-    float accA[KPL], cmpA[KPL], accB[KPL], cmpB[KPL];
+    // KAHAN: plain float32 sums over blocks of kKahanViews views, added to the
+    // total with Kahan compensation (compensated summation across views at a
+    // fraction of the per-view cost).
+    float accA[KPL], accB[KPL], tot[KPL], cmp[KPL];
 #pragma unroll
-    for (int i = 0; i < KPL; ++i) accA[i] = cmpA[i] = accB[i] = cmpB[i] = 0.0f;
+    for (int i = 0; i < KPL; ++i) accA[i] = accB[i] = tot[i] = cmp[i] = 0.0f;
     for (int vi = vb; vi < ve; ++vi) {
         const int vd = views ? views[vi] : vi;
         const float *recA = T + ((size_t)vi * npix + sA) * kRec;
         const float *recB = T + ((size_t)vi * npix + sB) * kRec;
-        back_pixel_view<GEOM, KPL, RAD, ET, POW, KAHAN, WGT, TAIL>(g, recA, sino, w, vd, k0, kend, cap, colA, lane,
-                                                             accA, cmpA);
+        back_pixel_view<GEOM, KPL, RAD, ET, POW, WGT, TAIL>(g, recA, sino, w, vd, k0, kend, cap, colA, lane, accA);
         if (hasB)
-            back_pixel_view<GEOM, KPL, RAD, ET, POW, KAHAN, WGT, TAIL>(g, recB, sino, w, vd, k0, kend, cap, colB,
-                                                                 lane, accB, cmpB);
+            back_pixel_view<GEOM, KPL, RAD, ET, POW, WGT, TAIL>(g, recB, sino, w, vd, k0, kend, cap, colB, lane,
+                                                                accB);
+        if (KAHAN && ((vi - vb) % kKahanViews == kKahanViews - 1 || vi == ve - 1)) {  // wave-uniform
+#pragma unroll
+            for (int i = 0; i < KPL; ++i) {
+                const float y = accA[i] - cmp[i];
+                const float t = tot[i] + y;
+                cmp[i] = (t - tot[i]) - y;
+                tot[i] = t;
+                accA[i] = 0.0f;
+            }
+        }
+    }
+    if (KAHAN) {
+#pragma unroll
+        for (int i = 0; i < KPL; ++i) accA[i] = tot[i];
     }
 #pragma unroll
     for (int i = 0; i < KPL; ++i) {
