@@ -845,15 +845,18 @@ __global__ __launch_bounds__(256) void fwd_bin_kernel(const int *__restrict__ NC
 // Staging metadata of one candidate pixel for this block.
 struct FwdSlot {
     int sc, kf, span;
+    int wmask;  // bit w: the pixel reaches a row of wave w
     float c0, c1;
 };
 
 template <int GEOM, int RB>
-__device__ __forceinline__ FwdSlot fwd_slot(const DevGeom &g, const float *__restrict__ Tv, int sc, int m0, int m1) {
+__device__ __forceinline__ FwdSlot fwd_slot(const DevGeom &g, const float *__restrict__ Tv, int sc, int m0, int m1,
+                                            int nw, int rpw) {
     FwdSlot q;
     q.sc = sc;
     q.c0 = 0.0f;
     q.c1 = 0.0f;
+    q.wmask = (1 << nw) - 1;
     if (GEOM == MBX_GEOM_CONE) {
         const float *rec = Tv + (size_t)sc * kRec;
         const float acr = rec[6], invW = rec[11];
@@ -865,6 +868,10 @@ __device__ __forceinline__ FwdSlot fwd_slot(const DevGeom &g, const float *__res
         q.kf = max(g.kref + (int)rintf(((float)(rlo - ai) - acr) * invW) - RB, 0);
         const int kl = min(g.kref + (int)rintf(((float)(rhi - ai) - acr) * invW) + RB, g.Nz - 1);
         q.span = (rlo <= rhi && q.kf <= kl) ? kl - q.kf + 1 : 0;
+        int mask = 0;
+        for (int w = 0; w < nw; ++w)
+            if (rlo <= m0 + (w + 1) * rpw - 1 && rhi >= m0 + w * rpw) mask |= 1 << w;
+        q.wmask = q.span > 0 ? mask : 0;  // nothing is staged for an empty span
     } else {
         q.kf = m0;
         q.span = m1 - m0;
@@ -929,11 +936,11 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
 
     // lane i of every wave holds slot i's staging data for the batch in use
     // (cur) and the one being loaded (nxt)
-    FwdSlot cur{0, 0, 0, 0.0f, 0.0f}, nxt{0, 0, 0, 0.0f, 0.0f};
+    FwdSlot cur{0, 0, 0, 0, 0.0f, 0.0f}, nxt{0, 0, 0, 0, 0.0f, 0.0f};
     auto load_slots = [&](int bi) {
-        FwdSlot q{0, 0, 0, 0.0f, 0.0f};
+        FwdSlot q{0, 0, 0, 0, 0.0f, 0.0f};
         const int i = lo + bi * nbatch + lane;
-        if (lane < nbatch && i < hi) q = fwd_slot<GEOM, RB>(g, Tv, lv[i], m0, m1);
+        if (lane < nbatch && i < hi) q = fwd_slot<GEOM, RB>(g, Tv, lv[i], m0, m1, nw, kWave * RPL);
         return q;
     };
     // asynchronous global -> LDS copy of the raw slices of this wave's slots
@@ -970,9 +977,14 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
                 const float c0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, q.c0), slot));
                 const float c1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, q.c1), slot));
                 const int nch = (span + kWave - 1) / kWave;
-                for (int c = 0; c < nch; ++c) {
-                    const int kk = c * kWave + lane;
-                    const float uu = fmaf(c1, (float)(kf + kk - g.kref), c0);
+                float uu = fmaf(c1, (float)(kf + lane - g.kref), c0);  // v/sdd of this lane's slice
+                const float du = c1 * (float)kWave;
+                for (int c = 0; c < nch - 1; ++c, uu += du) {
+                    float *px = cyl + P + c * kWave + lane;
+                    *px = *px * __builtin_amdgcn_sqrtf(fmaf(uu, uu, 1.0f));
+                }
+                {  // last chunk: slices past the span read as zero
+                    const int kk = (nch - 1) * kWave + lane;
                     const float x = cyl[P + kk] * __builtin_amdgcn_sqrtf(fmaf(uu, uu, 1.0f));
                     cyl[P + kk] = kk < span ? x : 0.0f;
                 }
@@ -1011,62 +1023,53 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
             for (int D = 0; D < ND; ++D) {
                 const int slo = max(__builtin_amdgcn_readlane(l_bs, D), b0) - b0;
                 const int shi = min(__builtin_amdgcn_readlane(l_bs, D + 1), b0 + nb) - b0;
-                if (slo >= shi) continue;  // wave-uniform
-                float vs[RPL][2 * RAD + 1];
-#pragma unroll
-                for (int r = 0; r < RPL; ++r)
-#pragma unroll
-                    for (int t = 0; t <= 2 * RAD; ++t) vs[r][t] = 0.0f;
                 for (int slot = slo; slot < shi; ++slot) {
+                    if (!((__builtin_amdgcn_readlane(cur.wmask, slot) >> wave) & 1))
+                        continue;  // no row of this wave is reached (wave-uniform)
                     const int span = __builtin_amdgcn_readlane(cur.span, slot);
-                    if (span == 0) continue;
                     const int kf = __builtin_amdgcn_readlane(cur.kf, slot);
                     const int sc = __builtin_amdgcn_readlane(cur.sc, slot);
                     const float4 *r4 = (const float4 *)(Tv + (size_t)sc * kRec);
                     const float4 R0 = r4[0], R1 = r4[1], R2 = r4[2], R3 = r4[3];
-                    if (GEOM == MBX_GEOM_CONE &&
-                        (rec_mlo(R3.w) > wm0 + kWave * RPL - 1 || rec_mhi(R3.w) < wm0))
-                        continue;  // no row of this wave is reached
                     const float Aall[5] = {R0.y, R0.z, R0.w, R1.x, R1.y};  // Ac(n_c - 2 .. n_c + 2)
+                    const float kbase = (float)(g.kref + P - kf);  // staged index of slice kref
+                    // cyl[i] is slice kf - P + i; the index is clamped so that the
+                    // taps stay on the staged slices or their zero pads
                     const float *cyl = buf + (size_t)slot * cstride;
+                    const float klo = (float)(P - RB - 1), khi = (float)(P + span + RB);
                     const float acr = R1.z, W = R1.w, Wlo = R2.x, hv = R2.y, Lv = R2.z, invW = R2.w;
                     const float aif = (float)__float_as_int(R3.z), nai = -(acr * invW);
                     const float q1 = hv - W, q2 = hv - 2.0f * W;  // taps one and two slices off the nearest
-                    const int kbase = g.kref + P - kf;
 #pragma unroll
                     for (int r = 0; r < RPL; ++r) {
                         const float mf = (float)(wm0 + lane + kWave * r);
                         float v;
                         if (GEOM == MBX_GEOM_CONE) {
-                            const float mp = mf - aif;                      // m - ai, exact
-                            const float kr = rintf(fmaf(mp, invW, nai));    // nearest slice - kref
+                            const float mp = mf - aif;                    // m - ai, exact
+                            const float kr = rintf(fmaf(mp, invW, nai));  // nearest slice - kref
                             // t - m for slice kref + kr: the first fma rounds once, at |t - m| < 1
                             const float dm = fmaf(W, kr, -mp) + fmaf(Wlo, kr, acr);
-                            // taps past the staged slices read the zero pads
-                            const int off = min(max((int)kr + kbase, P - RB - 1), P + span + RB);
+                            const float *cp = cyl + (int)__builtin_amdgcn_fmed3f(kr + kbase, klo, khi);
                             // |dm| <= W/2, so |dm -+ jW| = jW -+ dm
-                            v = __builtin_amdgcn_fmed3f(hv - fabsf(dm), 0.0f, Lv) * cyl[off];
-                            v = fmaf(__builtin_amdgcn_fmed3f(q1 + dm, 0.0f, Lv), cyl[off - 1], v);
-                            v = fmaf(__builtin_amdgcn_fmed3f(q1 - dm, 0.0f, Lv), cyl[off + 1], v);
+                            v = __builtin_amdgcn_fmed3f(hv - fabsf(dm), 0.0f, Lv) * cp[0];
+                            v = fmaf(__builtin_amdgcn_fmed3f(q1 + dm, 0.0f, Lv), cp[-1], v);
+                            v = fmaf(__builtin_amdgcn_fmed3f(q1 - dm, 0.0f, Lv), cp[1], v);
                             if (RB == 2) {
-                                v = fmaf(__builtin_amdgcn_fmed3f(q2 + dm, 0.0f, Lv), cyl[off - 2], v);
-                                v = fmaf(__builtin_amdgcn_fmed3f(q2 - dm, 0.0f, Lv), cyl[off + 2], v);
+                                v = fmaf(__builtin_amdgcn_fmed3f(q2 + dm, 0.0f, Lv), cp[-2], v);
+                                v = fmaf(__builtin_amdgcn_fmed3f(q2 - dm, 0.0f, Lv), cp[2], v);
                             }
                         } else {
-                            v = cyl[P + (int)mf - m0];  // row m sees slice m (rows past the block read don't-cares)
+                            // row m sees slice m (rows past the block read don't-cares)
+                            v = buf[(size_t)slot * cstride + P + (int)mf - m0];
                         }
+                        // bucket D is centre channel n0 + D - RAD: tap t lands on channel D - 2*RAD + t
 #pragma unroll
-                        for (int t = 0; t <= 2 * RAD; ++t) vs[r][t] = fmaf(Aall[2 - RAD + t], v, vs[r][t]);
+                        for (int t = 0; t <= 2 * RAD; ++t) {
+                            const int c = D - 2 * RAD + t;
+                            if (c >= 0 && c < CB) acc[r][c] = fmaf(Aall[2 - RAD + t], v, acc[r][c]);
+                        }
                     }
                 }
-                // bucket D is centre channel n0 + D - RAD: tap t lands on channel D - 2*RAD + t
-#pragma unroll
-                for (int r = 0; r < RPL; ++r)
-#pragma unroll
-                    for (int t = 0; t <= 2 * RAD; ++t) {
-                        const int c = D - 2 * RAD + t;
-                        if (c >= 0 && c < CB) acc[r][c] += vs[r][t];
-                    }
             }
         }
     }

@@ -172,26 +172,13 @@ __device__ __forceinline__ void axpy_body(TE *__restrict__ e, const TD *__restri
     const bool vec = ((uintptr_t)e % (4 * sizeof(TE)) == 0) && ((uintptr_t)d % (4 * sizeof(TD)) == 0);
     const int64_t n4 = vec ? (n >> 2) : 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    constexpr int U = 4;  // groups per thread and sweep: all their loads in flight before the stores
-    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < n4; i0 += U * stride) {
-        float ev[U][4], dv[U][4];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        float ev[4], dv[4];
+        load4(e + 4 * i, ev);
+        load4(d + 4 * i, dv);
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int64_t i = i0 + u * stride;
-            if (i < n4) {
-                load4(e + 4 * i, ev[u]);
-                load4(d + 4 * i, dv[u]);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int64_t i = i0 + u * stride;
-            if (i < n4) {
-#pragma unroll
-                for (int k = 0; k < 4; ++k) ev[u][k] = ev[u][k] - a * dv[u][k];
-                store4(e + 4 * i, ev[u]);
-            }
-        }
+        for (int u = 0; u < 4; ++u) ev[u] = ev[u] - a * dv[u];
+        store4(e + 4 * i, ev);
     }
     for (int64_t i = 4 * n4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
         e[i] = from_float<TE>(to_float(e[i]) - a * to_float(d[i]));
@@ -204,7 +191,7 @@ __global__ __launch_bounds__(256) void axpy_kernel(TE *__restrict__ e, const TD 
 }
 
 static int launch_axpy(void *e, int edt, const void *d, int ddt, const double *alpha, int64_t n, hipStream_t st) {
-    const int64_t blocks = std::min<int64_t>((n / 4 + 255) / 256 + 1, 4096);
+    const int64_t blocks = std::min<int64_t>((n / 4 + 255) / 256 + 1, 8192);
     if (n == 0) return MBX_OK;
     if (edt == MBX_DTYPE_F32 && ddt == MBX_DTYPE_F32)
         hipLaunchKernelGGL((axpy_kernel<float, float>), dim3((unsigned)blocks), dim3(256), 0, st, (float *)e,
@@ -541,7 +528,7 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
         if (rc != MBX_OK) return rc;
     }
     // (l) error sinogram update
-    const int64_t blocks = std::min<int64_t>((nsino / 4 + 255) / 256 + 1, 4096);
+    const int64_t blocks = std::min<int64_t>((nsino / 4 + 255) / 256 + 1, 8192);
     if (e_dtype == MBX_DTYPE_F32 && delta_dtype == MBX_DTYPE_F32)
         hipLaunchKernelGGL((axpy_scal_kernel<float, float>), dim3((unsigned)blocks), dim3(256), 0, st, (float *)e,
                            (const float *)dsino, scal, nsino);
