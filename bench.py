@@ -176,6 +176,20 @@ def copy_calibration(nbytes=1 << 30):
     return dict(bytes=2 * nbytes, GBps=2 * nbytes / t / 1e9)
 
 
+def profiled_traffic(kernel, workload, precision):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy), or None."""
+    import glob
+    import re
+    pat = re.compile(r"traffic_bytes_per_launch %s %s %s: ([0-9.eE+]+)" % (kernel, workload, precision))
+    for f in sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "*.md"))):
+        with open(f) as fh:
+            m = pat.search(fh.read())
+        if m:
+            return float(m.group(1)), os.path.relpath(f, os.path.dirname(os.path.abspath(__file__)))
+    return None, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +220,8 @@ def main():
     t_dom = r["t_%s_ms" % dominant] / 1e3
     b_dom = r["b_%s" % dominant]
     achieved = b_dom / t_dom / 1e9
+    kname = "back_kernel" if dominant == "back" else "fwd_kernel"
+    traffic, traffic_src = profiled_traffic(kname, args.workload, main_p)
     V, R, C = model.get_params('sinogram_shape')
     line = {
         "metric": METRIC,
@@ -226,9 +242,9 @@ def main():
                                                                part.shape[0], part.shape[1]),
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "back_kernel" if dominant == "back" else "fwd_kernel",
-                     "bytes_per_launch": b_dom, "avg_launch_ms": t_dom * 1e3},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": kname, "bytes_per_launch": b_dom, "avg_launch_ms": t_dom * 1e3,
+                     "traffic_source": traffic_src},
         "projector_GBps": r["proj_GBps"],
         "precisions": {k: {kk: vv for kk, vv in v.items() if kk not in ("b_back", "b_fwd")}
                        for k, v in results.items()},
