@@ -213,3 +213,53 @@ def test_c4_subproblem_mixed_and_fp32_vs_oracle(gpu):
         print("C4 sub-problem %s NRMSE vs FP64 oracle %.3e (vs phantom %.3f)" % (
             precision, nrmse, np.linalg.norm(r - phantom) / np.linalg.norm(phantom)), flush=True)
         assert nrmse <= bar
+
+
+def test_c1_parallel_64_recon_vs_oracle(gpu):
+    """C1: ParallelBeamModel, 64 views over [0, pi), 64 x 64 detector, 64^3
+    recon, ellipsoid phantom + 1 % noise, sharpness 0, 10 iterations."""
+    V, R, C = 64, 64, 64
+    ang = np.linspace(0, np.pi, V, endpoint=False)
+    m = mj.ParallelBeamModel((V, R, C), ang)
+    g = op.Geometry("parallel", (V, R, C), ang)
+    assert m.get_params('recon_shape') == g.recon_shape == (64, 64, 64)
+    phantom = mj.utilities.gen_random_ellipsoid_phantom(g.recon_shape, seed=0)
+    ror = np.flatnonzero(opart.get_2d_ror_mask(g.recon_shape))
+    sino = op.forward(g, phantom.reshape(-1, 64)[ror], ror)
+    sino = (sino + 0.01 * np.max(sino) * np.random.default_rng(1).standard_normal(sino.shape)).astype(np.float32)
+    m.set_params(sharpness=0.0, verbose=0)
+    np.random.seed(0)
+    recon, _ = m.recon(sino, max_iterations=10, stop_threshold_change_pct=0, print_logs=False, logfile_path=None)
+    np.random.seed(0)
+    ref, _ = ovcd.OracleModel(g, sharpness=0.0).recon(sino, max_iterations=10, stop_threshold_change_pct=0)
+    r = recon.cpu().numpy().astype(np.float64)
+    nrmse = np.linalg.norm(r - ref) / np.linalg.norm(ref)
+    print("C1 recon NRMSE vs oracle %.3e, vs phantom %.3f" % (nrmse, np.linalg.norm(r - phantom) /
+                                                                np.linalg.norm(phantom)), flush=True)
+    assert nrmse <= 1e-3
+
+
+def test_c2_cone_recon_vs_oracle(gpu):
+    """C2: cone 128 views over [0, 2 pi), 128 x 192 detector, sdd 768, sid 384
+    (recon 192 x 192 x 128), ellipsoid phantom + 1 % noise.  The FP64 oracle
+    needs ~20 s of host time per iteration at this size, so the comparison
+    runs 3 iterations (the first three granularities, K = 1, 4, 16)."""
+    V, R, C = 128, 128, 192
+    ang = np.linspace(0, 2 * np.pi, V, endpoint=False)
+    m = mj.ConeBeamModel((V, R, C), ang, source_detector_dist=768.0, source_iso_dist=384.0)
+    g = op.Geometry("cone", (V, R, C), ang, 768.0, 384.0)
+    assert m.get_params('recon_shape') == g.recon_shape == (192, 192, 128)
+    phantom = mj.utilities.gen_random_ellipsoid_phantom(g.recon_shape, seed=0)
+    ror = np.flatnonzero(opart.get_2d_ror_mask(g.recon_shape))
+    sino = op.forward(g, phantom.reshape(-1, 128)[ror], ror)
+    sino = (sino + 0.01 * np.max(sino) * np.random.default_rng(1).standard_normal(sino.shape)).astype(np.float32)
+    m.set_params(sharpness=0.0, verbose=0)
+    np.random.seed(0)
+    recon, _ = m.recon(sino, max_iterations=3, stop_threshold_change_pct=0, print_logs=False, logfile_path=None)
+    np.random.seed(0)
+    ref, _ = ovcd.OracleModel(g, sharpness=0.0).recon(sino, max_iterations=3, stop_threshold_change_pct=0)
+    r = recon.cpu().numpy().astype(np.float64)
+    nrmse = np.linalg.norm(r - ref) / np.linalg.norm(ref)
+    print("C2 3-iteration recon NRMSE vs oracle %.3e, vs phantom %.3f" % (
+        nrmse, np.linalg.norm(r - phantom) / np.linalg.norm(phantom)), flush=True)
+    assert nrmse <= 1e-3
