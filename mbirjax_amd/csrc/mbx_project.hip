@@ -752,19 +752,58 @@ FwdPlan plan_fwd(const DevGeom &g, int nviews) {
 static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 static int fwd_nbk(const DevGeom &g) { return g.C + 2 * g.rh; }  // centre-channel buckets n_c + r
 
+// Forward record of one (view, pixel) pair, stored in the view's binned order
+// (centre channel, then pixel): what the forward update reads per candidate
+// pixel, so that a block walks its candidates through contiguous scalar loads.
+//   [0..2] Ac(n_c-1 .. n_c+1)   [3] W   [4] W_lo
+//   [5] acr (acr - (1-W)/2 in the two-slice mode)   [6] 1/W   [7] -acr/W
+//   [8] ai (float)   [9], [10] clamp range of the resampling slice index,
+//   relative to kref   [11] kf | span << 16: the slices kf .. kf+span-1 reach
+//   a detector row   [12] bit g: a row of 128g .. 128g+127 is reached (all
+//   bits when the detector has more than 4096 rows)   [13] (1-W)/2
+//   [14] v/sdd at kref   [15] its step per slice
+constexpr int kFRec = 16;
+
 // Forward scratch after the geometry table: per-block FP64 partials of the
-// squared norm, then per view the pixel list sorted by centre channel and
-// the bucket starts.
+// squared norm, then per view the pixel list sorted by centre channel, the
+// bucket starts and the forward records in that order.
 int64_t fwd_aux_bytes(const DevGeom &g, int chunk, int nviews) {
     return (int64_t)(al256((size_t)plan_fwd(g, nviews).nblocks * sizeof(double)) +
                      al256((size_t)nviews * chunk * sizeof(int)) +
-                     al256((size_t)nviews * (fwd_nbk(g) + 1) * sizeof(int)));
+                     al256((size_t)nviews * (fwd_nbk(g) + 1) * sizeof(int)) +
+                     al256((size_t)nviews * chunk * kFRec * sizeof(float)));
 }
 
 int64_t fwd_workspace_bytes(const DevGeom &g, int npix, int nviews) {
     if (nviews == 0) return 0;  // an empty pixel set still writes the views (zeros or unchanged)
     const int chunk = table_chunk(g, std::max(npix, 1), nviews);
     return table_bytes(chunk, nviews) + fwd_aux_bytes(g, chunk, nviews);
+}
+
+// Vertical resampling of a staged cylinder onto row m (row-driven fan,
+// mbirjax/cone_beam.py:347-372).  The weights clip((W+1)/2 - |t_k - m|, 0,
+// min(1, W)) of the slices k are a box of width W convolved with a box of
+// width 1, sampled at spacing W, so over all k they sum to 1.  Three modes:
+//   kVertGeneral: every tap of the bp_psf_radius window with its own weight;
+//   kVertUnit:    W >= 1/2 for every pixel (bp_psf_radius 1): the slices
+//                 k_r -+ 1 around the nearest k_r carry clip((1-W)/2 -+ dm, 0, 1)
+//                 (never above 1/2 <= min(1, W)) and k_r the rest of 1;
+//   kVertTwo:     W >= 1 for every pixel: only k0 = floor((m - t_0)/W) and
+//                 k0 + 1 are non-zero, the latter with f = clip(m - t_k0 +
+//                 (1-W)/2, 0, 1), the former with 1 - f.
+// The smallest pixel magnification is bounded through the recon's circumradius.
+constexpr int kVertGeneral = 0, kVertUnit = 1, kVertTwo = 2;
+
+static int fwd_vert_mode(const DevGeom &g) {
+    if (g.kind != MBX_GEOM_CONE || g.rb != 1) return kVertGeneral;
+    const mbx_geom &h = g.h;
+    const double rmax = h.delta_voxel * std::sqrt(0.25 * (double)h.num_recon_rows * h.num_recon_rows +
+                                                  0.25 * (double)h.num_recon_cols * h.num_recon_cols);
+    const double inv_sdd = std::isfinite(h.source_detector_dist) ? 1.0 / h.source_detector_dist : 0.0;
+    const double den = 1.0 / h.magnification + rmax * inv_sdd;
+    if (!(den > 0)) return kVertGeneral;
+    const double wmin = (h.delta_voxel / den) / h.delta_det_row;
+    return wmin >= 1.0 ? kVertTwo : wmin >= 0.5 ? kVertUnit : kVertGeneral;
 }
 
 template <typename T>
@@ -781,13 +820,17 @@ __device__ __forceinline__ T block_sum(T v, T *scratch) {
 }
 
 // Per view: the subset pixels sorted by centre channel n_c (bucket n_c + r),
-// ascending pixel slot within a bucket, and the bucket starts.  Pixels whose
-// taps miss the detector are dropped.  Counting sort with LDS counters, then
-// every bucket is put in pixel order, so the result (and the forward
-// summation order) does not depend on atomic ordering.
-__global__ __launch_bounds__(256) void fwd_bin_kernel(const int *__restrict__ NC, int npix, int rad, int nbk,
-                                                      int *__restrict__ list, int *__restrict__ cstart) {
+// ascending pixel slot within a bucket, the bucket starts, and the forward
+// records in that order.  Pixels whose taps miss the detector are dropped.
+// Counting sort with LDS counters, then every bucket is put in pixel order,
+// so the result (and the forward summation order) does not depend on atomic
+// ordering.
+__global__ __launch_bounds__(256) void fwd_bin_kernel(DevGeom g, const float *__restrict__ T,
+                                                      const int *__restrict__ NC, int npix, int nbk, int vmode,
+                                                      int *__restrict__ list, int *__restrict__ cstart,
+                                                      float *__restrict__ FR) {
     extern __shared__ int cnt[];  // [nbk]
+    const int rad = g.rh;
     const int v = blockIdx.x, tid = threadIdx.x;
     const int *ncv = NC + (size_t)v * npix;
     int *lv = list + (size_t)v * npix;
@@ -805,6 +848,7 @@ __global__ __launch_bounds__(256) void fwd_bin_kernel(const int *__restrict__ NC
     int run = 0;
     for (int b = b0; b < b1; ++b) run += cnt[b];
     __shared__ int wsum[4];
+    __shared__ int total;
     int incl = run;
     for (int o = 1; o < 64; o <<= 1) {
         const int y = __shfl_up(incl, o, 64);
@@ -820,7 +864,10 @@ __global__ __launch_bounds__(256) void fwd_bin_kernel(const int *__restrict__ NC
         cnt[b] = base;  // placement cursor
         base += c;
     }
-    if (tid == blockDim.x - 1) cs[nbk] = base;
+    if (tid == blockDim.x - 1) {
+        cs[nbk] = base;
+        total = base;
+    }
     __syncthreads();
     for (int s = tid; s < npix; s += blockDim.x) {
         const int b = ncv[s] + rad;
@@ -840,54 +887,91 @@ __global__ __launch_bounds__(256) void fwd_bin_kernel(const int *__restrict__ NC
             lv[j + 1] = x;
         }
     }
-}
-
-// Staging metadata of one candidate pixel for this block.
-struct FwdSlot {
-    int sc, kf, span;
-    int wmask;  // bit w: the pixel reaches a row of wave w
-    float c0, c1;
-};
-
-template <int GEOM, int RB>
-__device__ __forceinline__ FwdSlot fwd_slot(const DevGeom &g, const float *__restrict__ Tv, int sc, int m0, int m1,
-                                            int nw, int rpw) {
-    FwdSlot q;
-    q.sc = sc;
-    q.c0 = 0.0f;
-    q.c1 = 0.0f;
-    q.wmask = (1 << nw) - 1;
-    if (GEOM == MBX_GEOM_CONE) {
-        const float *rec = Tv + (size_t)sc * kRec;
-        const float acr = rec[6], invW = rec[11];
-        q.c0 = rec[12];
-        q.c1 = rec[13];
-        const int ai = __float_as_int(rec[14]);
-        const int rlo = max(m0, rec_mlo(rec[15]));
-        const int rhi = min(m1 - 1, rec_mhi(rec[15]));
-        q.kf = max(g.kref + (int)rintf(((float)(rlo - ai) - acr) * invW) - RB, 0);
-        const int kl = min(g.kref + (int)rintf(((float)(rhi - ai) - acr) * invW) + RB, g.Nz - 1);
-        q.span = (rlo <= rhi && q.kf <= kl) ? kl - q.kf + 1 : 0;
-        int mask = 0;
-        for (int w = 0; w < nw; ++w)
-            if (rlo <= m0 + (w + 1) * rpw - 1 && rhi >= m0 + w * rpw) mask |= 1 << w;
-        q.wmask = q.span > 0 ? mask : 0;  // nothing is staged for an empty span
-    } else {
-        q.kf = m0;
-        q.span = m1 - m0;
+    __syncthreads();
+    __threadfence_block();
+    // forward records in binned order
+    const int n = total;
+    for (int i = tid; i < n; i += blockDim.x) {
+        const int s = lv[i];
+        const float4 *r4 = (const float4 *)(T + ((size_t)v * npix + s) * kRec);
+        const float4 R0 = r4[0], R1 = r4[1], R2 = r4[2], R3 = r4[3];
+        float o[kFRec];
+        o[0] = R0.z;
+        o[1] = R0.w;
+        o[2] = R1.x;
+        int rlo = 0, rhi = g.R - 1, kf = 0, span = 0;
+        constexpr int P = kFwdPad;
+        float clo = 0.0f, chi = 0.0f;
+        if (g.kind == MBX_GEOM_CONE) {
+            const float acr = R1.z, W = R1.w, Wlo = R2.x, hv = R2.y, invW = R2.w;
+            const int ai = __float_as_int(R3.z);
+            rlo = rec_mlo(R3.w);
+            rhi = rec_mhi(R3.w);
+            // slices whose taps reach rows rlo .. rhi (nearest slice -+ rb)
+            kf = max(g.kref + (int)rintf(((float)(rlo - ai) - acr) * invW) - g.rb, 0);
+            const int kl = min(g.kref + (int)rintf(((float)(rhi - ai) - acr) * invW) + g.rb, g.Nz - 1);
+            span = (rlo <= rhi && kf <= kl) ? kl - kf + 1 : 0;
+            if (span == 0) {  // no wave's rows lie in [0xffff, 0]
+                kf = 0;
+                rlo = 0xffff;
+                rhi = 0;
+            }
+            const float q1 = hv - W;
+            o[3] = W;
+            o[4] = Wlo;
+            o[5] = vmode == kVertTwo ? acr - q1 : acr;
+            o[6] = invW;
+            o[7] = -(acr * invW);
+            o[8] = (float)ai;
+            o[13] = q1;
+            o[14] = R3.x;
+            o[15] = R3.y;
+            // staged slice i of the cylinder is slice kf - P + i; slices kf-P ..
+            // kf-1 and kf+span .. kf+span+P-1 are zeros.  Clamping the nearest
+            // (or lower) slice keeps every tap there; a clamped row only meets zeros.
+            if (vmode == kVertTwo) {
+                clo = (float)(kf - g.kref - 2);
+                chi = (float)(kf - g.kref + span);
+            } else {
+                clo = (float)(kf - g.kref - P + g.rb);
+                chi = (float)(kf - g.kref + span + P - 1 - g.rb);
+            }
+        } else {
+            o[3] = 1.0f;
+            o[4] = 0.0f;
+            o[5] = 0.0f;
+            o[6] = 1.0f;
+            o[7] = 0.0f;
+            o[8] = 0.0f;
+            o[13] = 0.0f;
+            o[14] = 0.0f;
+            o[15] = 0.0f;
+        }
+        o[9] = clo;
+        o[10] = chi;
+        o[11] = __int_as_float(kf | (span << 16));
+        unsigned gm = 0;
+        if (g.R > 32 * 128) {
+            gm = rlo <= rhi ? 0xffffffffu : 0u;
+        } else if (rlo <= rhi) {
+            for (int q = rlo >> 7; q <= (rhi >> 7); ++q) gm |= 1u << q;
+        }
+        o[12] = __uint_as_float(gm);
+        float4 *dst = (float4 *)(FR + ((size_t)v * npix + i) * kFRec);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dst[q] = make_float4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
     }
-    return q;
 }
 
 // One workgroup per (view, 16-channel block, row block); lane l of wave w owns
 // detector rows m0 + 128w + l and m0 + 128w + 64 + l and keeps their 16
 // channels in registers.  The block's candidate pixels are a contiguous run
-// of the view's binned list (sorted by centre channel, then pixel) and are
+// of the view's binned records (sorted by centre channel, then pixel) and are
 // handled in batches, double-buffered in LDS:
-//   staging: the slices of each candidate's cylinder that the block's rows
-//            reach are copied global -> LDS asynchronously (buffer loads into
-//            LDS) while the previous batch is being used, then scaled in place
-//            by 1/cos(phi_k) with zero pads;
+//   staging: the slices of each candidate's cylinder that reach the detector
+//            are copied global -> LDS asynchronously (buffer loads into LDS)
+//            while the previous batch is being used, then scaled in place by
+//            1/cos(phi_k) with zero pads;
 //   update:  for each centre channel (a compile-time loop, so every tap lands
 //            in a fixed register) every wave resamples that channel's staged
 //            cylinders onto its rows (row-driven vertical fan,
@@ -895,9 +979,10 @@ __device__ __forceinline__ FwdSlot fwd_slot(const DevGeom &g, const float *__res
 //            taps (mbirjax/cone_beam.py:276-282) and adds them to its tile.
 // Each output element has one writer and a fixed summation order; the tile is
 // written to HBM once.
-template <int GEOM, int RAD, int RB, typename OT, int LAYOUT>
+template <int GEOM, int RAD, int RB, int VM, typename OT, int LAYOUT>
 __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__restrict__ T,
-                                                  const int *__restrict__ list, const int *__restrict__ cstart,
+                                                  const float *__restrict__ FR, const int *__restrict__ list,
+                                                  const int *__restrict__ cstart,
                                                   const float *__restrict__ vox, int npix, int ncb, int nrb,
                                                   int nbatch, int cstride, OT *__restrict__ sino, int accumulate,
                                                   const float *__restrict__ w, double *__restrict__ part_sumsq) {
@@ -905,7 +990,7 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
     constexpr int ND = CB + 2 * RAD;  // centre-channel buckets n0 - RAD .. n0 + CB - 1 + RAD
     extern __shared__ float lds[];
     const int nw = blockDim.x >> 6;
-    float *bufs = lds;                                            // [2][nbatch][cstride]
+    float *bufs = lds;                                                  // [2][nbatch][cstride]
     double *dscratch = (double *)(lds + 2 * (size_t)nbatch * cstride);  // [8]
 
     const int tid = threadIdx.x;
@@ -918,14 +1003,17 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
     const int rows_pb = nw * kWave * RPL;
     const int m0 = rbi * rows_pb, m1 = min(g.R, m0 + rows_pb);
     const int wm0 = m0 + wave * kWave * RPL;  // this wave's rows: wm0 + lane + 64 r
+    const int wml = min(m1, wm0 + kWave * RPL) - 1;
     const float *Tv = T + (size_t)vi * npix * kRec;
+    const float *FRv = FR + (size_t)vi * npix * kFRec;
+    const int *lv = list + (size_t)vi * npix;  // pixel slot of each record
     const int nbk = g.C + 2 * RAD;
-    const int *lv = list + (size_t)vi * npix;
     const int *csv = cstart + (size_t)vi * (nbk + 1);
-    // bucket starts of this block: lane D holds the start of centre channel n0 + D - RAD
-    const int l_bs = lane <= ND ? csv[min(n0 + lane, nbk)] : 0;
-    const int lo = __builtin_amdgcn_readlane(l_bs, 0);
-    const int hi = __builtin_amdgcn_readlane(l_bs, ND);
+    // bucket starts of this block: bs[D] is the start of centre channel n0 + D - RAD
+    int bs[ND + 1];
+#pragma unroll
+    for (int D = 0; D <= ND; ++D) bs[D] = csv[min(n0 + D, nbk)];
+    const int lo = bs[0], hi = bs[ND];
     const int nbat = (hi - lo + nbatch - 1) / nbatch;
 
     float acc[RPL][CB];
@@ -934,23 +1022,32 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
 #pragma unroll
         for (int c = 0; c < CB; ++c) acc[r][c] = 0.0f;
 
-    // lane i of every wave holds slot i's staging data for the batch in use
-    // (cur) and the one being loaded (nxt)
-    FwdSlot cur{0, 0, 0, 0, 0.0f, 0.0f}, nxt{0, 0, 0, 0, 0.0f, 0.0f};
-    auto load_slots = [&](int bi) {
-        FwdSlot q{0, 0, 0, 0, 0.0f, 0.0f};
+    // Staging data of a batch, lane i holding slot i's: kf | span << 16, pixel
+    // slot, v/sdd at kref and its step per slice.  Loaded a batch ahead.
+    struct Meta {
+        int ks, sc;
+        float u0, du;
+    };
+    auto load_meta = [&](int bi) {
+        Meta q{0, 0, 0.0f, 0.0f};
         const int i = lo + bi * nbatch + lane;
-        if (lane < nbatch && i < hi) q = fwd_slot<GEOM, RB>(g, Tv, lv[i], m0, m1, nw, kWave * RPL);
+        if (lane < nbatch && i < hi) {
+            const float *rf = FRv + (size_t)i * kFRec;
+            q.ks = GEOM == MBX_GEOM_CONE ? __float_as_int(rf[11]) : (m0 | ((m1 - m0) << 16));
+            q.sc = lv[i];
+            q.u0 = rf[14];
+            q.du = rf[15];
+        }
         return q;
     };
     // asynchronous global -> LDS copy of the raw slices of this wave's slots
-    auto issue = [&](const FwdSlot &q, int bi) {
+    auto issue = [&](const Meta &q, int bi) {
         float *buf = bufs + (size_t)(bi & 1) * nbatch * cstride;
         const int nb = min(nbatch, hi - lo - bi * nbatch);
         for (int slot = wave; slot < nb; slot += nw) {
-            const int span = __builtin_amdgcn_readlane(q.span, slot);
+            const int ks = __builtin_amdgcn_readlane(q.ks, slot);
+            const int kf = ks & 0xffff, span = ks >> 16;
             if (span == 0) continue;
-            const int kf = __builtin_amdgcn_readlane(q.kf, slot);
             const int sc = __builtin_amdgcn_readlane(q.sc, slot);
             // slices past the cylinder end read 0 (num_records bounds the buffer)
             const __amdgpu_buffer_rsrc_t rs =
@@ -965,18 +1062,18 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
         }
     };
     // in-place scaling by 1/cos(phi_k), zero past the span, zero pads
-    auto finish = [&](const FwdSlot &q, int bi) {
+    auto finish = [&](const Meta &q, int bi) {
         float *buf = bufs + (size_t)(bi & 1) * nbatch * cstride;
         const int nb = min(nbatch, hi - lo - bi * nbatch);
         for (int slot = wave; slot < nb; slot += nw) {
-            const int span = __builtin_amdgcn_readlane(q.span, slot);
+            const int ks = __builtin_amdgcn_readlane(q.ks, slot);
+            const int kf = ks & 0xffff, span = ks >> 16;
             if (span == 0) continue;
             float *cyl = buf + (size_t)slot * cstride;
+            const int nch = (span + kWave - 1) / kWave;
             if (GEOM == MBX_GEOM_CONE) {
-                const int kf = __builtin_amdgcn_readlane(q.kf, slot);
-                const float c0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, q.c0), slot));
-                const float c1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, q.c1), slot));
-                const int nch = (span + kWave - 1) / kWave;
+                const float c0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, q.u0), slot));
+                const float c1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, q.du), slot));
                 float uu = fmaf(c1, (float)(kf + lane - g.kref), c0);  // v/sdd of this lane's slice
                 const float du = c1 * (float)kWave;
                 for (int c = 0; c < nch - 1; ++c, uu += du) {
@@ -989,7 +1086,6 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
                     cyl[P + kk] = kk < span ? x : 0.0f;
                 }
             } else {
-                const int nch = (span + kWave - 1) / kWave;
                 for (int c = 0; c < nch; ++c) {
                     const int kk = c * kWave + lane;
                     if (kk >= span) cyl[P + kk] = 0.0f;
@@ -1002,73 +1098,116 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
         }
     };
 
+    float mrow[RPL];  // this lane's rows
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) mrow[r] = (float)(wm0 + lane + kWave * r);
+    const int wgrp = (wm0 >> 7) & 31;  // this wave's 128-row group in the record masks
+
+    Meta mA{0, 0, 0.0f, 0.0f}, mB{0, 0, 0.0f, 0.0f};
     if (nbat > 0) {
-        nxt = load_slots(0);
-        issue(nxt, 0);
+        mA = load_meta(0);
+        issue(mA, 0);
+        if (nbat > 1) mB = load_meta(1);
     }
     for (int bi = 0; bi < nbat; ++bi) {
-        cur = nxt;
-        __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of batch bi have landed
-        finish(cur, bi);
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of batch bi (and the next batch's data) have landed
+        finish(mA, bi);
         __syncthreads();  // batch bi staged by every wave; batch bi-1 no longer read
         if (bi + 1 < nbat) {
-            nxt = load_slots(bi + 1);
-            issue(nxt, bi + 1);
+            issue(mB, bi + 1);
+            mA = mB;
+            if (bi + 2 < nbat) mB = load_meta(bi + 2);
         }
         const int b0 = lo + bi * nbatch;
         const int nb = min(nbatch, hi - b0);
-        const float *buf = bufs + (size_t)(bi & 1) * nbatch * cstride;
+        const int boff = (bi & 1) * nbatch * cstride;  // LDS float offset of this batch
         if (wm0 < m1) {  // wave-uniform
 #pragma unroll
             for (int D = 0; D < ND; ++D) {
-                const int slo = max(__builtin_amdgcn_readlane(l_bs, D), b0) - b0;
-                const int shi = min(__builtin_amdgcn_readlane(l_bs, D + 1), b0 + nb) - b0;
-                for (int slot = slo; slot < shi; ++slot) {
-                    if (!((__builtin_amdgcn_readlane(cur.wmask, slot) >> wave) & 1))
-                        continue;  // no row of this wave is reached (wave-uniform)
-                    const int span = __builtin_amdgcn_readlane(cur.span, slot);
-                    const int kf = __builtin_amdgcn_readlane(cur.kf, slot);
-                    const int sc = __builtin_amdgcn_readlane(cur.sc, slot);
-                    const float4 *r4 = (const float4 *)(Tv + (size_t)sc * kRec);
-                    const float4 R0 = r4[0], R1 = r4[1], R2 = r4[2], R3 = r4[3];
-                    const float Aall[5] = {R0.y, R0.z, R0.w, R1.x, R1.y};  // Ac(n_c - 2 .. n_c + 2)
-                    const float kbase = (float)(g.kref + P - kf);  // staged index of slice kref
-                    // cyl[i] is slice kf - P + i; the index is clamped so that the
-                    // taps stay on the staged slices or their zero pads
-                    const float *cyl = buf + (size_t)slot * cstride;
-                    const float klo = (float)(P - RB - 1), khi = (float)(P + span + RB);
-                    const float acr = R1.z, W = R1.w, Wlo = R2.x, hv = R2.y, Lv = R2.z, invW = R2.w;
-                    const float aif = (float)__float_as_int(R3.z), nai = -(acr * invW);
-                    const float q1 = hv - W, q2 = hv - 2.0f * W;  // taps one and two slices off the nearest
+                const int slo = max(bs[D], b0) - b0;
+                const int shi = min(bs[D + 1], b0 + nb) - b0;
+                const float4 *rp = (const float4 *)(FRv + (size_t)(b0 + slo) * kFRec);
+                // LDS index of slice kref of slot s, less its kf: cyl[i] is slice kf - P + i
+                int sb = boff + slo * cstride + g.kref + P - (VM == kVertTwo ? 0 : RB);
+                for (int slot = slo; slot < shi; ++slot, rp += 4, sb += cstride) {
+                    const float4 R0 = rp[0], R1 = rp[1], R2 = rp[2], R3 = rp[3];
+                    // the whole record arrives in one round trip, before the test below
+                    asm volatile("" ::"s"(R0.x), "s"(R0.y), "s"(R0.z), "s"(R0.w), "s"(R1.x), "s"(R1.y), "s"(R1.z),
+                                 "s"(R1.w), "s"(R2.x), "s"(R2.y), "s"(R2.z), "s"(R2.w), "s"(R3.x), "s"(R3.y));
+                    float v[RPL];
+                    if (GEOM == MBX_GEOM_CONE) {
+                        if (!((__float_as_uint(R3.x) >> wgrp) & 1u)) continue;  // no row of this wave is reached
+                        const float W = R0.w, Wlo = R1.x, acr = R1.y, invW = R1.z, nai = R1.w, aif = R2.x;
+                        const float clo = R2.y, chi = R2.z, q1 = R3.y;
+                        // base at the lowest tap (LDS offsets are unsigned)
+                        int ib = sb - (__float_as_int(R2.w) & 0xffff);
+                        asm volatile("" : "+s"(ib));  // one scalar base: the address is one shift-add per row
+                        if (VM == kVertTwo) {
 #pragma unroll
-                    for (int r = 0; r < RPL; ++r) {
-                        const float mf = (float)(wm0 + lane + kWave * r);
-                        float v;
-                        if (GEOM == MBX_GEOM_CONE) {
-                            const float mp = mf - aif;                    // m - ai, exact
-                            const float kr = rintf(fmaf(mp, invW, nai));  // nearest slice - kref
-                            // t - m for slice kref + kr: the first fma rounds once, at |t - m| < 1
-                            const float dm = fmaf(W, kr, -mp) + fmaf(Wlo, kr, acr);
-                            const float *cp = cyl + (int)__builtin_amdgcn_fmed3f(kr + kbase, klo, khi);
-                            // |dm| <= W/2, so |dm -+ jW| = jW -+ dm
-                            v = __builtin_amdgcn_fmed3f(hv - fabsf(dm), 0.0f, Lv) * cp[0];
-                            v = fmaf(__builtin_amdgcn_fmed3f(q1 + dm, 0.0f, Lv), cp[-1], v);
-                            v = fmaf(__builtin_amdgcn_fmed3f(q1 - dm, 0.0f, Lv), cp[1], v);
-                            if (RB == 2) {
-                                v = fmaf(__builtin_amdgcn_fmed3f(q2 + dm, 0.0f, Lv), cp[-2], v);
-                                v = fmaf(__builtin_amdgcn_fmed3f(q2 - dm, 0.0f, Lv), cp[2], v);
+                            for (int r = 0; r < RPL; ++r) {
+                                const float mp = mrow[r] - aif;  // m - ai, exact
+                                const float jf = floorf(__builtin_amdgcn_fmed3f(fmaf(mp, invW, nai), clo, chi));
+                                // t_k0 - m - (1-W)/2: the first fma rounds once, at |t - m| < W
+                                const float dm = fmaf(W, jf, -mp) + fmaf(Wlo, jf, acr);
+                                const float f = __builtin_amdgcn_fmed3f(-dm, 0.0f, 1.0f);
+                                const float *cp = lds + (ib + (int)jf);
+                                const float c0 = cp[0], c1 = cp[1];
+                                v[r] = fmaf(f, c1 - c0, c0);
+                            }
+                        } else if (VM == kVertUnit) {
+#pragma unroll
+                            for (int r = 0; r < RPL; ++r) {
+                                const float mp = mrow[r] - aif;  // m - ai, exact
+                                const float kr = rintf(__builtin_amdgcn_fmed3f(fmaf(mp, invW, nai), clo, chi));
+                                // t - m for slice kref + kr: the first fma rounds once, at |t - m| < 1
+                                const float dm = fmaf(W, kr, -mp) + fmaf(Wlo, kr, acr);
+                                const float *cp = lds + (ib + (int)kr);  // slice kr - 1
+                                const float cm = cp[0], c0 = cp[1], cq = cp[2];
+                                // |dm| <= W/2, so |t -+ W - m| = W -+ dm
+                                const float wm = __builtin_amdgcn_fmed3f(q1 + dm, 0.0f, 1.0f);
+                                const float wq = __builtin_amdgcn_fmed3f(q1 - dm, 0.0f, 1.0f);
+                                v[r] = fmaf(wq, cq - c0, fmaf(wm, cm - c0, c0));
                             }
                         } else {
-                            // row m sees slice m (rows past the block read don't-cares)
-                            v = buf[(size_t)slot * cstride + P + (int)mf - m0];
+                            const float *rec = Tv + (size_t)lv[b0 + slot] * kRec;
+                            const float hv = rec[9], Lv = rec[10], q2 = q1 - W;  // q2: taps two slices off the nearest
+#pragma unroll
+                            for (int r = 0; r < RPL; ++r) {
+                                const float mp = mrow[r] - aif;  // m - ai, exact
+                                const float kr = rintf(__builtin_amdgcn_fmed3f(fmaf(mp, invW, nai), clo, chi));
+                                const float dm = fmaf(W, kr, -mp) + fmaf(Wlo, kr, acr);
+                                const float *cp = lds + (ib + (int)kr) + RB;  // slice kr
+                                // |dm| <= W/2, so |dm -+ jW| = jW -+ dm
+                                float x = __builtin_amdgcn_fmed3f(hv - fabsf(dm), 0.0f, Lv) * cp[0];
+                                x = fmaf(__builtin_amdgcn_fmed3f(q1 + dm, 0.0f, Lv), cp[-1], x);
+                                x = fmaf(__builtin_amdgcn_fmed3f(q1 - dm, 0.0f, Lv), cp[1], x);
+                                if (RB == 2) {
+                                    x = fmaf(__builtin_amdgcn_fmed3f(q2 + dm, 0.0f, Lv), cp[-2], x);
+                                    x = fmaf(__builtin_amdgcn_fmed3f(q2 - dm, 0.0f, Lv), cp[2], x);
+                                }
+                                v[r] = x;
+                            }
                         }
-                        // bucket D is centre channel n0 + D - RAD: tap t lands on channel D - 2*RAD + t
+                    } else {
+                        // row m sees slice m (rows past the block read don't-cares)
+#pragma unroll
+                        for (int r = 0; r < RPL; ++r)
+                            v[r] = lds[boff + slot * cstride + P + wm0 + lane + kWave * r - m0];
+                    }
+                    // bucket D is centre channel n0 + D - RAD: tap t lands on channel D - 2*RAD + t
+                    float Aall[5] = {0.0f, R0.x, R0.y, R0.z, 0.0f};  // Ac(n_c - 2 .. n_c + 2)
+                    if (RAD == 2) {
+                        const float *rec = Tv + (size_t)lv[b0 + slot] * kRec;
+                        Aall[0] = rec[1];
+                        Aall[4] = rec[5];
+                    }
+#pragma unroll
+                    for (int r = 0; r < RPL; ++r)
 #pragma unroll
                         for (int t = 0; t <= 2 * RAD; ++t) {
                             const int c = D - 2 * RAD + t;
-                            if (c >= 0 && c < CB) acc[r][c] = fmaf(Aall[2 - RAD + t], v, acc[r][c]);
+                            if (c >= 0 && c < CB) acc[r][c] = fmaf(Aall[2 - RAD + t], v[r], acc[r][c]);
                         }
-                    }
                 }
             }
         }
@@ -1104,33 +1243,34 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
     }
 }
 
-template <int GEOM, int RAD, int RB, typename OT, int LAYOUT>
-static void launch_fwd_t(const DevGeom &g, const FwdPlan &pl, const float *T, const int *list, const int *cstart,
+template <int GEOM, int RAD, int RB, int VM, typename OT, int LAYOUT>
+static void launch_fwd_t(const DevGeom &g, const FwdPlan &pl, const float *T, const float *FR, const int *list,
+                         const int *cstart,
                          const float *vox, int npix, OT *sino, int accumulate, const float *w, double *part,
                          hipStream_t st) {
-    hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, OT, LAYOUT>), dim3((unsigned)pl.nblocks), dim3(64 * pl.nw), pl.lds,
-                       st, g, T, list, cstart, vox, npix, pl.ncb, pl.nrb, pl.nb, pl.cstride, sino, accumulate, w,
+    hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, VM, OT, LAYOUT>), dim3((unsigned)pl.nblocks), dim3(64 * pl.nw),
+                       pl.lds, st, g, T, FR, list, cstart, vox, npix, pl.ncb, pl.nrb, pl.nb, pl.cstride, sino, accumulate, w,
                        part);
 }
 
-template <int GEOM, int RAD, int RB>
-static void dispatch_fwd_2(const DevGeom &g, const FwdPlan &pl, const float *T, const int *list, const int *cstart,
-                           const float *vox, int npix, void *sino, int dt, int layout, int accumulate, const float *w,
-                           double *part, hipStream_t st) {
+template <int GEOM, int RAD, int RB, int VM>
+static void dispatch_fwd_2(const DevGeom &g, const FwdPlan &pl, const float *T, const float *FR, const int *list,
+                           const int *cstart, const float *vox, int npix, void *sino, int dt, int layout,
+                           int accumulate, const float *w, double *part, hipStream_t st) {
     if (dt == MBX_DTYPE_F32) {
         if (layout == MBX_LAYOUT_VCR)
-            launch_fwd_t<GEOM, RAD, RB, float, MBX_LAYOUT_VCR>(g, pl, T, list, cstart, vox, npix, (float *)sino,
-                                                               accumulate, w, part, st);
+            launch_fwd_t<GEOM, RAD, RB, VM, float, MBX_LAYOUT_VCR>(g, pl, T, FR, list, cstart, vox, npix,
+                                                                   (float *)sino, accumulate, w, part, st);
         else
-            launch_fwd_t<GEOM, RAD, RB, float, MBX_LAYOUT_VRC>(g, pl, T, list, cstart, vox, npix, (float *)sino,
-                                                               accumulate, w, part, st);
+            launch_fwd_t<GEOM, RAD, RB, VM, float, MBX_LAYOUT_VRC>(g, pl, T, FR, list, cstart, vox, npix,
+                                                                   (float *)sino, accumulate, w, part, st);
     } else {
         if (layout == MBX_LAYOUT_VCR)
-            launch_fwd_t<GEOM, RAD, RB, bf16, MBX_LAYOUT_VCR>(g, pl, T, list, cstart, vox, npix, (bf16 *)sino,
-                                                              accumulate, w, part, st);
+            launch_fwd_t<GEOM, RAD, RB, VM, bf16, MBX_LAYOUT_VCR>(g, pl, T, FR, list, cstart, vox, npix,
+                                                                  (bf16 *)sino, accumulate, w, part, st);
         else
-            launch_fwd_t<GEOM, RAD, RB, bf16, MBX_LAYOUT_VRC>(g, pl, T, list, cstart, vox, npix, (bf16 *)sino,
-                                                              accumulate, w, part, st);
+            launch_fwd_t<GEOM, RAD, RB, VM, bf16, MBX_LAYOUT_VRC>(g, pl, T, FR, list, cstart, vox, npix,
+                                                                  (bf16 *)sino, accumulate, w, part, st);
     }
 }
 
@@ -1145,37 +1285,48 @@ int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float 
     const int nbk = fwd_nbk(g);
     if (nbk > kBinMax) return fail(MBX_EUNSUPPORTED, "num_det_channels too large for the forward binning");
     if (!aux) return fail(MBX_EWORKSPACE, "forward projection needs a workspace");
+    if (npix > chunk) return fail(MBX_EINVAL, "forward projection chunk larger than its workspace");
     char *a8 = (char *)aux;
     double *part = (double *)a8;
     int *list = (int *)(a8 + al256((size_t)pl.nblocks * sizeof(double)));
     int *cstart = (int *)((char *)list + al256((size_t)nviews * chunk * sizeof(int)));
+    float *FR = (float *)((char *)cstart + al256((size_t)nviews * (nbk + 1) * sizeof(int)));
+    const int vm = fwd_vert_mode(g);
     if (nviews > 0) {
-        hipLaunchKernelGGL(fwd_bin_kernel, dim3(nviews), dim3(256), nbk * sizeof(int), st, NC, npix, g.rh, nbk, list,
-                           cstart);
+        hipLaunchKernelGGL(fwd_bin_kernel, dim3(nviews), dim3(256), nbk * sizeof(int), st, g, T, NC, npix, nbk, vm,
+                           list, cstart, FR);
         MBX_LAUNCH_CHECK();
     }
     double *p = sumsq ? part : nullptr;
+#define MBX_FWD(G, RA, RB_, VM_)                                                                                      \
+    dispatch_fwd_2<G, RA, RB_, VM_>(g, pl, T, FR, list, cstart, vox, npix, sino, sino_dtype, layout, accumulate,      \
+                                    weights, p, st)
     if (g.kind == MBX_GEOM_CONE) {
-        if (g.rh == 1 && g.rb == 1)
-            dispatch_fwd_2<MBX_GEOM_CONE, 1, 1>(g, pl, T, list, cstart, vox, npix, sino, sino_dtype, layout, accumulate,
-                                                weights, p, st);
-        else if (g.rh == 1)
-            dispatch_fwd_2<MBX_GEOM_CONE, 1, 2>(g, pl, T, list, cstart, vox, npix, sino, sino_dtype, layout, accumulate,
-                                                weights, p, st);
-        else if (g.rb == 1)
-            dispatch_fwd_2<MBX_GEOM_CONE, 2, 1>(g, pl, T, list, cstart, vox, npix, sino, sino_dtype, layout, accumulate,
-                                                weights, p, st);
-        else
-            dispatch_fwd_2<MBX_GEOM_CONE, 2, 2>(g, pl, T, list, cstart, vox, npix, sino, sino_dtype, layout, accumulate,
-                                                weights, p, st);
+        if (vm == kVertTwo) {
+            if (g.rh == 1)
+                MBX_FWD(MBX_GEOM_CONE, 1, 1, kVertTwo);
+            else
+                MBX_FWD(MBX_GEOM_CONE, 2, 1, kVertTwo);
+        } else if (vm == kVertUnit) {
+            if (g.rh == 1)
+                MBX_FWD(MBX_GEOM_CONE, 1, 1, kVertUnit);
+            else
+                MBX_FWD(MBX_GEOM_CONE, 2, 1, kVertUnit);
+        } else if (g.rh == 1 && g.rb == 1) {
+            MBX_FWD(MBX_GEOM_CONE, 1, 1, kVertGeneral);
+        } else if (g.rh == 1) {
+            MBX_FWD(MBX_GEOM_CONE, 1, 2, kVertGeneral);
+        } else if (g.rb == 1) {
+            MBX_FWD(MBX_GEOM_CONE, 2, 1, kVertGeneral);
+        } else {
+            MBX_FWD(MBX_GEOM_CONE, 2, 2, kVertGeneral);
+        }
+    } else if (g.rh == 1) {
+        MBX_FWD(MBX_GEOM_PARALLEL, 1, 1, kVertGeneral);
     } else {
-        if (g.rh == 1)
-            dispatch_fwd_2<MBX_GEOM_PARALLEL, 1, 1>(g, pl, T, list, cstart, vox, npix, sino, sino_dtype, layout,
-                                                    accumulate, weights, p, st);
-        else
-            dispatch_fwd_2<MBX_GEOM_PARALLEL, 2, 1>(g, pl, T, list, cstart, vox, npix, sino, sino_dtype, layout,
-                                                    accumulate, weights, p, st);
+        MBX_FWD(MBX_GEOM_PARALLEL, 2, 1, kVertGeneral);
     }
+#undef MBX_FWD
     MBX_LAUNCH_CHECK();
     if (sumsq) return launch_sum_partials(part, pl.nblocks, sumsq, 0, st);
     return MBX_OK;
