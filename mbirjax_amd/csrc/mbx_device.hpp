@@ -69,6 +69,38 @@ __device__ __forceinline__ void store4(bf16 *p, const float v[4]) {
     *(uint2 *)p = t;
 }
 
+// Streaming variants (nontemporal: data touched once per pass, larger than the caches).
+typedef float mbx_f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t mbx_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void load4_nt(const float *p, float v[4]) {
+    const mbx_f32x4 t = __builtin_nontemporal_load((const mbx_f32x4 *)p);
+    v[0] = t.x;
+    v[1] = t.y;
+    v[2] = t.z;
+    v[3] = t.w;
+}
+__device__ __forceinline__ void load4_nt(const bf16 *p, float v[4]) {
+    const mbx_u32x2 t = __builtin_nontemporal_load((const mbx_u32x2 *)p);
+    v[0] = __uint_as_float(t.x << 16);
+    v[1] = __uint_as_float(t.x & 0xffff0000u);
+    v[2] = __uint_as_float(t.y << 16);
+    v[3] = __uint_as_float(t.y & 0xffff0000u);
+}
+__device__ __forceinline__ void store4_nt(float *p, const float v[4]) {
+    mbx_f32x4 t;
+    t.x = v[0];
+    t.y = v[1];
+    t.z = v[2];
+    t.w = v[3];
+    __builtin_nontemporal_store(t, (mbx_f32x4 *)p);
+}
+__device__ __forceinline__ void store4_nt(bf16 *p, const float v[4]) {
+    mbx_u32x2 t;
+    t.x = (uint32_t)from_float<bf16>(v[0]).bits | ((uint32_t)from_float<bf16>(v[1]).bits << 16);
+    t.y = (uint32_t)from_float<bf16>(v[2]).bits | ((uint32_t)from_float<bf16>(v[3]).bits << 16);
+    __builtin_nontemporal_store(t, (mbx_u32x2 *)p);
+}
+
 // ---------------------------------------------------------------- buffer loads
 // Raw buffer loads through a wave-uniform resource (base address in SGPRs):
 // one 32-bit per-lane byte offset plus an SGPR byte offset, no 64-bit

@@ -978,14 +978,14 @@ __global__ __launch_bounds__(256) void fwd_bin_kernel(DevGeom g, const float *__
 //            mbirjax/cone_beam.py:347-372), weights them with the horizontal
 //            taps (mbirjax/cone_beam.py:276-282) and adds them to its tile.
 // Each output element has one writer and a fixed summation order; the tile is
-// written to HBM once.
+// written to HBM once.  At most four waves per SIMD are resident (the LDS of
+// four blocks per CU).
 template <int GEOM, int RAD, int RB, int VM, typename OT, int LAYOUT>
-__global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__restrict__ T,
-                                                  const float *__restrict__ FR, const int *__restrict__ list,
-                                                  const int *__restrict__ cstart,
-                                                  const float *__restrict__ vox, int npix, int ncb, int nrb,
-                                                  int nbatch, int cstride, OT *__restrict__ sino, int accumulate,
-                                                  const float *__restrict__ w, double *__restrict__ part_sumsq) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void fwd_kernel(
+    DevGeom g, const float *__restrict__ T, const float *__restrict__ FR, const int *__restrict__ list,
+    const int *__restrict__ cstart, const float *__restrict__ vox, int npix, int ncb, int nrb, int nbatch,
+    int cstride, OT *__restrict__ sino, int accumulate, const float *__restrict__ w,
+    double *__restrict__ part_sumsq) {
     constexpr int CB = kFwdCB, P = kFwdPad, RPL = kFwdRPL;
     constexpr int ND = CB + 2 * RAD;  // centre-channel buckets n0 - RAD .. n0 + CB - 1 + RAD
     extern __shared__ float lds[];
@@ -1023,21 +1023,21 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
         for (int c = 0; c < CB; ++c) acc[r][c] = 0.0f;
 
     // Staging data of a batch, lane i holding slot i's: kf | span << 16, pixel
-    // slot, v/sdd at kref and its step per slice.  Loaded a batch ahead.
+    // slot, v/sdd at kref and its step per slice.  Loaded a batch ahead, without
+    // conditions (lanes past the batch load some record of the block and are
+    // never read), so that nothing waits for the loads before they are used.
     struct Meta {
         int ks, sc;
         float u0, du;
     };
     auto load_meta = [&](int bi) {
-        Meta q{0, 0, 0.0f, 0.0f};
-        const int i = lo + bi * nbatch + lane;
-        if (lane < nbatch && i < hi) {
-            const float *rf = FRv + (size_t)i * kFRec;
-            q.ks = GEOM == MBX_GEOM_CONE ? __float_as_int(rf[11]) : (m0 | ((m1 - m0) << 16));
-            q.sc = lv[i];
-            q.u0 = rf[14];
-            q.du = rf[15];
-        }
+        Meta q;
+        const int i = min(lo + bi * nbatch + min(lane, nbatch - 1), hi - 1);
+        const float *rf = FRv + (size_t)i * kFRec;
+        q.ks = GEOM == MBX_GEOM_CONE ? __float_as_int(rf[11]) : (m0 | ((m1 - m0) << 16));
+        q.sc = lv[i];
+        q.u0 = rf[14];
+        q.du = rf[15];
         return q;
     };
     // asynchronous global -> LDS copy of the raw slices of this wave's slots
@@ -1068,20 +1068,34 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
         for (int slot = wave; slot < nb; slot += nw) {
             const int ks = __builtin_amdgcn_readlane(q.ks, slot);
             const int kf = ks & 0xffff, span = ks >> 16;
-            if (span == 0) continue;
+            if (span == 0) continue;  // no row reaches it: never read
             float *cyl = buf + (size_t)slot * cstride;
             const int nch = (span + kWave - 1) / kWave;
             if (GEOM == MBX_GEOM_CONE) {
                 const float c0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, q.u0), slot));
                 const float c1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, q.du), slot));
-                float uu = fmaf(c1, (float)(kf + lane - g.kref), c0);  // v/sdd of this lane's slice
+                const float u0 = fmaf(c1, (float)(kf + lane - g.kref), c0);  // v/sdd of this lane's slice in chunk 0
                 const float du = c1 * (float)kWave;
-                for (int c = 0; c < nch - 1; ++c, uu += du) {
-                    float *px = cyl + P + c * kWave + lane;
-                    *px = *px * __builtin_amdgcn_sqrtf(fmaf(uu, uu, 1.0f));
+                float *px = cyl + P + lane;
+                // full chunks four at a time: the four LDS reads are in flight together
+                int c = 0;
+                for (; c + 4 <= nch - 1; c += 4) {
+                    float x[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) x[u] = px[(c + u) * kWave];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const float uu = fmaf(du, (float)(c + u), u0);
+                        px[(c + u) * kWave] = x[u] * __builtin_amdgcn_sqrtf(fmaf(uu, uu, 1.0f));
+                    }
+                }
+                for (; c < nch - 1; ++c) {
+                    const float uu = fmaf(du, (float)c, u0);
+                    px[c * kWave] = px[c * kWave] * __builtin_amdgcn_sqrtf(fmaf(uu, uu, 1.0f));
                 }
                 {  // last chunk: slices past the span read as zero
                     const int kk = (nch - 1) * kWave + lane;
+                    const float uu = fmaf(du, (float)(nch - 1), u0);
                     const float x = cyl[P + kk] * __builtin_amdgcn_sqrtf(fmaf(uu, uu, 1.0f));
                     cyl[P + kk] = kk < span ? x : 0.0f;
                 }
@@ -1103,21 +1117,99 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
     for (int r = 0; r < RPL; ++r) mrow[r] = (float)(wm0 + lane + kWave * r);
     const int wgrp = (wm0 >> 7) & 31;  // this wave's 128-row group in the record masks
 
+    // vertical resampling of candidate record i (slot s of the batch at LDS base
+    // sb) onto this lane's rows
+    auto vert = [&](const float4 &R0, const float4 &R1, const float4 &R2, const float4 &R3, int i, int slot, int sb,
+                    float (&v)[RPL]) {
+        if (GEOM == MBX_GEOM_CONE) {
+            const float W = R0.w, Wlo = R1.x, acr = R1.y, invW = R1.z, nai = R1.w, aif = R2.x;
+            const float clo = R2.y, chi = R2.z, q1 = R3.y;
+            // base at the lowest tap (LDS offsets are unsigned)
+            int ib = sb - (__float_as_int(R2.w) & 0xffff);
+            asm("" : "+s"(ib));  // one scalar base: the address is one shift-add per row
+            if (VM == kVertTwo) {
+#pragma unroll
+                for (int r = 0; r < RPL; ++r) {
+                    const float mp = mrow[r] - aif;  // m - ai, exact
+                    const float jf = floorf(__builtin_amdgcn_fmed3f(fmaf(mp, invW, nai), clo, chi));
+                    // t_k0 - m - (1-W)/2: the first fma rounds once, at |t - m| < W
+                    const float dm = fmaf(W, jf, -mp) + fmaf(Wlo, jf, acr);
+                    const float f = __builtin_amdgcn_fmed3f(-dm, 0.0f, 1.0f);
+                    const float *cp = lds + (ib + (int)jf);
+                    const float c0 = cp[0], c1 = cp[1];
+                    v[r] = fmaf(f, c1 - c0, c0);
+                }
+            } else if (VM == kVertUnit) {
+#pragma unroll
+                for (int r = 0; r < RPL; ++r) {
+                    const float mp = mrow[r] - aif;  // m - ai, exact
+                    const float kr = rintf(__builtin_amdgcn_fmed3f(fmaf(mp, invW, nai), clo, chi));
+                    // t - m for slice kref + kr: the first fma rounds once, at |t - m| < 1
+                    const float dm = fmaf(W, kr, -mp) + fmaf(Wlo, kr, acr);
+                    const float *cp = lds + (ib + (int)kr);  // slice kr - 1
+                    const float cm = cp[0], c0 = cp[1], cq = cp[2];
+                    // |dm| <= W/2, so |t -+ W - m| = W -+ dm
+                    const float wm = __builtin_amdgcn_fmed3f(q1 + dm, 0.0f, 1.0f);
+                    const float wq = __builtin_amdgcn_fmed3f(q1 - dm, 0.0f, 1.0f);
+                    v[r] = fmaf(wq, cq - c0, fmaf(wm, cm - c0, c0));
+                }
+            } else {
+                const float *rec = Tv + (size_t)lv[i] * kRec;
+                const float hv = rec[9], Lv = rec[10], q2 = q1 - W;  // q2: taps two slices off the nearest
+#pragma unroll
+                for (int r = 0; r < RPL; ++r) {
+                    const float mp = mrow[r] - aif;  // m - ai, exact
+                    const float kr = rintf(__builtin_amdgcn_fmed3f(fmaf(mp, invW, nai), clo, chi));
+                    const float dm = fmaf(W, kr, -mp) + fmaf(Wlo, kr, acr);
+                    const float *cp = lds + (ib + (int)kr) + RB;  // slice kr
+                    // |dm| <= W/2, so |dm -+ jW| = jW -+ dm
+                    float x = __builtin_amdgcn_fmed3f(hv - fabsf(dm), 0.0f, Lv) * cp[0];
+                    x = fmaf(__builtin_amdgcn_fmed3f(q1 + dm, 0.0f, Lv), cp[-1], x);
+                    x = fmaf(__builtin_amdgcn_fmed3f(q1 - dm, 0.0f, Lv), cp[1], x);
+                    if (RB == 2) {
+                        x = fmaf(__builtin_amdgcn_fmed3f(q2 + dm, 0.0f, Lv), cp[-2], x);
+                        x = fmaf(__builtin_amdgcn_fmed3f(q2 - dm, 0.0f, Lv), cp[2], x);
+                    }
+                    v[r] = x;
+                }
+            }
+        } else {
+            // row m sees slice m (rows past the block read don't-cares)
+            const int base = sb - (g.kref + P - RB) + P + wm0 - m0 + lane;  // slot's slice m0 at index P
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) v[r] = lds[base + kWave * r];
+        }
+    };
+    // bucket D is centre channel n0 + D - RAD: tap t lands on channel D - 2*RAD + t
+    auto add_taps = [&](int D, const float4 &R0, int i, const float (&v)[RPL]) {
+        float Aall[5] = {0.0f, R0.x, R0.y, R0.z, 0.0f};  // Ac(n_c - 2 .. n_c + 2)
+        if (RAD == 2) {
+            const float *rec = Tv + (size_t)lv[i] * kRec;
+            Aall[0] = rec[1];
+            Aall[4] = rec[5];
+        }
+#pragma unroll
+        for (int r = 0; r < RPL; ++r)
+#pragma unroll
+            for (int t = 0; t <= 2 * RAD; ++t) {
+                const int c = D - 2 * RAD + t;
+                if (c >= 0 && c < CB) acc[r][c] = fmaf(Aall[2 - RAD + t], v[r], acc[r][c]);
+            }
+    };
+
     Meta mA{0, 0, 0.0f, 0.0f}, mB{0, 0, 0.0f, 0.0f};
     if (nbat > 0) {
         mA = load_meta(0);
         issue(mA, 0);
-        if (nbat > 1) mB = load_meta(1);
+        mB = load_meta(min(1, nbat - 1));
     }
     for (int bi = 0; bi < nbat; ++bi) {
         __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of batch bi (and the next batch's data) have landed
         finish(mA, bi);
         __syncthreads();  // batch bi staged by every wave; batch bi-1 no longer read
-        if (bi + 1 < nbat) {
-            issue(mB, bi + 1);
-            mA = mB;
-            if (bi + 2 < nbat) mB = load_meta(bi + 2);
-        }
+        issue(mB, bi + 1);  // no slots past the last batch
+        mA = mB;
+        mB = load_meta(min(bi + 2, nbat - 1));
         const int b0 = lo + bi * nbatch;
         const int nb = min(nbatch, hi - b0);
         const int boff = (bi & 1) * nbatch * cstride;  // LDS float offset of this batch
@@ -1131,83 +1223,13 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
                 int sb = boff + slo * cstride + g.kref + P - (VM == kVertTwo ? 0 : RB);
                 for (int slot = slo; slot < shi; ++slot, rp += 4, sb += cstride) {
                     const float4 R0 = rp[0], R1 = rp[1], R2 = rp[2], R3 = rp[3];
-                    // the whole record arrives in one round trip, before the test below
                     asm volatile("" ::"s"(R0.x), "s"(R0.y), "s"(R0.z), "s"(R0.w), "s"(R1.x), "s"(R1.y), "s"(R1.z),
                                  "s"(R1.w), "s"(R2.x), "s"(R2.y), "s"(R2.z), "s"(R2.w), "s"(R3.x), "s"(R3.y));
-                    float v[RPL];
-                    if (GEOM == MBX_GEOM_CONE) {
-                        if (!((__float_as_uint(R3.x) >> wgrp) & 1u)) continue;  // no row of this wave is reached
-                        const float W = R0.w, Wlo = R1.x, acr = R1.y, invW = R1.z, nai = R1.w, aif = R2.x;
-                        const float clo = R2.y, chi = R2.z, q1 = R3.y;
-                        // base at the lowest tap (LDS offsets are unsigned)
-                        int ib = sb - (__float_as_int(R2.w) & 0xffff);
-                        asm volatile("" : "+s"(ib));  // one scalar base: the address is one shift-add per row
-                        if (VM == kVertTwo) {
-#pragma unroll
-                            for (int r = 0; r < RPL; ++r) {
-                                const float mp = mrow[r] - aif;  // m - ai, exact
-                                const float jf = floorf(__builtin_amdgcn_fmed3f(fmaf(mp, invW, nai), clo, chi));
-                                // t_k0 - m - (1-W)/2: the first fma rounds once, at |t - m| < W
-                                const float dm = fmaf(W, jf, -mp) + fmaf(Wlo, jf, acr);
-                                const float f = __builtin_amdgcn_fmed3f(-dm, 0.0f, 1.0f);
-                                const float *cp = lds + (ib + (int)jf);
-                                const float c0 = cp[0], c1 = cp[1];
-                                v[r] = fmaf(f, c1 - c0, c0);
-                            }
-                        } else if (VM == kVertUnit) {
-#pragma unroll
-                            for (int r = 0; r < RPL; ++r) {
-                                const float mp = mrow[r] - aif;  // m - ai, exact
-                                const float kr = rintf(__builtin_amdgcn_fmed3f(fmaf(mp, invW, nai), clo, chi));
-                                // t - m for slice kref + kr: the first fma rounds once, at |t - m| < 1
-                                const float dm = fmaf(W, kr, -mp) + fmaf(Wlo, kr, acr);
-                                const float *cp = lds + (ib + (int)kr);  // slice kr - 1
-                                const float cm = cp[0], c0 = cp[1], cq = cp[2];
-                                // |dm| <= W/2, so |t -+ W - m| = W -+ dm
-                                const float wm = __builtin_amdgcn_fmed3f(q1 + dm, 0.0f, 1.0f);
-                                const float wq = __builtin_amdgcn_fmed3f(q1 - dm, 0.0f, 1.0f);
-                                v[r] = fmaf(wq, cq - c0, fmaf(wm, cm - c0, c0));
-                            }
-                        } else {
-                            const float *rec = Tv + (size_t)lv[b0 + slot] * kRec;
-                            const float hv = rec[9], Lv = rec[10], q2 = q1 - W;  // q2: taps two slices off the nearest
-#pragma unroll
-                            for (int r = 0; r < RPL; ++r) {
-                                const float mp = mrow[r] - aif;  // m - ai, exact
-                                const float kr = rintf(__builtin_amdgcn_fmed3f(fmaf(mp, invW, nai), clo, chi));
-                                const float dm = fmaf(W, kr, -mp) + fmaf(Wlo, kr, acr);
-                                const float *cp = lds + (ib + (int)kr) + RB;  // slice kr
-                                // |dm| <= W/2, so |dm -+ jW| = jW -+ dm
-                                float x = __builtin_amdgcn_fmed3f(hv - fabsf(dm), 0.0f, Lv) * cp[0];
-                                x = fmaf(__builtin_amdgcn_fmed3f(q1 + dm, 0.0f, Lv), cp[-1], x);
-                                x = fmaf(__builtin_amdgcn_fmed3f(q1 - dm, 0.0f, Lv), cp[1], x);
-                                if (RB == 2) {
-                                    x = fmaf(__builtin_amdgcn_fmed3f(q2 + dm, 0.0f, Lv), cp[-2], x);
-                                    x = fmaf(__builtin_amdgcn_fmed3f(q2 - dm, 0.0f, Lv), cp[2], x);
-                                }
-                                v[r] = x;
-                            }
-                        }
-                    } else {
-                        // row m sees slice m (rows past the block read don't-cares)
-#pragma unroll
-                        for (int r = 0; r < RPL; ++r)
-                            v[r] = lds[boff + slot * cstride + P + wm0 + lane + kWave * r - m0];
+                    if (GEOM != MBX_GEOM_CONE || ((__float_as_uint(R3.x) >> wgrp) & 1u)) {
+                        float v[RPL];
+                        vert(R0, R1, R2, R3, b0 + slot, slot, sb, v);
+                        add_taps(D, R0, b0 + slot, v);
                     }
-                    // bucket D is centre channel n0 + D - RAD: tap t lands on channel D - 2*RAD + t
-                    float Aall[5] = {0.0f, R0.x, R0.y, R0.z, 0.0f};  // Ac(n_c - 2 .. n_c + 2)
-                    if (RAD == 2) {
-                        const float *rec = Tv + (size_t)lv[b0 + slot] * kRec;
-                        Aall[0] = rec[1];
-                        Aall[4] = rec[5];
-                    }
-#pragma unroll
-                    for (int r = 0; r < RPL; ++r)
-#pragma unroll
-                        for (int t = 0; t <= 2 * RAD; ++t) {
-                            const int c = D - 2 * RAD + t;
-                            if (c >= 0 && c < CB) acc[r][c] = fmaf(Aall[2 - RAD + t], v[r], acc[r][c]);
-                        }
                 }
             }
         }
@@ -1245,12 +1267,11 @@ __global__ __launch_bounds__(256) void fwd_kernel(DevGeom g, const float *__rest
 
 template <int GEOM, int RAD, int RB, int VM, typename OT, int LAYOUT>
 static void launch_fwd_t(const DevGeom &g, const FwdPlan &pl, const float *T, const float *FR, const int *list,
-                         const int *cstart,
-                         const float *vox, int npix, OT *sino, int accumulate, const float *w, double *part,
-                         hipStream_t st) {
+                         const int *cstart, const float *vox, int npix, OT *sino, int accumulate, const float *w,
+                         double *part, hipStream_t st) {
     hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, VM, OT, LAYOUT>), dim3((unsigned)pl.nblocks), dim3(64 * pl.nw),
-                       pl.lds, st, g, T, FR, list, cstart, vox, npix, pl.ncb, pl.nrb, pl.nb, pl.cstride, sino, accumulate, w,
-                       part);
+                       pl.lds, st, g, T, FR, list, cstart, vox, npix, pl.ncb, pl.nrb, pl.nb, pl.cstride, sino,
+                       accumulate, w, part);
 }
 
 template <int GEOM, int RAD, int RB, int VM>

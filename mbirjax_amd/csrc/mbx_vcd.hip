@@ -165,24 +165,29 @@ __global__ __launch_bounds__(256) void dot_kernel(const TA *__restrict__ a, cons
     if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
-// e = e - float(alpha) * d, four elements per access (n4 groups), then the tail
+// e = e - float(alpha) * d: thread t updates elements 4t .. 4t+3 with streaming
+// (nontemporal) accesses, the grid covering n/4 threads; the remaining (or, for
+// unaligned arrays, all) elements go element-wise over the grid.
 template <typename TE, typename TD>
 __device__ __forceinline__ void axpy_body(TE *__restrict__ e, const TD *__restrict__ d, float a, int64_t n) {
     // vector path only when both arrays are aligned for the 4-element access
     const bool vec = ((uintptr_t)e % (4 * sizeof(TE)) == 0) && ((uintptr_t)d % (4 * sizeof(TD)) == 0);
     const int64_t n4 = vec ? (n >> 2) : 0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n4) {
         float ev[4], dv[4];
-        load4(e + 4 * i, ev);
-        load4(d + 4 * i, dv);
+        load4_nt(e + 4 * t, ev);
+        load4_nt(d + 4 * t, dv);
 #pragma unroll
         for (int u = 0; u < 4; ++u) ev[u] = ev[u] - a * dv[u];
-        store4(e + 4 * i, ev);
+        store4_nt(e + 4 * t, ev);
     }
-    for (int64_t i = 4 * n4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        e[i] = from_float<TE>(to_float(e[i]) - a * to_float(d[i]));
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = 4 * n4 + t; i < n; i += stride) e[i] = from_float<TE>(to_float(e[i]) - a * to_float(d[i]));
 }
+
+// one thread per four elements
+static int64_t axpy_blocks(int64_t n) { return (n / 4 + 255) / 256 + 1; }
 
 template <typename TE, typename TD>
 __global__ __launch_bounds__(256) void axpy_kernel(TE *__restrict__ e, const TD *__restrict__ d,
@@ -191,7 +196,7 @@ __global__ __launch_bounds__(256) void axpy_kernel(TE *__restrict__ e, const TD 
 }
 
 static int launch_axpy(void *e, int edt, const void *d, int ddt, const double *alpha, int64_t n, hipStream_t st) {
-    const int64_t blocks = std::min<int64_t>((n / 4 + 255) / 256 + 1, 8192);
+    const int64_t blocks = axpy_blocks(n);
     if (n == 0) return MBX_OK;
     if (edt == MBX_DTYPE_F32 && ddt == MBX_DTYPE_F32)
         hipLaunchKernelGGL((axpy_kernel<float, float>), dim3((unsigned)blocks), dim3(256), 0, st, (float *)e,
@@ -528,7 +533,7 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
         if (rc != MBX_OK) return rc;
     }
     // (l) error sinogram update
-    const int64_t blocks = std::min<int64_t>((nsino / 4 + 255) / 256 + 1, 8192);
+    const int64_t blocks = axpy_blocks(nsino);
     if (e_dtype == MBX_DTYPE_F32 && delta_dtype == MBX_DTYPE_F32)
         hipLaunchKernelGGL((axpy_scal_kernel<float, float>), dim3((unsigned)blocks), dim3(256), 0, st, (float *)e,
                            (const float *)dsino, scal, nsino);
