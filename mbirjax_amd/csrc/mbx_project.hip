@@ -825,15 +825,19 @@ __device__ __forceinline__ T block_sum(T v, T *scratch) {
 // Counting sort with LDS counters, then every bucket is put in pixel order,
 // so the result (and the forward summation order) does not depend on atomic
 // ordering.
-__global__ __launch_bounds__(256) void fwd_bin_kernel(DevGeom g, const float *__restrict__ T,
-                                                      const int *__restrict__ NC, int npix, int nbk, int vmode,
-                                                      int *__restrict__ list, int *__restrict__ cstart,
-                                                      float *__restrict__ FR) {
-    extern __shared__ int cnt[];  // [nbk]
+constexpr int kBinThreads = 1024;
+constexpr size_t kBinLds = 64 * 1024;  // counters and, when they fit, the view's list
+// sort_in_lds: the list is sorted in LDS (after the counters) and then copied out.
+__global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const float *__restrict__ T,
+                                                              const int *__restrict__ NC, int npix, int nbk,
+                                                              int vmode, int sort_in_lds, int *__restrict__ list,
+                                                              int *__restrict__ cstart, float *__restrict__ FR) {
+    extern __shared__ int cnt[];  // [nbk] (+ [npix])
     const int rad = g.rh;
     const int v = blockIdx.x, tid = threadIdx.x;
     const int *ncv = NC + (size_t)v * npix;
-    int *lv = list + (size_t)v * npix;
+    int *lg = list + (size_t)v * npix;
+    int *lv = sort_in_lds ? cnt + nbk : lg;
     int *cs = cstart + (size_t)v * (nbk + 1);
     for (int b = tid; b < nbk; b += blockDim.x) cnt[b] = 0;
     __syncthreads();
@@ -847,7 +851,7 @@ __global__ __launch_bounds__(256) void fwd_bin_kernel(DevGeom g, const float *__
     const int b0 = tid * per, b1 = min(nbk, b0 + per);
     int run = 0;
     for (int b = b0; b < b1; ++b) run += cnt[b];
-    __shared__ int wsum[4];
+    __shared__ int wsum[kBinThreads / 64];
     __shared__ int total;
     int incl = run;
     for (int o = 1; o < 64; o <<= 1) {
@@ -876,7 +880,7 @@ __global__ __launch_bounds__(256) void fwd_bin_kernel(DevGeom g, const float *__
     __syncthreads();
     __threadfence_block();
     for (int b = tid; b < nbk; b += blockDim.x) {  // insertion sort of each bucket
-        const int lo = cs[b], hi = cs[b + 1];
+        const int lo = b == 0 ? 0 : cnt[b - 1], hi = cnt[b];  // the cursors now hold the bucket ends
         for (int i = lo + 1; i < hi; ++i) {
             const int x = lv[i];
             int j = i - 1;
@@ -893,6 +897,7 @@ __global__ __launch_bounds__(256) void fwd_bin_kernel(DevGeom g, const float *__
     const int n = total;
     for (int i = tid; i < n; i += blockDim.x) {
         const int s = lv[i];
+        if (sort_in_lds) lg[i] = s;
         const float4 *r4 = (const float4 *)(T + ((size_t)v * npix + s) * kRec);
         const float4 R0 = r4[0], R1 = r4[1], R2 = r4[2], R3 = r4[3];
         float o[kFRec];
@@ -1314,8 +1319,10 @@ int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float 
     float *FR = (float *)((char *)cstart + al256((size_t)nviews * (nbk + 1) * sizeof(int)));
     const int vm = fwd_vert_mode(g);
     if (nviews > 0) {
-        hipLaunchKernelGGL(fwd_bin_kernel, dim3(nviews), dim3(256), nbk * sizeof(int), st, g, T, NC, npix, nbk, vm,
-                           list, cstart, FR);
+        const bool in_lds = (size_t)(nbk + npix) * sizeof(int) <= kBinLds;
+        const size_t lds = (size_t)(nbk + (in_lds ? npix : 0)) * sizeof(int);
+        hipLaunchKernelGGL(fwd_bin_kernel, dim3(nviews), dim3(kBinThreads), lds, st, g, T, NC, npix, nbk, vm,
+                           (int)in_lds, list, cstart, FR);
         MBX_LAUNCH_CHECK();
     }
     double *p = sumsq ? part : nullptr;
