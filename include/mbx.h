@@ -61,6 +61,7 @@ extern "C" {
 
 #define MBX_GEOM_PARALLEL 0
 #define MBX_GEOM_CONE 1
+#define MBX_GEOM_TRANSLATION 2 /* mbirjax/translation_model.py: no rotation, a translation per view */
 
 #define MBX_DTYPE_F32 0
 #define MBX_DTYPE_BF16 1
@@ -70,9 +71,15 @@ extern "C" {
 
 /* Geometry of one TomographyModel, mirroring the reference's projector_params
  * = (sinogram_shape, recon_shape, geometry_params): ConeBeamModel.get_geometry_parameters
- * (mbirjax/cone_beam.py:110-136) and ParallelBeamModel.get_geometry_parameters
- * (mbirjax/parallel_beam.py:94-113).  Doubles are rounded to float32 on the
- * device, as the reference's jitted float32 arithmetic does. */
+ * (mbirjax/cone_beam.py:110-136), ParallelBeamModel.get_geometry_parameters
+ * (mbirjax/parallel_beam.py:94-113) and TranslationModel.get_geometry_parameters
+ * (mbirjax/translation_model.py:83-107).  Doubles are rounded to float32 on the
+ * device, as the reference's jitted float32 arithmetic does.
+ *
+ * View parameters ("angles" in the entry points below): num_views float32
+ * angles for MBX_GEOM_PARALLEL and MBX_GEOM_CONE; num_views x 3 float32
+ * translation vectors (x, y, z) for MBX_GEOM_TRANSLATION, whose forward
+ * vertical fan uses psf_radius (bp_psf_radius is ignored). */
 typedef struct mbx_geom {
     int32_t kind;             /* MBX_GEOM_PARALLEL or MBX_GEOM_CONE */
     int32_t num_views;        /* length of the angles array */
@@ -92,6 +99,7 @@ typedef struct mbx_geom {
     double magnification;        /* source_detector_dist / source_iso_dist; 1 for parallel */
     double delta_voxel;
     double recon_slice_offset;
+    double delta_recon_row;      /* MBX_GEOM_TRANSLATION: pitch of the recon rows (y); unused otherwise */
 } mbx_geom;
 
 /* qGGMRF prior parameters: b = get_b_from_nbr_wts(qggmrf_nbr_wts) in the order

@@ -20,6 +20,7 @@ MBX_EWORKSPACE = -4
 
 GEOM_PARALLEL = 0
 GEOM_CONE = 1
+GEOM_TRANSLATION = 2
 DTYPE_F32 = 0
 DTYPE_BF16 = 1
 LAYOUT_VCR = 0
@@ -54,6 +55,7 @@ class MbxGeom(ctypes.Structure):
         ("magnification", ctypes.c_double),
         ("delta_voxel", ctypes.c_double),
         ("recon_slice_offset", ctypes.c_double),
+        ("delta_recon_row", ctypes.c_double),
     ]
 
 
@@ -132,12 +134,13 @@ def stream():
 
 def np_geom_struct(kind, sinogram_shape, recon_shape, psf_radius, bp_psf_radius, delta_det_row, delta_det_channel,
                    det_row_offset, det_channel_offset, source_detector_dist, magnification, delta_voxel,
-                   recon_slice_offset):
+                   recon_slice_offset, delta_recon_row=0.0):
     V, R, C = (int(v) for v in sinogram_shape)
     rows, cols, nz = (int(v) for v in recon_shape)
     return MbxGeom(int(kind), V, R, C, rows, cols, nz, int(psf_radius), int(bp_psf_radius), 0,
                    float(delta_det_row), float(delta_det_channel), float(det_row_offset), float(det_channel_offset),
-                   float(source_detector_dist), float(magnification), float(delta_voxel), float(recon_slice_offset))
+                   float(source_detector_dist), float(magnification), float(delta_voxel), float(recon_slice_offset),
+                   float(delta_recon_row))
 
 
 def qggmrf_struct(b, sigma_x, p, q, T):

@@ -7,7 +7,7 @@ from typing import Any
 
 FILE_FORMAT_NUMBER = 1.0
 
-_geometry_types_for_tests = ['parallel', 'cone']
+_geometry_types_for_tests = ['parallel', 'cone', 'translation']
 
 
 @dataclass

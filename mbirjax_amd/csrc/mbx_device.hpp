@@ -156,6 +156,10 @@ struct DevGeom {
     mbx_geom h; // the caller's double-precision geometry (table kernel)
 };
 
+// Cone and translation geometries share the fan projectors (the vertical and
+// horizontal fans); only the per-(pixel, view) table differs.
+__host__ __device__ inline bool is_fan(const DevGeom &g) { return g.kind != MBX_GEOM_PARALLEL; }
+
 // The view-dependent part of one (pixel, view) pair that both projectors
 // store or recompute: the pixel magnification, the fractional channel n_p,
 // cos(alpha_xy) and cos(theta_p).

@@ -41,8 +41,9 @@ int fail(int code, const std::string &msg) {
 
 int make_dev_geom(const mbx_geom *g, DevGeom *d) {
     if (!g) return fail(MBX_EINVAL, "geometry pointer is NULL");
-    if (g->kind != MBX_GEOM_PARALLEL && g->kind != MBX_GEOM_CONE)
-        return fail(MBX_EINVAL, "geometry kind must be MBX_GEOM_PARALLEL or MBX_GEOM_CONE");
+    if (g->kind != MBX_GEOM_PARALLEL && g->kind != MBX_GEOM_CONE && g->kind != MBX_GEOM_TRANSLATION)
+        return fail(MBX_EINVAL, "geometry kind must be MBX_GEOM_PARALLEL, MBX_GEOM_CONE or MBX_GEOM_TRANSLATION");
+    const bool fan = g->kind != MBX_GEOM_PARALLEL;
     if (g->num_views <= 0 || g->num_det_rows <= 0 || g->num_det_channels <= 0 || g->num_recon_rows <= 0 ||
         g->num_recon_cols <= 0 || g->num_recon_slices <= 0)
         return fail(MBX_EINVAL, "all sinogram and recon dimensions must be positive");
@@ -58,15 +59,21 @@ int make_dev_geom(const mbx_geom *g, DevGeom *d) {
         return fail(MBX_EINVAL, "delta_det_channel and delta_voxel must be positive and finite");
     if (g->psf_radius < 1 || g->psf_radius > 2)
         return fail(MBX_EUNSUPPORTED, "psf_radius must be 1 or 2 (got " + std::to_string(g->psf_radius) + ")");
-    if (g->kind == MBX_GEOM_CONE) {
+    if (fan) {
         if (!(g->delta_det_row > 0) || !std::isfinite(g->delta_det_row))
             return fail(MBX_EINVAL, "delta_det_row must be positive and finite");
         if (!(g->magnification > 0) || !std::isfinite(g->magnification))
             return fail(MBX_EINVAL, "magnification must be positive and finite");
         if (!(g->source_detector_dist > 0)) return fail(MBX_EINVAL, "source_detector_dist must be positive");
-        if (g->bp_psf_radius < 1 || g->bp_psf_radius > 2)
+        if (g->kind == MBX_GEOM_CONE && (g->bp_psf_radius < 1 || g->bp_psf_radius > 2))
             return fail(MBX_EUNSUPPORTED,
                         "bp_psf_radius must be 1 or 2 (got " + std::to_string(g->bp_psf_radius) + ")");
+        // the translation geometry has every pixel at its own distance y from
+        // the source, all views sharing it (mbirjax/translation_model.py:69-81)
+        if (g->kind == MBX_GEOM_TRANSLATION &&
+            (!std::isfinite(g->source_detector_dist) || !(g->delta_recon_row > 0) || !std::isfinite(g->delta_recon_row)))
+            return fail(MBX_EINVAL, "translation geometry needs a finite source_detector_dist and a positive "
+                                    "delta_recon_row");
     } else {
         if (g->num_recon_slices != g->num_det_rows)
             return fail(MBX_EINVAL, "parallel beam needs num_recon_slices == num_det_rows");
@@ -80,13 +87,15 @@ int make_dev_geom(const mbx_geom *g, DevGeom *d) {
     o.cols = g->num_recon_cols;
     o.Nz = g->num_recon_slices;
     o.rh = g->psf_radius;
-    o.rb = g->kind == MBX_GEOM_CONE ? g->bp_psf_radius : 1;
-    o.dr = (float)(g->kind == MBX_GEOM_CONE ? g->delta_det_row : 1.0);
+    // forward vertical taps: bp_psf_radius (cone), psf_radius (translation,
+    // mbirjax/translation_model.py:364)
+    o.rb = g->kind == MBX_GEOM_CONE ? g->bp_psf_radius : g->kind == MBX_GEOM_TRANSLATION ? g->psf_radius : 1;
+    o.dr = (float)(fan ? g->delta_det_row : 1.0);
     o.dc = (float)g->delta_det_channel;
     o.row_off = (float)g->det_row_offset;
     o.ch_off = (float)g->det_channel_offset;
-    o.inv_sdd = g->kind == MBX_GEOM_CONE ? (float)(1.0 / g->source_detector_dist) : 0.0f;
-    o.inv_M = g->kind == MBX_GEOM_CONE ? (float)(1.0 / g->magnification) : 1.0f;
+    o.inv_sdd = fan ? (float)(1.0 / g->source_detector_dist) : 0.0f;
+    o.inv_M = fan ? (float)(1.0 / g->magnification) : 1.0f;
     o.dv = (float)g->delta_voxel;
     o.slice_off = (float)g->recon_slice_offset;
     o.row_center = (float)((g->num_det_rows - 1) / 2.0);
@@ -95,7 +104,7 @@ int make_dev_geom(const mbx_geom *g, DevGeom *d) {
     o.img_col_center = (float)((g->num_recon_cols - 1) / 2.0);
     o.slice_center = (float)((g->num_recon_slices - 1) / 2.0);
     o.dv_over_dc = (float)(g->delta_voxel / g->delta_det_channel);
-    o.dv_over_dr = g->kind == MBX_GEOM_CONE ? (float)(g->delta_voxel / g->delta_det_row) : 1.0f;
+    o.dv_over_dr = fan ? (float)(g->delta_voxel / g->delta_det_row) : 1.0f;
     o.kref = g->num_recon_slices / 2;
     o.h = *g;
     return MBX_OK;
@@ -148,32 +157,52 @@ __global__ __launch_bounds__(256) void pv_table_kernel(DevGeom g, const float *_
     if (s >= npix) return;
     const mbx_geom &h = g.h;
     const int va = views ? views[vi] : vi;
-    double sn, cs;
-    sincos((double)angles[va], &sn, &cs);
-    // recon_ijk_to_xyz and geometry_xyz_to_uv_mag (mbirjax/cone_beam.py:646-681),
-    // compute_horizontal_data (593-644); parallel: parallel_beam.py:223-284
     const int p = pix[s];
     const int i = p / g.cols, j = p - (p / g.cols) * g.cols;
     const double dv = h.delta_voxel, dc = h.delta_det_channel;
-    const double yt = dv * ((double)i - 0.5 * (h.num_recon_rows - 1));
-    const double xt = dv * ((double)j - 0.5 * (h.num_recon_cols - 1));
-    const double x = cs * xt - sn * yt;
-    double mag = 1.0, np, cosa, ctp = 1.0;
-    if (GEOM == MBX_GEOM_CONE) {
-        const double y = sn * xt + cs * yt;
+    // Wc: voxel width on the detector in channels; sc: channel weight per unit
+    // overlap; zoff: z offset of the slices
+    double mag = 1.0, np, Wc, sc, zoff = h.recon_slice_offset;
+    if (GEOM == MBX_GEOM_TRANSLATION) {
+        // recon_ijk_to_xyz, geometry_xyz_to_uv_mag, compute_horizontal_data
+        // (mbirjax/translation_model.py:595-672): no rotation, the object
+        // shifted by -t; channel weight L / cos(theta_p)
+        const float *tv = angles + 3 * (size_t)va;
+        const double x = dv * ((double)j - 0.5 * (h.num_recon_cols - 1)) - (double)tv[0];
+        const double y = h.delta_recon_row * ((double)i - 0.5 * (h.num_recon_rows - 1)) - (double)tv[1];
+        zoff = -(double)tv[2];
         mag = 1.0 / (1.0 / h.magnification - y / h.source_detector_dist);
         const double u = mag * x;
         np = (u + h.det_channel_offset) / dc + 0.5 * (h.num_det_channels - 1);
         const double us = u / h.source_detector_dist;
-        ctp = 1.0 / sqrt(1.0 + us * us);  // cos(theta_p)
-        const double stp = us * ctp;      // sin(theta_p)
-        cosa = fmax(fabs(cs * ctp + sn * stp), fabs(sn * ctp - cs * stp));
+        Wc = mag * (dv / dc);
+        sc = sqrt(1.0 + us * us);  // 1 / cos(theta_p)
     } else {
-        np = (x + h.det_channel_offset) / dc + 0.5 * (h.num_det_channels - 1);
-        cosa = fmax(fabs(cs), fabs(sn));
+        // recon_ijk_to_xyz and geometry_xyz_to_uv_mag (mbirjax/cone_beam.py:646-681),
+        // compute_horizontal_data (593-644); parallel: parallel_beam.py:223-284
+        double sn, cs;
+        sincos((double)angles[va], &sn, &cs);
+        const double yt = dv * ((double)i - 0.5 * (h.num_recon_rows - 1));
+        const double xt = dv * ((double)j - 0.5 * (h.num_recon_cols - 1));
+        const double x = cs * xt - sn * yt;
+        double cosa, ctp = 1.0;
+        if (GEOM == MBX_GEOM_CONE) {
+            const double y = sn * xt + cs * yt;
+            mag = 1.0 / (1.0 / h.magnification - y / h.source_detector_dist);
+            const double u = mag * x;
+            np = (u + h.det_channel_offset) / dc + 0.5 * (h.num_det_channels - 1);
+            const double us = u / h.source_detector_dist;
+            ctp = 1.0 / sqrt(1.0 + us * us);  // cos(theta_p)
+            const double stp = us * ctp;      // sin(theta_p)
+            cosa = fmax(fabs(cs * ctp + sn * stp), fabs(sn * ctp - cs * stp));
+        } else {
+            np = (x + h.det_channel_offset) / dc + 0.5 * (h.num_det_channels - 1);
+            cosa = fmax(fabs(cs), fabs(sn));
+        }
+        Wc = (GEOM == MBX_GEOM_CONE ? mag * (cosa / ctp) : cosa) * (dv / dc);
+        sc = dv / cosa;
     }
-    const double Wc = (GEOM == MBX_GEOM_CONE ? mag * (cosa / ctp) : cosa) * (dv / dc);
-    const double hc = 0.5 * (Wc + 1.0), Lc = fmin(1.0, Wc), sc = dv / cosa;
+    const double hc = 0.5 * (Wc + 1.0), Lc = fmin(1.0, Wc);
     const int nc = (int)rint(np);
     float r[kRec];
     r[0] = __int_as_float(nc);
@@ -184,12 +213,13 @@ __global__ __launch_bounds__(256) void pv_table_kernel(DevGeom g, const float *_
         r[3 + t] = (t >= -RAD && t <= RAD && n >= 0 && n < g.C) ? (float)(sc * L) : 0.0f;
     }
     int mlo = 0, mhi = g.R - 1, ai = g.kref;
-    if (GEOM == MBX_GEOM_CONE) {
-        // compute_vertical_data_single_pixel (mbirjax/cone_beam.py:543-591):
-        // t_k = (mag * z_k + det_row_offset) / dr + (R-1)/2, linear in k
+    if (GEOM != MBX_GEOM_PARALLEL) {
+        // compute_vertical_data_single_pixel (mbirjax/cone_beam.py:543-591,
+        // translation_model.py:544-593): t_k = (mag * z_k + det_row_offset) / dr
+        // + (R-1)/2, linear in k
         const double dr = h.delta_det_row;
         const double W = mag * dv / dr;
-        const double zref = h.recon_slice_offset + dv * ((double)g.kref - 0.5 * (h.num_recon_slices - 1));
+        const double zref = zoff + dv * ((double)g.kref - 0.5 * (h.num_recon_slices - 1));
         const double tref = (mag * zref + h.det_row_offset) / dr + 0.5 * (h.num_det_rows - 1);
         ai = (int)rint(tref);
         const float Wf = (float)W;
@@ -230,7 +260,14 @@ int launch_table(const DevGeom &g, const float *angles, const int32_t *pix, int 
                  float *T, int *NC, hipStream_t st) {
     if (npix == 0 || nviews == 0) return MBX_OK;
     dim3 grid((npix + 255) / 256, nviews);
-    if (g.kind == MBX_GEOM_CONE) {
+    if (g.kind == MBX_GEOM_TRANSLATION) {
+        if (g.rh == 1)
+            hipLaunchKernelGGL((pv_table_kernel<MBX_GEOM_TRANSLATION, 1>), grid, dim3(256), 0, st, g, angles, pix, npix,
+                               views, T, NC);
+        else
+            hipLaunchKernelGGL((pv_table_kernel<MBX_GEOM_TRANSLATION, 2>), grid, dim3(256), 0, st, g, angles, pix, npix,
+                               views, T, NC);
+    } else if (g.kind == MBX_GEOM_CONE) {
         if (g.rh == 1)
             hipLaunchKernelGGL((pv_table_kernel<MBX_GEOM_CONE, 1>), grid, dim3(256), 0, st, g, angles, pix, npix, views,
                                T, NC);
@@ -276,7 +313,7 @@ BackPlan plan_back(const DevGeom &g, int npix, int nviews, int ppw) {
     // height + 2 plus a chunk (the clamped fallback layout).
     const int round = kWave;
     p.cap = g.R + 2 + round;
-    if (g.kind == MBX_GEOM_CONE) {
+    if (g.kind == MBX_GEOM_CONE) {  // (translation: the clamped layout covers every magnification)
         const double rmax = g.dv * std::sqrt((double)g.img_row_center * g.img_row_center +
                                              (double)g.img_col_center * g.img_col_center) + g.dv;
         const double den = (double)g.inv_M - rmax * (double)g.inv_sdd;
@@ -680,7 +717,7 @@ int launch_back_tab(const DevGeom &g, const float *T, int npix, const void *sino
     const BackPlan pl = plan_back(g, npix, nviews, kahan ? 1 : 2);
     if (pl.nsplit > 1 && !part) return fail(MBX_EWORKSPACE, "back projection needs a partials workspace");
     int rc;
-    if (g.kind == MBX_GEOM_CONE)
+    if (is_fan(g))
         rc = dispatch_back_1<MBX_GEOM_CONE>(g, pl, T, sino, sino_dtype, weights, npix, views, nviews, coeff_power,
                                             kahan, scale, accumulate, out, part, st);
     else
@@ -741,7 +778,7 @@ FwdPlan plan_fwd(const DevGeom &g, int nviews) {
     p.list = 0;
     // A staged cylinder holds the slices the block's rows reach (at most all
     // slices) in whole 64-slice chunks, and the zero pads.
-    const int span_max = g.kind == MBX_GEOM_CONE ? g.Nz : p.rows_pb;
+    const int span_max = is_fan(g) ? g.Nz : p.rows_pb;
     p.cstride = ((span_max + kWave - 1) / kWave) * kWave + 2 * kFwdPad + 6;
     p.nb = (int)std::max<size_t>(1, std::min<size_t>(kFwdMaxBatch, kFwdLds / (2 * (size_t)p.cstride * sizeof(float))));
     p.lds = 2 * (size_t)p.nb * p.cstride * sizeof(float) + 64 * sizeof(int);
@@ -907,7 +944,7 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
         int rlo = 0, rhi = g.R - 1, kf = 0, span = 0;
         constexpr int P = kFwdPad;
         float clo = 0.0f, chi = 0.0f;
-        if (g.kind == MBX_GEOM_CONE) {
+        if (is_fan(g)) {
             const float acr = R1.z, W = R1.w, Wlo = R2.x, hv = R2.y, invW = R2.w;
             const int ai = __float_as_int(R3.z);
             rlo = rec_mlo(R3.w);
@@ -1329,7 +1366,7 @@ int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float 
 #define MBX_FWD(G, RA, RB_, VM_)                                                                                      \
     dispatch_fwd_2<G, RA, RB_, VM_>(g, pl, T, FR, list, cstart, vox, npix, sino, sino_dtype, layout, accumulate,      \
                                     weights, p, st)
-    if (g.kind == MBX_GEOM_CONE) {
+    if (is_fan(g)) {
         if (vm == kVertTwo) {
             if (g.rh == 1)
                 MBX_FWD(MBX_GEOM_CONE, 1, 1, kVertTwo);

@@ -95,7 +95,7 @@ class TomographyModel(ParameterHandler):
             self.GEOM_KIND, sinogram_shape, recon_shape, g['psf_radius'], g.get('bp_psf_radius', 1),
             g.get('delta_det_row', 1.0), g['delta_det_channel'], g.get('det_row_offset', 0.0),
             g['det_channel_offset'], g.get('source_detector_dist', math.inf), g.get('magnification', 1.0),
-            g['delta_voxel'], g.get('recon_slice_offset', 0.0))
+            g['delta_voxel'], g.get('recon_slice_offset', 0.0), g.get('delta_recon_row', 0.0))
         self._angles_dev = None
 
     def set_params(self, no_warning=False, no_compile=False, **kwargs):

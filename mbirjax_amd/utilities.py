@@ -78,3 +78,16 @@ def _place(vol, device):
         return vol
     import torch
     return torch.as_tensor(vol, device=device)
+
+
+def gen_translation_vectors(num_x_translations, num_z_translations, x_spacing, z_spacing):
+    """(views, 3) translation vectors on a centred x-z grid, x fastest, y = 0
+    (mbirjax/utilities.py:1003-1031)."""
+    num_views = num_x_translations * num_z_translations
+    tv = np.zeros((num_views, 3))
+    x_center = (num_x_translations - 1) / 2
+    z_center = (num_z_translations - 1) / 2
+    for row in range(num_z_translations):
+        for col in range(num_x_translations):
+            tv[row * num_x_translations + col] = [(col - x_center) * x_spacing, 0.0, (row - z_center) * z_spacing]
+    return tv

@@ -20,6 +20,10 @@
  *   back vertical fan              mbirjax/cone_beam.py:480-591
  *   parallel compute_proj_data     mbirjax/parallel_beam.py:223-284
  *   parallel forward / back        mbirjax/parallel_beam.py:138-221
+ *   translation geometry           mbirjax/translation_model.py:244-284 (horizontal
+ *                                  fan), 286-377 (forward vertical), 412-541 (back),
+ *                                  544-706 (geometry helpers); the view parameter
+ *                                  is a translation vector (x, y, z) per view
  *   view_indices semantics         mbirjax/projectors.py:109-111, 155-157;
  *                                  mbirjax/tomography_model.py:661-663
  * Rounding: jnp.round is round-half-to-even, restated with rint() in the
@@ -35,24 +39,43 @@
 #endif
 
 typedef struct {
-    int32_t kind; /* 0 parallel, 1 cone */
+    int32_t kind; /* 0 parallel, 1 cone, 2 translation */
     int32_t num_views, num_det_rows, num_det_channels;
     int32_t num_recon_rows, num_recon_cols, num_recon_slices;
     int32_t psf_radius, bp_psf_radius, reserved;
     double delta_det_row, delta_det_channel, det_row_offset, det_channel_offset;
     double source_detector_dist, magnification, delta_voxel, recon_slice_offset;
+    double delta_recon_row; /* translation: recon row (y) pitch */
 } ogeom;
 
 typedef REAL real;
 
 static real clipr(real x, real lo, real hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
-/* Horizontal data for pixel p at angle th (cone_beam.py:593-644 / parallel_beam.py:223-263). */
-static void horizontal(const ogeom *g, int p, real th, real *n_p, int *n_c, real *W_c, real *cos_a, real *mag_out,
-                       real *y_out) {
+/* Horizontal data for pixel p at view parameters vp (angle, or translation
+ * vector for kind 2): cone_beam.py:593-644, parallel_beam.py:223-263,
+ * translation_model.py:595-656.  *cos_a returns the divisor of the channel
+ * weight: cos(alpha) (times 1/dv via chan_w) or, for kind 2, cos(theta_p). */
+static void horizontal(const ogeom *g, int p, const double *vp, real *n_p, int *n_c, real *W_c, real *cos_a,
+                       real *mag_out, real *y_out) {
     const int rows = g->num_recon_rows, cols = g->num_recon_cols;
     const int i = p / cols, j = p % cols;
     const real dv = (real)g->delta_voxel;
+    if (g->kind == 2) {
+        const real x = dv * ((real)j - (real)(cols - 1) / 2) - (real)vp[0];
+        const real y = (real)g->delta_recon_row * ((real)i - (real)(rows - 1) / 2) - (real)vp[1];
+        const real sdd = (real)g->source_detector_dist;
+        const real mag = 1 / (1 / (real)g->magnification - y / sdd);
+        const real u = mag * x;
+        *n_p = (u + (real)g->det_channel_offset) / (real)g->delta_det_channel + (real)(g->num_det_channels - 1) / 2;
+        *cos_a = cos(atan2(u, sdd));
+        *W_c = mag * (dv / (real)g->delta_det_channel);
+        *mag_out = mag;
+        *y_out = y;
+        *n_c = (int)rint(*n_p);
+        return;
+    }
+    const real th = (real)vp[0];
     const real yt = dv * ((real)i - (real)(rows - 1) / 2);
     const real xt = dv * ((real)j - (real)(cols - 1) / 2);
     const real c = cos(th), s = sin(th);
@@ -78,60 +101,74 @@ static void horizontal(const ogeom *g, int p, real th, real *n_p, int *n_c, real
     *n_c = (int)rint(*n_p);
 }
 
+/* Channel weight: dv * L / cos(alpha) (cone_beam.py:276-282), or L / cos(theta_p)
+ * for the translation geometry (translation_model.py:276-284). */
 static real chan_w(const ogeom *g, real n_p, int n, real W_c, real cos_a) {
     if (n < 0 || n >= g->num_det_channels) return 0;
     const real L = clipr((W_c + 1) / 2 - fabs(n_p - (real)n), 0, fmin((real)1, W_c));
-    return (real)g->delta_voxel * L / cos_a;
+    return (g->kind == 2 ? (real)1 : (real)g->delta_voxel) * L / cos_a;
 }
 
-/* z of slice k (cone_beam.py:665) */
-static real slice_z(const ogeom *g, int k) {
-    return (real)g->delta_voxel * ((real)k - (real)(g->num_recon_slices - 1) / 2) + (real)g->recon_slice_offset;
+/* z of slice k: cone_beam.py:665; translation_model.py:646-656 (z shifted by
+ * -t_z, no slice offset).  tz is the view's z translation (0 for the cone). */
+static real slice_z(const ogeom *g, int k, real tz) {
+    const real z0 = (real)g->delta_voxel * ((real)k - (real)(g->num_recon_slices - 1) / 2);
+    return g->kind == 2 ? z0 - tz : z0 + (real)g->recon_slice_offset;
 }
 
 /* detector row position of slice k of a pixel with magnification mag
  * (detector_uv_to_mn, cone_beam.py:684-707) */
-static real slice_row(const ogeom *g, real mag, int k) {
-    const real v = mag * slice_z(g, k);
+static real slice_row(const ogeom *g, real mag, int k, real tz) {
+    const real v = mag * slice_z(g, k, tz);
     return (v + (real)g->det_row_offset) / (real)g->delta_det_row + (real)(g->num_det_rows - 1) / 2;
 }
 
-static real inv_cos_phi(const ogeom *g, real mag, int k) {
-    const real v = mag * slice_z(g, k);
+static real inv_cos_phi(const ogeom *g, real mag, int k, real tz) {
+    const real v = mag * slice_z(g, k, tz);
     return 1 / cos(atan2(v, (real)g->source_detector_dist));
 }
 
-/* sino (nviews, R, C) = A x over the view slots; angle index views[i] (or i). */
-void oracle_forward(const ogeom *g, const double *angles, const REAL *vox, const int32_t *pix, int npix,
+/* per-view parameter block: one angle, or a translation vector */
+static const double *view_params(const ogeom *g, const double *vparams, int v) {
+    return vparams + (size_t)v * (g->kind == 2 ? 3 : 1);
+}
+
+/* sino (nviews, R, C) = A x over the view slots; view parameter index views[i] (or i). */
+void oracle_forward(const ogeom *g, const double *vparams, const REAL *vox, const int32_t *pix, int npix,
                     const int32_t *views, int nviews, REAL *sino) {
     const int R = g->num_det_rows, C = g->num_det_channels, Nz = g->num_recon_slices;
+    const int fan = g->kind != 0;
     memset(sino, 0, sizeof(REAL) * (size_t)nviews * R * C);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int vi = 0; vi < nviews; ++vi) {
-        const real th = (real)angles[views ? views[vi] : vi];
+        const double *vp = view_params(g, vparams, views ? views[vi] : vi);
+        const real tz = g->kind == 2 ? (real)vp[2] : 0;
         real *sv = sino + (size_t)vi * R * C;
         real *col = (real *)malloc(sizeof(real) * (size_t)R);
         for (int s = 0; s < npix; ++s) {
             real n_p, W_c, cos_a, mag, y;
             int n_c;
-            horizontal(g, pix[s], th, &n_p, &n_c, &W_c, &cos_a, &mag, &y);
+            horizontal(g, pix[s], vp, &n_p, &n_c, &W_c, &cos_a, &mag, &y);
             const REAL *xs = vox + (size_t)s * Nz;
-            if (g->kind == 1) {
+            if (fan) {
                 /* forward_vertical_fan_one_pixel_to_one_view: detector-driven over rows m,
-                 * slices k in round(k_m) +- bp_psf_radius (cone_beam.py:347-372) */
+                 * slices k in round(k_m) +- bp_psf_radius (cone_beam.py:347-372), or
+                 * +- psf_radius for the translation geometry (translation_model.py:348-377) */
+                const int rk = g->kind == 2 ? g->psf_radius : g->bp_psf_radius;
                 const real W = mag * (real)g->delta_voxel / (real)g->delta_det_row;
                 const real Lmax = fmin((real)1, W);
                 for (int m = 0; m < R; ++m) {
                     const real v_m = ((real)m - (real)(R - 1) / 2) * (real)g->delta_det_row - (real)g->det_row_offset;
                     const real z_m = v_m / mag;
-                    const real k_m = (z_m - (real)g->recon_slice_offset) / (real)g->delta_voxel + (real)(Nz - 1) / 2;
+                    const real zs = g->kind == 2 ? z_m + tz : z_m - (real)g->recon_slice_offset;
+                    const real k_m = zs / (real)g->delta_voxel + (real)(Nz - 1) / 2;
                     const int kc = (int)rint(k_m);
                     real acc = 0;
-                    for (int ko = -g->bp_psf_radius; ko <= g->bp_psf_radius; ++ko) {
+                    for (int ko = -rk; ko <= rk; ++ko) {
                         const int k = kc + ko;
                         if (k < 0 || k >= Nz) continue;
-                        const real A = clipr((W + 1) / 2 - fabs(slice_row(g, mag, k) - (real)m), 0, Lmax);
-                        acc += A * (real)xs[k] * inv_cos_phi(g, mag, k);
+                        const real A = clipr((W + 1) / 2 - fabs(slice_row(g, mag, k, tz) - (real)m), 0, Lmax);
+                        acc += A * (real)xs[k] * inv_cos_phi(g, mag, k, tz);
                     }
                     col[m] = acc;
                 }
@@ -140,7 +177,7 @@ void oracle_forward(const ogeom *g, const double *angles, const REAL *vox, const
                 const int n = n_c + no;
                 const real A = chan_w(g, n_p, n, W_c, cos_a);
                 if (A == 0) continue;
-                if (g->kind == 1) {
+                if (fan) {
                     for (int m = 0; m < R; ++m) sv[(size_t)m * C + n] += A * col[m];
                 } else {
                     for (int k = 0; k < Nz; ++k) sv[(size_t)k * C + n] += A * (real)xs[k];
@@ -153,7 +190,7 @@ void oracle_forward(const ogeom *g, const double *angles, const REAL *vox, const
 
 /* out (npix, Nz) = sum over view slots of A_v^T sino[view]; the slot uses
  * sinogram view and angle views[i] (or i).  coeff_power 1 or 2. */
-void oracle_back(const ogeom *g, const double *angles, const REAL *sino, const int32_t *pix, int npix,
+void oracle_back(const ogeom *g, const double *vparams, const REAL *sino, const int32_t *pix, int npix,
                  const int32_t *views, int nviews, int coeff_power, REAL *out) {
     const int R = g->num_det_rows, C = g->num_det_channels, Nz = g->num_recon_slices;
 #pragma omp parallel for schedule(dynamic, 16)
@@ -162,11 +199,12 @@ void oracle_back(const ogeom *g, const double *angles, const REAL *sino, const i
         real *col = (real *)malloc(sizeof(real) * (size_t)R);
         for (int vi = 0; vi < nviews; ++vi) {
             const int vd = views ? views[vi] : vi;
-            const real th = (real)angles[vd];
+            const double *vp = view_params(g, vparams, vd);
+            const real tz = g->kind == 2 ? (real)vp[2] : 0;
             const REAL *sv = sino + (size_t)vd * R * C;
             real n_p, W_c, cos_a, mag, y;
             int n_c;
-            horizontal(g, pix[s], th, &n_p, &n_c, &W_c, &cos_a, &mag, &y);
+            horizontal(g, pix[s], vp, &n_p, &n_c, &W_c, &cos_a, &mag, &y);
             /* back_horizontal_fan: column over rows */
             for (int m = 0; m < R; ++m) col[m] = 0;
             for (int no = -g->psf_radius; no <= g->psf_radius; ++no) {
@@ -176,14 +214,14 @@ void oracle_back(const ogeom *g, const double *angles, const REAL *sino, const i
                 if (coeff_power == 2) A = A * A;
                 for (int m = 0; m < R; ++m) col[m] += A * (real)sv[(size_t)m * C + n];
             }
-            if (g->kind == 1) {
+            if (g->kind != 0) {
                 /* back_vertical_fan: voxel-driven, rows round(m_p) +- psf_radius */
                 const real W = mag * (real)g->delta_voxel / (real)g->delta_det_row;
                 const real Lmax = fmin((real)1, W);
                 for (int k = 0; k < Nz; ++k) {
-                    const real m_p = slice_row(g, mag, k);
+                    const real m_p = slice_row(g, mag, k, tz);
                     const int mc = (int)rint(m_p);
-                    const real icp = inv_cos_phi(g, mag, k);
+                    const real icp = inv_cos_phi(g, mag, k, tz);
                     real t = 0;
                     for (int mo = -g->psf_radius; mo <= g->psf_radius; ++mo) {
                         const int m = mc + mo;

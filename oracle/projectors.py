@@ -6,6 +6,8 @@ and a restatement of the geometry helpers:
   ConeBeamModel.auto_set_recon_size   mbirjax/cone_beam.py:174-187
   ParallelBeamModel.get_psf_radius    mbirjax/parallel_beam.py:115-123
   ParallelBeamModel.auto_set_recon_size mbirjax/parallel_beam.py:125-136
+  TranslationModel.get_psf_radius     mbirjax/translation_model.py:109-140
+  TranslationModel.auto_set_recon_size mbirjax/translation_model.py:142-186
   TomographyModel.__init__ delta_voxel default  mbirjax/tomography_model.py:59-64
 
 Parity pins (tests/test_oracle.py): adjoint identity to ~1e-12, Hessian ==
@@ -44,6 +46,7 @@ class OGeom(ctypes.Structure):
         ("magnification", ctypes.c_double),
         ("delta_voxel", ctypes.c_double),
         ("recon_slice_offset", ctypes.c_double),
+        ("delta_recon_row", ctypes.c_double),
     ]
 
 
@@ -70,19 +73,24 @@ class Geometry:
 
     def __init__(self, kind, sinogram_shape, angles, source_detector_dist=None, source_iso_dist=None,
                  recon_shape=None, delta_det_channel=1.0, delta_det_row=1.0, det_channel_offset=0.0,
-                 det_row_offset=0.0, delta_voxel=None, recon_slice_offset=0.0):
+                 det_row_offset=0.0, delta_voxel=None, recon_slice_offset=0.0, delta_recon_row=None):
+        """kind: "parallel", "cone" or "translation"; angles: (V,) angles, or
+        for "translation" the (V, 3) translation vectors (x, y, z)."""
         self.kind = kind
         self.sinogram_shape = tuple(int(v) for v in sinogram_shape)
-        # The reference holds the angles as a float32 JAX array (jnp.asarray,
-        # mbirjax/cone_beam.py:56, parallel_beam.py:50): evaluate the geometry
-        # in FP64 at those float32 values.
+        # The reference holds the view parameters as float32 JAX arrays
+        # (jnp.asarray, mbirjax/cone_beam.py:56, parallel_beam.py:50,
+        # translation_model.py:63): evaluate the geometry in FP64 at those
+        # float32 values.
         self.angles = np.asarray(angles, dtype=np.float64).astype(np.float32).astype(np.float64)
+        if kind == "translation":
+            self.angles = self.angles.reshape(-1, 3)
         self.delta_det_channel = float(delta_det_channel)
         self.delta_det_row = float(delta_det_row)
         self.det_channel_offset = float(det_channel_offset)
         self.det_row_offset = float(det_row_offset)
         self.recon_slice_offset = float(recon_slice_offset)
-        if kind == "cone":
+        if kind in ("cone", "translation"):
             self.source_detector_dist = float(source_detector_dist)
             self.source_iso_dist = float(source_iso_dist)
         else:
@@ -90,12 +98,32 @@ class Geometry:
             self.source_iso_dist = math.inf
         mag = self.magnification()
         self.delta_voxel = float(delta_voxel) if delta_voxel is not None else self.delta_det_channel / mag
-        self.recon_shape = tuple(int(v) for v in recon_shape) if recon_shape is not None else self.auto_recon_shape()
+        self.delta_recon_row = float(delta_recon_row) if delta_recon_row is not None else self.delta_voxel
+        if recon_shape is not None:
+            self.recon_shape = tuple(int(v) for v in recon_shape)
+        elif kind == "translation":
+            self.recon_shape, self.delta_recon_row = self.auto_translation_recon()
+        else:
+            self.recon_shape = self.auto_recon_shape()
 
     def magnification(self):
-        if self.kind != "cone" or math.isinf(self.source_detector_dist):
+        if self.kind == "parallel" or math.isinf(self.source_detector_dist):
             return 1.0
         return self.source_detector_dist / self.source_iso_dist
+
+    def auto_translation_recon(self):
+        """(recon_shape, delta_recon_row) as TranslationModel.auto_set_recon_size."""
+        V, R, C = self.sinogram_shape
+        mag = self.magnification()
+        box = np.array([self.delta_det_channel * C, self.delta_det_row * R])
+        half_cone = math.atan2(box.max(), 2 * self.source_detector_dist)
+        drow = math.sqrt(2) * (self.delta_voxel / math.tan(half_cone))
+        t = self.angles.astype(np.float32)
+        cube = t.max(axis=0) - t.min(axis=0)
+        rbox = np.ceil((np.array([cube[0], cube[2]]) + box / mag / 2) / self.delta_voxel)
+        rows = math.ceil(V * C * R / (rbox[0] * rbox[1]))
+        rows = min(rows, math.floor((self.source_iso_dist - cube[1]) / drow))
+        return (int(rows), int(rbox[0]), int(rbox[1])), drow
 
     def auto_recon_shape(self):
         _, R, C = self.sinogram_shape
@@ -109,6 +137,12 @@ class Geometry:
         return (rows, rows, slices)
 
     def psf_radii(self):
+        if self.kind == "translation":
+            d_det = min(self.delta_det_row, self.delta_det_channel)
+            closest = self.source_iso_dist - 0.5 * self.recon_shape[0] * self.delta_recon_row - self.angles[:, 1].max()
+            max_mag = self.source_detector_dist / closest
+            psf = int(np.ceil(np.ceil(self.delta_voxel * max_mag / d_det) / 2))
+            return psf, psf
         if self.kind != "cone":
             return int(np.ceil(np.ceil(self.delta_voxel / self.delta_det_channel) / 2)), 1
         d_det = min(self.delta_det_row, self.delta_det_channel)
@@ -128,9 +162,11 @@ class Geometry:
         V, R, C = self.sinogram_shape
         rows, cols, nz = self.recon_shape
         psf, bp = self.psf_radii()
-        return OGeom(1 if self.kind == "cone" else 0, V, R, C, rows, cols, nz, psf, bp, 0,
+        kind = {"parallel": 0, "cone": 1, "translation": 2}[self.kind]
+        return OGeom(kind, V, R, C, rows, cols, nz, psf, bp, 0,
                      self.delta_det_row, self.delta_det_channel, self.det_row_offset, self.det_channel_offset,
-                     self.source_detector_dist, self.magnification(), self.delta_voxel, self.recon_slice_offset)
+                     self.source_detector_dist, self.magnification(), self.delta_voxel, self.recon_slice_offset,
+                     self.delta_recon_row)
 
 
 def _views(views, n_default):

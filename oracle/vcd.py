@@ -66,6 +66,8 @@ class OracleModel:
     def direct_filter(self, sinogram):
         if self.geom.kind == "cone":
             return filters.fdk_filter(self.geom, sinogram)
+        if self.geom.kind == "translation":  # no direct recon: zeros (mbirjax/tomography_model.py:934-966)
+            return np.zeros_like(np.asarray(sinogram, np.float64))
         return filters.fbp_filter(self.geom, sinogram)
 
     def direct_recon(self, sinogram):

@@ -45,6 +45,22 @@ def make_par(V, R, C, span=np.pi, **params):
     return m, g
 
 
+def make_trans(V, R, C, sdd, sid, recon_shape, delta_recon_row, seed=0, tv=None, **params):
+    if tv is None:
+        rng = np.random.default_rng(seed)
+        tv = np.stack([rng.uniform(-3, 3, V), rng.uniform(-2, 2, V), rng.uniform(-2, 2, V)], axis=1)
+    m = mj.TranslationModel((V, R, C), tv, source_detector_dist=sdd, source_iso_dist=sid)
+    m.set_params(recon_shape=recon_shape, delta_recon_row=delta_recon_row, **params)
+    g = op.Geometry("translation", (V, R, C), m.get_params('translation_vectors'), sdd, sid,
+                    recon_shape=m.get_params('recon_shape'), delta_det_row=m.get_params('delta_det_row'),
+                    delta_det_channel=m.get_params('delta_det_channel'),
+                    det_row_offset=m.get_params('det_row_offset'),
+                    det_channel_offset=m.get_params('det_channel_offset'), delta_voxel=m.get_params('delta_voxel'),
+                    delta_recon_row=m.get_params('delta_recon_row'))
+    assert g.psf_radii()[0] == m.get_psf_radius()
+    return m, g
+
+
 CASES = {
     "cone_m2": lambda: make_cone(24, 20, 36, 4 * 36, 2 * 36),
     "cone_m1_ref_test_geometry": lambda: make_cone(16, 10, 32, 4 * 32, 4 * 32, span=np.pi),
@@ -57,6 +73,10 @@ CASES = {
     # the back projector, several row blocks in the forward projector
     "parallel_multichunk": lambda: make_par(6, 1100, 10),
     "cone_multichunk": lambda: make_cone(6, 1300, 12, 2600.0, 1300.0, recon_shape=(8, 8, 1200)),
+    "translation": lambda: make_trans(12, 20, 36, 144.0, 72.0, (6, 30, 24), 1.5),
+    "translation_grid_offsets": lambda: make_trans(20, 24, 40, 160.0, 80.0, (8, 44, 30), 1.0,
+                                                   tv=mj.utilities.gen_translation_vectors(5, 4, 4.0, 3.0),
+                                                   det_row_offset=0.8, det_channel_offset=-1.1),
 }
 
 
@@ -77,7 +97,7 @@ def test_full_projection_parity(gpu, case):
     assert rel(h, op.back(g, np.abs(y), pix, coeff_power=2)) <= TOL
 
 
-@pytest.mark.parametrize("case", ["cone_m2", "parallel", "cone_offsets_odd"])
+@pytest.mark.parametrize("case", ["cone_m2", "parallel", "cone_offsets_odd", "translation"])
 def test_subset_and_view_parity(gpu, case):
     print("case", case, flush=True)
     m, g = CASES[case]()
@@ -138,7 +158,7 @@ def test_benchmark_c4_subset_parity(gpu):
     assert rb <= TOL
 
 
-@pytest.mark.parametrize("case", ["cone_m2", "parallel"])
+@pytest.mark.parametrize("case", ["cone_m2", "parallel", "translation"])
 def test_adjoint_fp32(gpu, case):
     m, g = CASES[case]()
     rows, cols, nz = m.get_params('recon_shape')

@@ -239,6 +239,37 @@ def test_c1_parallel_64_recon_vs_oracle(gpu):
     assert nrmse <= 1e-3
 
 
+def test_translation_recon_vs_oracle(gpu):
+    """TranslationModel: 5 x 4 grid of x-z translations (mbirjax
+    utilities.gen_translation_vectors), 24 x 40 detector, sdd 160, sid 80,
+    recon 8 x 44 x 30 with row pitch 1.0, ellipsoid phantom + 1 % noise,
+    qggmrf_nbr_wts (0.5, 1, 1), zero initial recon (the geometry has no direct
+    recon), 10 iterations."""
+    V, R, C = 20, 24, 40
+    tv = mj.utilities.gen_translation_vectors(5, 4, 4.0, 3.0)
+    m = mj.TranslationModel((V, R, C), tv, source_detector_dist=160.0, source_iso_dist=80.0)
+    m.set_params(recon_shape=(8, 44, 30), delta_recon_row=1.0)
+    g = op.Geometry("translation", (V, R, C), tv, 160.0, 80.0, recon_shape=(8, 44, 30), delta_recon_row=1.0)
+    assert m.get_params('qggmrf_nbr_wts') == (0.5, 1, 1)
+    phantom = mj.utilities.gen_random_ellipsoid_phantom(g.recon_shape, seed=0)
+    ror = np.flatnonzero(opart.get_2d_ror_mask(g.recon_shape))
+    sino = op.forward(g, phantom.reshape(-1, 30)[ror], ror)
+    sino = (sino + 0.01 * np.max(sino) * np.random.default_rng(1).standard_normal(sino.shape)).astype(np.float32)
+    m.set_params(sharpness=0.0, verbose=0)
+    np.random.seed(0)
+    with pytest.warns(UserWarning):  # direct recon is not available: zero initial recon
+        recon, _ = m.recon(sino, max_iterations=10, stop_threshold_change_pct=0, print_logs=False,
+                           logfile_path=None)
+    np.random.seed(0)
+    ref, _ = ovcd.OracleModel(g, sharpness=0.0, qggmrf_nbr_wts=[0.5, 1.0, 1.0]).recon(
+        sino, max_iterations=10, stop_threshold_change_pct=0)
+    r = recon.cpu().numpy().astype(np.float64)
+    nrmse = np.linalg.norm(r - ref) / np.linalg.norm(ref)
+    print("translation recon NRMSE vs oracle %.3e, vs phantom %.3f" % (
+        nrmse, np.linalg.norm(r - phantom) / np.linalg.norm(phantom)), flush=True)
+    assert nrmse <= 1e-3
+
+
 def test_c2_cone_recon_vs_oracle(gpu):
     """C2: cone 128 views over [0, 2 pi), 128 x 192 detector, sdd 768, sid 384
     (recon 192 x 192 x 128), ellipsoid phantom + 1 % noise.  The FP64 oracle

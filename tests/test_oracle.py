@@ -99,8 +99,16 @@ def test_psf_radii_at_benchmark_shapes():
     assert g1.recon_shape == (64, 64, 64) and g1.psf_radii() == (1, 1)
 
 
+def trans_geom(V=6, R=8, C=12, sdd=48.0, sid=24.0, recon_shape=(4, 10, 8), delta_recon_row=1.3, seed=0, **kw):
+    rng = np.random.default_rng(seed)
+    tv = np.stack([rng.uniform(-2, 2, V), rng.uniform(-1, 1, V), rng.uniform(-1.5, 1.5, V)], axis=1)
+    return op.Geometry("translation", (V, R, C), tv, sdd, sid, recon_shape=recon_shape,
+                       delta_recon_row=delta_recon_row, **kw)
+
+
 @pytest.mark.parametrize("geom", [cone_geom(), cone_geom(det_row_offset=0.7, det_channel_offset=-1.3,
-                                                         recon_slice_offset=0.4), par_geom()])
+                                                         recon_slice_offset=0.4), par_geom(),
+                                  trans_geom(), trans_geom(det_row_offset=0.6, det_channel_offset=-0.9, seed=1)])
 def test_adjoint_fp64(geom):
     rng = np.random.default_rng(3)
     rows, cols, nz = geom.recon_shape
@@ -112,7 +120,8 @@ def test_adjoint_fp64(geom):
     assert abs(lhs - rhs) <= 1e-11 * (abs(lhs) + 1)
 
 
-@pytest.mark.parametrize("geom", [cone_geom(V=5, R=5, C=6, sdd=24.0, sid=12.0), par_geom(V=5, R=3, C=5)])
+@pytest.mark.parametrize("geom", [cone_geom(V=5, R=5, C=6, sdd=24.0, sid=12.0), par_geom(V=5, R=3, C=5),
+                                  trans_geom(V=3, R=5, C=6, sdd=24.0, sid=12.0, recon_shape=(2, 5, 4))])
 def test_hessian_is_squared_column_norm(geom):
     A = op.dense_matrix(geom)
     rows, cols, nz = geom.recon_shape
@@ -150,6 +159,48 @@ def test_cone_far_source_equals_parallel():
     x = np.random.default_rng(6).random((rows * cols, nz))
     pix = np.arange(rows * cols)
     assert np.allclose(op.forward(gc, x, pix), op.forward(gp, x, pix), rtol=1e-6, atol=1e-8)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_translation_shift_equivalence(axis):
+    """Translating the object by one voxel pitch along x (cols), y (rows, at
+    delta_recon_row) or z (slices) equals shifting the voxel array by one:
+    pins the sign, axis and pitch of each translation component."""
+    # translations and pitches exact in float32 (the view parameters are
+    # held as float32), so that t + pitch is exact
+    rng = np.random.default_rng(2)
+    tv = rng.integers(-64, 64, size=(4, 3)) / 32.0
+    geom = op.Geometry("translation", (4, 8, 12), tv, 48.0, 24.0, recon_shape=(4, 9, 7), delta_recon_row=1.25)
+    rows, cols, nz = geom.recon_shape
+    x = np.random.default_rng(8).random((rows, cols, nz))
+    arr_axis, pitch = {0: (1, geom.delta_voxel), 1: (0, geom.delta_recon_row), 2: (2, geom.delta_voxel)}[axis]
+    idx = [slice(None)] * 3
+    idx[arr_axis] = -1
+    x[tuple(idx)] = 0.0
+    xs = np.roll(x, 1, axis=arr_axis)  # voxel j -> j + 1 along the axis
+    shifted = op.Geometry("translation", geom.sinogram_shape, geom.angles + pitch * np.eye(3)[axis],
+                          geom.source_detector_dist, geom.source_iso_dist, recon_shape=geom.recon_shape,
+                          delta_recon_row=geom.delta_recon_row)
+    pix = np.arange(rows * cols)
+    a = op.forward(geom, x.reshape(-1, nz), pix)
+    b = op.forward(shifted, xs.reshape(-1, nz), pix)
+    assert np.max(np.abs(a)) > 0
+    assert np.allclose(a, b, rtol=1e-9, atol=1e-12)
+
+
+def test_translation_far_source_is_parallel_over_dv():
+    """With the source at infinity (magnification 1) and no translation, the
+    translation projector is the parallel projector at angle 0 without its
+    delta_voxel factor (mbirjax/translation_model.py:276-284 vs
+    parallel_beam.py:223-263)."""
+    V, R, C, dv = 1, 5, 8, 2.0
+    common = dict(recon_shape=(3, C, R), delta_voxel=dv, delta_det_channel=dv, delta_det_row=dv)
+    gt = op.Geometry("translation", (V, R, C), np.zeros((V, 3)), 1e9, 1e9, delta_recon_row=1.0, **common)
+    gp = op.Geometry("parallel", (V, R, C), np.zeros(V), **common)
+    rows, cols, nz = gt.recon_shape
+    x = np.random.default_rng(9).random((rows * cols, nz))
+    pix = np.arange(rows * cols)
+    assert np.allclose(op.forward(gt, x, pix), op.forward(gp, x, pix) / dv, rtol=1e-6, atol=1e-9)
 
 
 def test_view_subsets_are_additive():
