@@ -39,6 +39,9 @@
  *                           vcd_subset_updater (mbirjax/tomography_model.py:1310-1544)
  *                           with get_forward_lin_quad (1546-1631) and
  *                           update_recon (1820-1823)
+ *   mbx_denoise_subset_update <- vcd_subset_denoiser (mbirjax/denoising.py:366-411),
+ *                           driven by QGGMRFDenoiser.denoise (142-364)
+ *   mbx_median_filter3d  <- median_filter3d (mbirjax/denoising.py:414-496)
  *   mbx_dot              <- the jnp.sum reductions of vcd_recon / get_forward_model_loss
  *                           (mbirjax/tomography_model.py:1139-1152, 1633-1658)
  *
@@ -65,6 +68,7 @@ extern "C" {
 
 #define MBX_DTYPE_F32 0
 #define MBX_DTYPE_BF16 1
+#define MBX_DTYPE_F64 2 /* mbx_median_filter3d only */
 
 #define MBX_LAYOUT_VCR 0 /* (views, channels, rows): projector-native */
 #define MBX_LAYOUT_VRC 1 /* (views, rows, channels): mbirjax user layout */
@@ -186,6 +190,24 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
                           double fm_constant, const mbx_qggmrf *qparams, const float *prox_input,
                           double sigma_prox, int32_t positivity, int32_t delta_dtype, int32_t kahan,
                           double *stats, void *workspace, int64_t workspace_bytes, void *stream);
+
+/* One subset update of the qGGMRF proximal-map denoiser (vcd_subset_denoiser,
+ * mbirjax/denoising.py:366-411): the identity forward model on the image,
+ * so flat_error = noisy image - flat_image and the forward Hessian is 1.
+ * flat_image, flat_error: (num_rows*num_cols, num_slices) f32, updated in
+ * place at the cylinders pix.  fm_constant = 1/sigma_noise^2.  stats (device
+ * double[4], nullable) accumulates {sum |alpha*delta|, alpha, 1, 0}.
+ * workspace: mbx_denoise_workspace_bytes() bytes. */
+int64_t mbx_denoise_workspace_bytes(int32_t npix, int32_t num_slices);
+int mbx_denoise_subset_update(float *flat_image, float *flat_error, int32_t num_rows, int32_t num_cols,
+                              int32_t num_slices, const int32_t *pix, int32_t npix, double fm_constant,
+                              const mbx_qggmrf *qparams, double *stats, void *workspace, int64_t workspace_bytes,
+                              void *stream);
+
+/* 27-point (3x3x3) median filter with edge-replicated borders
+ * (median_filter3d, mbirjax/denoising.py:414-496).  x, out: (d0, d1, d2) in
+ * dtype MBX_DTYPE_F32 or MBX_DTYPE_F64, distinct buffers. */
+int mbx_median_filter3d(const void *x, int32_t dtype, int32_t d0, int32_t d1, int32_t d2, void *out, void *stream);
 
 #ifdef __cplusplus
 }

@@ -11,6 +11,7 @@ from .tomography_model import TomographyModel, ReconParams
 from .parallel_beam import ParallelBeamModel
 from .cone_beam import ConeBeamModel
 from .translation_model import TranslationModel
+from .denoising import QGGMRFDenoiser, median_filter3d
 from .qggmrf import (get_b_from_nbr_wts, qggmrf_gradient_and_hessian_at_indices, qggmrf_loss,
                      prox_gradient_at_indices)
 from .vcd_utils import (get_2d_ror_mask, gen_pixel_partition, gen_set_of_pixel_partitions, gen_partition_sequence,
@@ -20,7 +21,7 @@ from . import utilities
 from . import autograd
 
 __all__ = ['Param', 'ParameterHandler', 'TomographyModel', 'ReconParams', 'ParallelBeamModel', 'ConeBeamModel',
-           'TranslationModel',
+           'TranslationModel', 'QGGMRFDenoiser', 'median_filter3d',
            'get_b_from_nbr_wts', 'qggmrf_gradient_and_hessian_at_indices', 'qggmrf_loss',
            'prox_gradient_at_indices', 'get_2d_ror_mask', 'gen_pixel_partition', 'gen_set_of_pixel_partitions',
            'gen_partition_sequence', 'gen_full_indices', 'generate_direct_recon_filter', 'utilities', 'autograd']

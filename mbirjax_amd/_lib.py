@@ -23,6 +23,7 @@ GEOM_CONE = 1
 GEOM_TRANSLATION = 2
 DTYPE_F32 = 0
 DTYPE_BF16 = 1
+DTYPE_F64 = 2
 LAYOUT_VCR = 0
 LAYOUT_VRC = 1
 
@@ -31,6 +32,7 @@ EXPORTS = (
     "mbx_version", "mbx_last_error", "mbx_back_workspace_bytes", "mbx_forward_workspace_bytes",
     "mbx_forward_project", "mbx_back_project", "mbx_sino_transpose", "mbx_qggmrf_grad_hess",
     "mbx_dot_workspace_bytes", "mbx_dot", "mbx_sino_axpy", "mbx_vcd_workspace_bytes", "mbx_vcd_subset_update",
+    "mbx_denoise_workspace_bytes", "mbx_denoise_subset_update", "mbx_median_filter3d",
 )
 
 
@@ -95,6 +97,9 @@ def lib():
             "mbx_vcd_workspace_bytes": (i64, [G, i32, i32]),
             "mbx_vcd_subset_update": (i32, [G, vp, vp, vp, i32, vp, vp, vp, i32, f64, Q, vp, f64, i32, i32, i32, vp,
                                             vp, i64, vp]),
+            "mbx_denoise_workspace_bytes": (i64, [i32, i32]),
+            "mbx_denoise_subset_update": (i32, [vp, vp, i32, i32, i32, vp, i32, f64, Q, vp, vp, i64, vp]),
+            "mbx_median_filter3d": (i32, [vp, i32, i32, i32, i32, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

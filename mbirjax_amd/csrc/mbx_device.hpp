@@ -156,6 +156,22 @@ struct DevGeom {
     mbx_geom h; // the caller's double-precision geometry (table kernel)
 };
 
+// Sum of v over the block (blockDim.x a multiple of 64, at most 8 waves),
+// in a fixed order; valid in thread 0.  scratch: 8 elements of shared memory.
+template <typename T>
+__device__ __forceinline__ T block_reduce(T v, T *scratch) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    T r = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += scratch[i];
+    __syncthreads();
+    return r;
+}
+
 // Cone and translation geometries share the fan projectors (the vertical and
 // horizontal fans); only the per-(pixel, view) table differs.
 __host__ __device__ inline bool is_fan(const DevGeom &g) { return g.kind != MBX_GEOM_PARALLEL; }

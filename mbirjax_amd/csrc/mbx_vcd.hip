@@ -122,19 +122,6 @@ __global__ __launch_bounds__(256) void prox_grad_kernel(const float *__restrict_
 // Reductions
 // ======================================================================
 
-template <typename T>
-__device__ __forceinline__ T block_reduce(T v, T *scratch) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    __syncthreads();
-    if (lane == 0) scratch[wave] = v;
-    __syncthreads();
-    T r = 0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += scratch[i];
-    __syncthreads();
-    return r;
-}
 
 __global__ __launch_bounds__(256) void sum_partials_kernel(const double *__restrict__ part, int64_t n,
                                                            double *__restrict__ out, int accumulate) {

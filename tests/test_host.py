@@ -186,3 +186,21 @@ def test_auto_regularization_matches_oracle():
         om = ovcd.OracleModel(op.Geometry("cone", shape, ang, 64, 32), sharpness=0.5)
         ro = om.auto_set_regularization_params(sino, weights=weights)
         assert r == ro
+
+
+def test_denoiser_host_statistics_match_oracle():
+    """QGGMRFDenoiser's noise estimate and automatic sigma_x (host NumPy) equal
+    the oracle's restatement (mbirjax/denoising.py:80-137)."""
+    from oracle import denoise as oden
+    shape = (16, 16, 12)
+    clean = mj.utilities.gen_random_ellipsoid_phantom(shape, seed=3)
+    noisy = (clean + 0.05 * np.random.default_rng(4).standard_normal(shape)).astype(np.float32)
+    d = mj.QGGMRFDenoiser(shape)
+    assert d.get_params(['granularity', 'partition_sequence', 'sharpness']) == [[16], [0], 0]
+    assert d.estimate_image_noise_std(noisy) == pytest.approx(oden.estimate_image_noise_std(noisy), rel=1e-6)
+    d.set_params(sigma_noise=0.05)
+    reg = d.auto_set_regularization_params(noisy)
+    assert reg['sigma_y'] == pytest.approx(0.05)
+    assert reg['sigma_x'] == pytest.approx(oden.sigma_x_auto(noisy, 0.05, 0.0), rel=1e-6)
+    with pytest.raises(NotImplementedError):
+        d.recon(noisy)

@@ -9,6 +9,7 @@ import os
 import numpy as np
 import pytest
 
+from oracle import denoise as oden
 from oracle import partitions as opart
 from oracle import projectors as op
 from oracle import qggmrf as oq
@@ -238,3 +239,63 @@ def test_partitions_structure_and_counts():
     assert int(opart.get_2d_ror_mask((192, 192)).sum()) == 28600
     assert int(opart.get_2d_ror_mask((512, 512)).sum()) == 205012
     assert int(opart.get_2d_ror_mask((64, 64)).sum()) == 3096
+
+
+# ---------------------------------------------------------------- denoiser
+def test_median_golden_vector():
+    """The reference's 3x3x3 example (mbirjax tests/test_denoiser.py:55-67)."""
+    c = np.array([[[3., 3., 4.], [6., 6., 7.], [6., 7., 8.]],
+                  [[10., 11., 11.], [12., 13., 14.], [15., 15., 16.]],
+                  [[18., 19., 20.], [19., 20., 20.], [22., 23., 23.]]])
+    assert np.array_equal(oden.median_filter3d(np.arange(27).reshape(3, 3, 3)), c)
+
+
+def test_median_equals_scipy_nearest():
+    from scipy import ndimage
+    x = np.random.default_rng(10).standard_normal((9, 7, 5))
+    assert np.array_equal(oden.median_filter3d(x), ndimage.median_filter(x, size=3, mode='nearest'))
+
+
+def test_denoiser_quadratic_fixed_point():
+    """With p = q = 2, 2 b~(delta) = b / (2 sigma_x^2) for every delta, so the
+    prior gradient is L x with L the graph Laplacian of the 6-neighbourhood with
+    edge weights b_axis / (2 sigma_x^2) (clipped in-plane borders and reflected
+    slice ends contribute no edge), and the denoiser's limit solves
+    (I/sigma^2 + L) x = y/sigma^2.  rows*cols is a multiple of the 16 subsets
+    (no padded duplicates).  The step's absolute eps32 in the denominator
+    slows the last digits, so the bar is 3e-5 after 50 sweeps; a wrong
+    Laplacian weight or sign is off by O(1)."""
+    from scipy.sparse import lil_matrix
+    shape = (8, 4, 3)
+    rows, cols, nz = shape
+    rng = np.random.default_rng(11)
+    y = rng.random(shape)
+    sigma, sigma_x = 0.3, 0.25
+    b = oq.get_b_from_nbr_wts((1.0, 0.7, 1.3))
+    n = y.size
+    idx = np.arange(n).reshape(shape)
+    Lap = lil_matrix((n, n))
+    for axis, wb in ((0, b[0]), (1, b[2]), (2, b[4])):
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[axis], hi[axis] = slice(0, -1), slice(1, None)
+        for u, v in zip(idx[tuple(lo)].ravel(), idx[tuple(hi)].ravel()):
+            w = wb / (2 * sigma_x ** 2)
+            Lap[u, u] += w
+            Lap[v, v] += w
+            Lap[u, v] -= w
+            Lap[v, u] -= w
+    sigma = float(np.float32(sigma))
+    A = (np.eye(n) / sigma ** 2) + Lap.toarray()
+    x_star = np.linalg.solve(A, y.ravel() / sigma ** 2).reshape(shape)
+    np.random.seed(0)
+    x, _ = oden.denoise(y, np.float32(sigma), p=2.0, q=2.0, nbr_wts=(1.0, 0.7, 1.3), sigma_x=sigma_x,
+                        max_iterations=50, stop_threshold_change_pct=0.0)
+    assert np.linalg.norm(x - x_star) <= 3e-5 * np.linalg.norm(x_star)
+
+
+def test_denoiser_constant_image_is_fixed():
+    np.random.seed(1)
+    y = np.full((6, 5, 4), 0.7)
+    x, nmae = oden.denoise(y, 0.1, sigma_x=0.05, max_iterations=3, stop_threshold_change_pct=0.0)
+    assert np.allclose(x, y, rtol=0, atol=1e-12)
