@@ -243,7 +243,11 @@ def main():
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kname, "bytes_per_launch": b_dom, "avg_launch_ms": t_dom * 1e3,
+                     "kernel": kname,
+                     "timed": ("one %s projection call of the subset: geometry table%s + %s" % (
+                         "forward" if dominant == "fwd" else "back",
+                         " + binning" if dominant == "fwd" else "", kname)),
+                     "bytes_per_launch": b_dom, "avg_launch_ms": t_dom * 1e3,
                      "traffic_source": traffic_src},
         "projector_GBps": r["proj_GBps"],
         "precisions": {k: {kk: vv for kk, vv in v.items() if kk not in ("b_back", "b_fwd")}
