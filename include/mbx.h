@@ -85,7 +85,7 @@ extern "C" {
  * translation vectors (x, y, z) for MBX_GEOM_TRANSLATION, whose forward
  * vertical fan uses psf_radius (bp_psf_radius is ignored). */
 typedef struct mbx_geom {
-    int32_t kind;             /* MBX_GEOM_PARALLEL or MBX_GEOM_CONE */
+    int32_t kind;             /* MBX_GEOM_PARALLEL, MBX_GEOM_CONE or MBX_GEOM_TRANSLATION */
     int32_t num_views;        /* length of the angles array */
     int32_t num_det_rows;     /* R */
     int32_t num_det_channels; /* C */
