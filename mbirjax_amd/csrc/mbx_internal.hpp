@@ -70,7 +70,7 @@ struct FwdPlan {
     size_t lds;   // dynamic LDS bytes
     int64_t nblocks;
 };
-FwdPlan plan_fwd(const DevGeom &g, int nviews);
+FwdPlan plan_fwd(const DevGeom &g, int nviews, int layout);
 int64_t fwd_workspace_bytes(const DevGeom &g, int npix, int nviews);
 // Forward scratch besides the geometry table (partials, binned pixel lists).
 int64_t fwd_aux_bytes(const DevGeom &g, int chunk, int nviews);

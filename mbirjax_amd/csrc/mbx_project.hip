@@ -760,33 +760,41 @@ int launch_back(const DevGeom &g, const float *angles, const void *sino, int sin
 // Forward projection
 // ======================================================================
 
-constexpr int kFwdCB = 16;        // channels per block: the per-lane register tile
-constexpr int kFwdRPL = 2;        // detector rows per lane (64 apart)
+constexpr int kFwdCB = 16;        // channels per block, (views, rows, channels) output: the per-lane register tile
+constexpr int kFwdCBW = 32;       // channels per block, (views, channels, rows) output: the sliding window's sweep
+constexpr int kFwdRPL = 2;        // detector rows per lane (64 apart), (views, rows, channels) output
+constexpr int kFwdRPLW = 4;       // detector rows per lane, (views, channels, rows) output
 constexpr int kFwdPad = 5;        // zero slices either side of a staged cylinder (>= 2*bp_psf_radius + 1)
 constexpr int kFwdMaxBatch = 16;  // cylinders staged per batch
-constexpr size_t kFwdLds = 36 * 1024;  // both staging buffers of a block
+constexpr size_t kFwdLds = 36 * 1024;   // both staging buffers of a block, (views, rows, channels) output
+constexpr size_t kFwdLdsW = 20 * 1024;  // the same, (views, channels, rows) output: small batches, more blocks per CU
 constexpr int kBinMax = 16384;    // channel buckets the binning kernel holds in LDS
 
-FwdPlan plan_fwd(const DevGeom &g, int nviews) {
+FwdPlan plan_fwd(const DevGeom &g, int nviews, int layout) {
     FwdPlan p;
-    const int slab = kWave * kFwdRPL;  // rows per wave
+    const int slab = kWave * (layout == MBX_LAYOUT_VCR ? kFwdRPLW : kFwdRPL);  // rows per wave
     p.nw = (std::min(g.R, 4 * slab) + slab - 1) / slab;
     p.rows_pb = p.nw * slab;
     p.nrb = (g.R + p.rows_pb - 1) / p.rows_pb;
-    p.cb = kFwdCB;
+    p.cb = layout == MBX_LAYOUT_VCR ? kFwdCBW : kFwdCB;
     p.ncb = (g.C + p.cb - 1) / p.cb;
     p.list = 0;
     // A staged cylinder holds the slices the block's rows reach (at most all
     // slices) in whole 64-slice chunks, and the zero pads.
     const int span_max = is_fan(g) ? g.Nz : p.rows_pb;
     p.cstride = ((span_max + kWave - 1) / kWave) * kWave + 2 * kFwdPad + 6;
-    p.nb = (int)std::max<size_t>(1, std::min<size_t>(kFwdMaxBatch, kFwdLds / (2 * (size_t)p.cstride * sizeof(float))));
+    const size_t budget = layout == MBX_LAYOUT_VCR ? kFwdLdsW : kFwdLds;
+    p.nb = (int)std::max<size_t>(1, std::min<size_t>(kFwdMaxBatch, budget / (2 * (size_t)p.cstride * sizeof(float))));
     p.lds = 2 * (size_t)p.nb * p.cstride * sizeof(float) + 64 * sizeof(int);
     p.nblocks = (int64_t)nviews * p.ncb * p.nrb;
     return p;
 }
 
 static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+// FP64 partial slots of the squared norm: one per block of either output layout
+static int64_t fwd_part_blocks(const DevGeom &g, int nviews) {
+    return std::max(plan_fwd(g, nviews, MBX_LAYOUT_VCR).nblocks, plan_fwd(g, nviews, MBX_LAYOUT_VRC).nblocks);
+}
 static int fwd_nbk(const DevGeom &g) { return g.C + 2 * g.rh; }  // centre-channel buckets n_c + r
 
 // Forward record of one (view, pixel) pair, stored in the view's binned order
@@ -805,7 +813,7 @@ constexpr int kFRec = 16;
 // squared norm, then per view the pixel list sorted by centre channel, the
 // bucket starts and the forward records in that order.
 int64_t fwd_aux_bytes(const DevGeom &g, int chunk, int nviews) {
-    return (int64_t)(al256((size_t)plan_fwd(g, nviews).nblocks * sizeof(double)) +
+    return (int64_t)(al256((size_t)fwd_part_blocks(g, nviews) * sizeof(double)) +
                      al256((size_t)nviews * chunk * sizeof(int)) +
                      al256((size_t)nviews * (fwd_nbk(g) + 1) * sizeof(int)) +
                      al256((size_t)nviews * chunk * kFRec * sizeof(float)));
@@ -1028,8 +1036,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     const int *__restrict__ cstart, const float *__restrict__ vox, int npix, int ncb, int nrb, int nbatch,
     int cstride, OT *__restrict__ sino, int accumulate, const float *__restrict__ w,
     double *__restrict__ part_sumsq) {
-    constexpr int CB = kFwdCB, P = kFwdPad, RPL = kFwdRPL;
+    // (views, channels, rows) output: a sliding window of 2*RAD+1 channels per row
+    constexpr bool SW = LAYOUT == MBX_LAYOUT_VCR;
+    constexpr int CB = SW ? kFwdCBW : kFwdCB, P = kFwdPad, RPL = SW ? kFwdRPLW : kFwdRPL;
     constexpr int ND = CB + 2 * RAD;  // centre-channel buckets n0 - RAD .. n0 + CB - 1 + RAD
+    static_assert(!SW || ND < kWave, "the bucket starts are held one per lane");
     extern __shared__ float lds[];
     const int nw = blockDim.x >> 6;
     float *bufs = lds;                                                  // [2][nbatch][cstride]
@@ -1051,18 +1062,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     const int *lv = list + (size_t)vi * npix;  // pixel slot of each record
     const int nbk = g.C + 2 * RAD;
     const int *csv = cstart + (size_t)vi * (nbk + 1);
-    // bucket starts of this block: bs[D] is the start of centre channel n0 + D - RAD
-    int bs[ND + 1];
+    // bucket starts of this block: bs[D] (or lane D of bsl) is the start of
+    // centre channel n0 + D - RAD
+    int bs[SW ? 1 : ND + 1];
+    int bsl = 0, lo, hi;
+    if constexpr (SW) {
+        bsl = csv[min(n0 + min(lane, ND), nbk)];
+        lo = __builtin_amdgcn_readlane(bsl, 0);
+        hi = __builtin_amdgcn_readlane(bsl, ND);
+    } else {
 #pragma unroll
-    for (int D = 0; D <= ND; ++D) bs[D] = csv[min(n0 + D, nbk)];
-    const int lo = bs[0], hi = bs[ND];
+        for (int D = 0; D <= ND; ++D) bs[D] = csv[min(n0 + D, nbk)];
+        lo = bs[0];
+        hi = bs[ND];
+    }
     const int nbat = (hi - lo + nbatch - 1) / nbatch;
 
-    float acc[RPL][CB];
+    // the register tile (CB channels), or the window: acc[r][t] is channel
+    // n0 - 2*RAD + dc + t while centre-channel bucket dc is being added
+    constexpr int NA = SW ? 2 * RAD + 1 : CB;
+    float acc[RPL][NA];
 #pragma unroll
     for (int r = 0; r < RPL; ++r)
 #pragma unroll
-        for (int c = 0; c < CB; ++c) acc[r][c] = 0.0f;
+        for (int c = 0; c < NA; ++c) acc[r][c] = 0.0f;
 
     // Staging data of a batch, lane i holding slot i's: kf | span << 16, pixel
     // slot, v/sdd at kref and its step per slice.  Loaded a batch ahead, without
@@ -1120,14 +1143,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                 const float du = c1 * (float)kWave;
                 float *px = cyl + P + lane;
                 // full chunks four at a time: the four LDS reads are in flight together
+                float u0q[4];  // v/sdd of this lane's slice in chunk u (of each group of four)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) u0q[u] = fmaf(du, (float)u, u0);
                 int c = 0;
                 for (; c + 4 <= nch - 1; c += 4) {
                     float x[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) x[u] = px[(c + u) * kWave];
+                    const float cf = (float)c;
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        const float uu = fmaf(du, (float)(c + u), u0);
+                        const float uu = fmaf(du, cf, u0q[u]);
                         px[(c + u) * kWave] = x[u] * __builtin_amdgcn_sqrtf(fmaf(uu, uu, 1.0f));
                     }
                 }
@@ -1157,7 +1184,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     float mrow[RPL];  // this lane's rows
 #pragma unroll
     for (int r = 0; r < RPL; ++r) mrow[r] = (float)(wm0 + lane + kWave * r);
-    const int wgrp = (wm0 >> 7) & 31;  // this wave's 128-row group in the record masks
+    const int wgrp = (wm0 >> 7) & 31;  // this wave's first 128-row group in the record masks
+    constexpr unsigned gbits = (1u << (RPL / 2)) - 1;  // and its groups (64 * RPL rows)
 
     // vertical resampling of candidate record i (slot s of the batch at LDS base
     // sb) onto this lane's rows
@@ -1224,6 +1252,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     };
     // bucket D is centre channel n0 + D - RAD: tap t lands on channel D - 2*RAD + t
     auto add_taps = [&](int D, const float4 &R0, int i, const float (&v)[RPL]) {
+        if constexpr (SW) return;  // the window's add_window instead
         float Aall[5] = {0.0f, R0.x, R0.y, R0.z, 0.0f};  // Ac(n_c - 2 .. n_c + 2)
         if (RAD == 2) {
             const float *rec = Tv + (size_t)lv[i] * kRec;
@@ -1237,6 +1266,85 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                 const int c = D - 2 * RAD + t;
                 if (c >= 0 && c < CB) acc[r][c] = fmaf(Aall[2 - RAD + t], v[r], acc[r][c]);
             }
+    };
+
+    // window taps: tap t of the candidate lands on acc[.][t]
+    auto add_window = [&](const float4 &R0, int i, const float (&v)[RPL]) {
+        float Aall[5] = {0.0f, R0.x, R0.y, R0.z, 0.0f};  // Ac(n_c - 2 .. n_c + 2)
+        if (RAD == 2) {
+            const float *rec = Tv + (size_t)lv[i] * kRec;
+            Aall[0] = rec[1];
+            Aall[4] = rec[5];
+        }
+#pragma unroll
+        for (int r = 0; r < RPL; ++r)
+#pragma unroll
+            for (int t = 0; t <= 2 * RAD; ++t) acc[r][t] = fmaf(Aall[2 - RAD + t], v[r], acc[r][t]);
+    };
+    double ss = 0.0;  // this lane's share of sum w * (A delta)^2
+    // store element (row m, channel n) of the output (optionally adding to what is there)
+    auto put = [&](int m, int n, float v) {
+        const size_t o = LAYOUT == MBX_LAYOUT_VCR ? ((size_t)vi * g.C + n) * (size_t)g.R + m
+                                                  : ((size_t)vi * g.R + m) * (size_t)g.C + n;
+        if (accumulate) v += to_float(sino[o]);
+        const OT st = from_float<OT>(v);
+        sino[o] = st;
+        if (part_sumsq) {
+            const float vs = to_float(st);
+            ss += (double)vs * (double)vs * (w ? (double)w[o] : 1.0);
+        }
+    };
+    // move the window to the next bucket: channel n0 - 2*RAD + dc is complete
+    // (buckets up to its centre + RAD are added) and leaves it
+    int dc = 0;
+    int nextb = SW ? __builtin_amdgcn_readlane(bsl, 1) : 0;  // start of bucket dc + 1
+    // (channel-major output) this lane's first row of channel n0 - 2*RAD + dc;
+    // rows 64 apart are immediate offsets from it
+    const bool all_rows = wm0 + kWave * RPL <= m1;  // wave-uniform
+    OT *pch = sino + ((size_t)vi * g.C + (n0 - 2 * RAD)) * (size_t)g.R + wm0 + lane;
+    const float *wch = w ? w + ((size_t)vi * g.C + (n0 - 2 * RAD)) * (size_t)g.R + wm0 + lane : nullptr;
+    auto advance = [&]() {
+        const int n = n0 - 2 * RAD + dc;
+        if (n >= n0 && n < n1) {  // wave-uniform
+            if (SW && all_rows) {
+                float v[RPL];
+#pragma unroll
+                for (int r = 0; r < RPL; ++r) v[r] = acc[r][0];
+                if (accumulate) {
+#pragma unroll
+                    for (int r = 0; r < RPL; ++r) v[r] += to_float(pch[kWave * r]);
+                }
+                OT st[RPL];
+#pragma unroll
+                for (int r = 0; r < RPL; ++r) {
+                    st[r] = from_float<OT>(v[r]);
+                    pch[kWave * r] = st[r];
+                }
+                if (part_sumsq) {
+#pragma unroll
+                    for (int r = 0; r < RPL; ++r) {
+                        const double vs = (double)to_float(st[r]);
+                        ss += vs * vs * (wch ? (double)wch[kWave * r] : 1.0);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < RPL; ++r) {
+                    const int m = wm0 + lane + kWave * r;
+                    if (m < m1) put(m, n, acc[r][0]);
+                }
+            }
+        }
+        pch += g.R;
+        if (wch) wch += g.R;
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+#pragma unroll
+            for (int t = 0; t < 2 * RAD; ++t) acc[r][t] = acc[r][t + 1];
+            acc[r][2 * RAD] = 0.0f;
+        }
+        ++dc;
+        nextb = __builtin_amdgcn_readlane(bsl, min(dc + 1, ND));
     };
 
     Meta mA{0, 0, 0.0f, 0.0f}, mB{0, 0, 0.0f, 0.0f};
@@ -1255,7 +1363,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         const int b0 = lo + bi * nbatch;
         const int nb = min(nbatch, hi - b0);
         const int boff = (bi & 1) * nbatch * cstride;  // LDS float offset of this batch
-        if (wm0 < m1) {  // wave-uniform
+        if constexpr (SW) {
+            if (wm0 < m1) {  // wave-uniform
+                // the batch's candidates in list order; the window follows their buckets
+                const float4 *rp = (const float4 *)(FRv + (size_t)b0 * kFRec);
+                int sb = boff + g.kref + P - (VM == kVertTwo ? 0 : RB);
+                for (int slot = 0; slot < nb; ++slot, rp += 4, sb += cstride) {
+#pragma clang loop unroll(disable)
+                    while (b0 + slot >= nextb) advance();
+                    const float4 R0 = rp[0], R1 = rp[1], R2 = rp[2], R3 = rp[3];
+                    asm volatile("" ::"s"(R0.x), "s"(R0.y), "s"(R0.z), "s"(R0.w), "s"(R1.x), "s"(R1.y), "s"(R1.z),
+                                 "s"(R1.w), "s"(R2.x), "s"(R2.y), "s"(R2.z), "s"(R2.w), "s"(R3.x), "s"(R3.y));
+                    if (GEOM != MBX_GEOM_CONE || ((__float_as_uint(R3.x) >> wgrp) & gbits)) {
+                        float v[RPL];
+                        vert(R0, R1, R2, R3, b0 + slot, slot, sb, v);
+                        add_window(R0, b0 + slot, v);
+                    }
+                }
+            }
+        } else if (wm0 < m1) {  // wave-uniform
 #pragma unroll
             for (int D = 0; D < ND; ++D) {
                 const int slo = max(bs[D], b0) - b0;
@@ -1267,7 +1393,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                     const float4 R0 = rp[0], R1 = rp[1], R2 = rp[2], R3 = rp[3];
                     asm volatile("" ::"s"(R0.x), "s"(R0.y), "s"(R0.z), "s"(R0.w), "s"(R1.x), "s"(R1.y), "s"(R1.z),
                                  "s"(R1.w), "s"(R2.x), "s"(R2.y), "s"(R2.z), "s"(R2.w), "s"(R3.x), "s"(R3.y));
-                    if (GEOM != MBX_GEOM_CONE || ((__float_as_uint(R3.x) >> wgrp) & 1u)) {
+                    if (GEOM != MBX_GEOM_CONE || ((__float_as_uint(R3.x) >> wgrp) & gbits)) {
                         float v[RPL];
                         vert(R0, R1, R2, R3, b0 + slot, slot, sb, v);
                         add_taps(D, R0, b0 + slot, v);
@@ -1277,26 +1403,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         }
     }
 
-    // ---- write the register tile once (optionally adding to what is there)
-    double ss = 0.0;
+    if constexpr (SW) {
+        // the channels still in (or not yet reached by) the window
+        if (wm0 < m1)
+            while (dc < ND) advance();
+    } else {
+        // ---- write the register tile once
 #pragma unroll
-    for (int r = 0; r < RPL; ++r) {
-        const int m = wm0 + lane + kWave * r;
-        if (m < m1) {
+        for (int r = 0; r < RPL; ++r) {
+            const int m = wm0 + lane + kWave * r;
+            if (m < m1) {
 #pragma unroll
-            for (int c = 0; c < CB; ++c) {
-                const int n = n0 + c;
-                if (n < n1) {
-                    const size_t o = LAYOUT == MBX_LAYOUT_VCR ? ((size_t)vi * g.C + n) * (size_t)g.R + m
-                                                              : ((size_t)vi * g.R + m) * (size_t)g.C + n;
-                    float v = acc[r][c];
-                    if (accumulate) v += to_float(sino[o]);
-                    const OT st = from_float<OT>(v);
-                    sino[o] = st;
-                    if (part_sumsq) {
-                        const float vs = to_float(st);
-                        ss += (double)vs * (double)vs * (w ? (double)w[o] : 1.0);
-                    }
+                for (int c = 0; c < CB; ++c) {
+                    const int n = n0 + c;
+                    if (n < n1) put(m, n, acc[r][c]);
                 }
             }
         }
@@ -1342,7 +1462,7 @@ static void dispatch_fwd_2(const DevGeom &g, const FwdPlan &pl, const float *T, 
 int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float *vox, int npix, int nviews, void *sino,
                    int sino_dtype, int layout, int accumulate, const float *weights, void *aux, int chunk,
                    double *sumsq, hipStream_t st) {
-    const FwdPlan pl = plan_fwd(g, nviews);
+    const FwdPlan pl = plan_fwd(g, nviews, layout);
     if (pl.lds > 160 * 1024)
         return fail(MBX_EUNSUPPORTED, "forward projection LDS footprint too large (num_recon_slices too big)");
     const int nbk = fwd_nbk(g);
@@ -1351,7 +1471,7 @@ int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float 
     if (npix > chunk) return fail(MBX_EINVAL, "forward projection chunk larger than its workspace");
     char *a8 = (char *)aux;
     double *part = (double *)a8;
-    int *list = (int *)(a8 + al256((size_t)pl.nblocks * sizeof(double)));
+    int *list = (int *)(a8 + al256((size_t)fwd_part_blocks(g, nviews) * sizeof(double)));
     int *cstart = (int *)((char *)list + al256((size_t)nviews * chunk * sizeof(int)));
     float *FR = (float *)((char *)cstart + al256((size_t)nviews * (nbk + 1) * sizeof(int)));
     const int vm = fwd_vert_mode(g);
