@@ -764,7 +764,8 @@ constexpr int kFwdCB = 16;        // channels per block, (views, rows, channels)
 constexpr int kFwdCBW = 32;       // channels per block, (views, channels, rows) output: the sliding window's sweep
 constexpr int kFwdRPL = 2;        // detector rows per lane (64 apart), (views, rows, channels) output
 constexpr int kFwdRPLW = 4;       // detector rows per lane, (views, channels, rows) output
-constexpr int kFwdPad = 5;        // zero slices either side of a staged cylinder (>= 2*bp_psf_radius + 1)
+constexpr int kFwdPad = 8;        // zero slices either side of a staged cylinder (>= 2*bp_psf_radius + 1;
+                                  // a multiple of 4 keeps every staged cylinder 16-byte aligned)
 constexpr int kFwdMaxBatch = 16;  // cylinders staged per batch
 constexpr size_t kFwdLds = 36 * 1024;   // both staging buffers of a block, (views, rows, channels) output
 constexpr size_t kFwdLdsW = 20 * 1024;  // the same, (views, channels, rows) output: small batches, more blocks per CU
@@ -782,7 +783,7 @@ FwdPlan plan_fwd(const DevGeom &g, int nviews, int layout) {
     // A staged cylinder holds the slices the block's rows reach (at most all
     // slices) in whole 64-slice chunks, and the zero pads.
     const int span_max = is_fan(g) ? g.Nz : p.rows_pb;
-    p.cstride = ((span_max + kWave - 1) / kWave) * kWave + 2 * kFwdPad + 6;
+    p.cstride = ((span_max + kWave - 1) / kWave) * kWave + 2 * kFwdPad + 8;  // a multiple of 4
     const size_t budget = layout == MBX_LAYOUT_VCR ? kFwdLdsW : kFwdLds;
     p.nb = (int)std::max<size_t>(1, std::min<size_t>(kFwdMaxBatch, budget / (2 * (size_t)p.cstride * sizeof(float))));
     p.lds = 2 * (size_t)p.nb * p.cstride * sizeof(float) + 64 * sizeof(int);
@@ -1118,9 +1119,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             const __amdgpu_buffer_rsrc_t rs =
                 __builtin_amdgcn_make_buffer_rsrc((void *)(vox + (size_t)sc * g.Nz + kf), 0, (g.Nz - kf) * 4,
                                                   0x00020000);
-            float *dst = buf + (size_t)slot * cstride + P;
+            float *dst = buf + (size_t)slot * cstride + P;  // 16-byte aligned
+            // whole 256-slice blocks 16 bytes per lane, the rest 64 slices at a time
+            const int n4 = span >> 8;
+            for (int c = 0; c < n4; ++c)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rs, (__attribute__((address_space(3))) void *)(dst + c * 4 * kWave), 16, (c * kWave + lane) * 16, 0,
+                    0, 0);
             const int nch = (span + kWave - 1) / kWave;
-            for (int c = 0; c < nch; ++c)
+            for (int c = 4 * n4; c < nch; ++c)
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(
                     rs, (__attribute__((address_space(3))) void *)(dst + c * kWave), 4, (c * kWave + lane) * 4, 0, 0,
                     0);
