@@ -53,6 +53,7 @@ struct BackPlan {
     int kpl;     // slices per lane (chunk = 64*kpl slices)
     int nchunk;  // slice chunks
     int cap;     // LDS floats per wave for the detector column
+    int unit;    // every W = mag*dv/dr <= 2 with radius-1 row taps: the unit-sum weights (coeff_power 1)
 };
 BackPlan plan_back(const DevGeom &g, int npix, int nviews, int ppw);
 int64_t back_part_bytes(const DevGeom &g, int chunk, int nviews);  // view-group partials

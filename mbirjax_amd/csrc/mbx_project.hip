@@ -313,12 +313,14 @@ BackPlan plan_back(const DevGeom &g, int npix, int nviews, int ppw) {
     // height + 2 plus a chunk (the clamped fallback layout).
     const int round = kWave;
     p.cap = g.R + 2 + round;
+    p.unit = 0;
     if (g.kind == MBX_GEOM_CONE) {  // (translation: the clamped layout covers every magnification)
         const double rmax = g.dv * std::sqrt((double)g.img_row_center * g.img_row_center +
                                              (double)g.img_col_center * g.img_col_center) + g.dv;
         const double den = (double)g.inv_M - rmax * (double)g.inv_sdd;
         if (den > 0) {
             const double wmax = g.dv_over_dr / den;
+            p.unit = g.rh == 1 && wmax <= 2.0;
             const double want = std::ceil((kWave * p.kpl - 1) * wmax) + 2.0 * g.rh + 8.0 + round;
             if (want < 16384.0) p.cap = std::max(p.cap, (int)want);
         }
@@ -365,10 +367,14 @@ struct ColumnSource {
 // FIT: the column holds every row the chunk's taps reach (plus a zero or
 // don't-care margin row each side), so taps index it directly; otherwise the
 // indices are clamped to the stored rows.
-template <int KPL, int RAD, int POW, bool TAIL, bool FIT>
+// UNIT (coeff_power 1, radius 1, every W <= 2): the clip weights of rows
+// mc-1, mc, mc+1 sum to W (a box of width W convolved with a box of width 1,
+// sampled at unit spacing), and the side weights (W-1)/2 -+ d never exceed
+// W/2 <= min(1, W), so the centre row carries W minus the two sides.
+template <int KPL, int RAD, int POW, bool TAIL, bool FIT, bool UNIT>
 __device__ __forceinline__ void back_vertical(const float *col, int shift, int n_rows, double acd, double Wd,
-                                              float hv, float Lv, float c0, float c1, float kr0, float krlast,
-                                              float (&acc)[KPL]) {
+                                              float Wf, float hv, float Lv, float c0, float c1, float kr0,
+                                              float krlast, float (&acc)[KPL]) {
     // Slice k sits at row ai + t with t = acr + W*(k - kref), formed in FP64 so
     // that the tap distances d = t - rint(t) carry no float32 rounding at the
     // magnitude of a row index; rint is the 1.5*2^52 shift, whose low word is
@@ -376,18 +382,18 @@ __device__ __forceinline__ void back_vertical(const float *col, int shift, int n
     constexpr double kMagic = 6755399441055744.0;
     const float *colp = col - shift;  // colp[m - RAD] is row ai + m (32-bit LDS address arithmetic)
     const float hm1 = hv - 1.0f;
-    const double t0 = fma(Wd, (double)kr0, acd), W64 = Wd * (double)kWave;
+    const double W64 = Wd * (double)kWave;
     const float u0 = fmaf(c1, kr0, c0), c164 = c1 * (float)kWave;
+    double t = fma(Wd, (double)kr0, acd);
 #pragma unroll
     for (int i = 0; i < KPL; ++i) {
-        double t;
         float u;
         if (TAIL) {  // lanes past the last slice repeat it (their sums are never stored)
             const float kr = fminf(kr0 + (float)(kWave * i), krlast);
             t = fma(Wd, (double)kr, acd);
             u = fmaf(c1, kr, c0);
         } else {
-            t = fma(W64, (double)i, t0);
+            if (i > 0) t += W64;  // slices 64 apart: one FP64 add
             u = fmaf(c164, (float)i, u0);
         }
         const double sh = t + kMagic;
@@ -395,35 +401,43 @@ __device__ __forceinline__ void back_vertical(const float *col, int shift, int n
         const float d = (float)(t - (sh - kMagic));  // |d| <= 1/2
         float sum;
         if (RAD == 1) {
-            float wm = __builtin_amdgcn_fmed3f(hm1 - d, 0.0f, Lv);        // row mc-1
-            float w0 = __builtin_amdgcn_fmed3f(hv - fabsf(d), 0.0f, Lv);  // row mc
-            float wp = __builtin_amdgcn_fmed3f(hm1 + d, 0.0f, Lv);        // row mc+1
-            if (POW == 2) {
-                w0 *= w0;
-                wp *= wp;
-                wm *= wm;
-            }
             float cm, cc, cp;
             if (FIT) {
-                cm = colp[mi];
-                cc = colp[mi + 1];
-                cp = colp[mi + 2];
+                const float *q = colp + mi;  // one address, three immediate offsets
+                cm = q[0];
+                cc = q[1];
+                cp = q[2];
             } else {
                 const int base = mi - shift;
                 cm = col[min(max(base, 0), n_rows - 1)];
                 cc = col[min(max(base + 1, 0), n_rows - 1)];
                 cp = col[min(max(base + 2, 0), n_rows - 1)];
             }
-            sum = w0 * cc;
-            sum = fmaf(wp, cp, sum);
-            sum = fmaf(wm, cm, sum);
+            if (UNIT) {
+                const float wm = __builtin_amdgcn_fmed3f(hm1 - d, 0.0f, 1.0f);  // row mc-1
+                const float wp = __builtin_amdgcn_fmed3f(hm1 + d, 0.0f, 1.0f);  // row mc+1
+                sum = fmaf(wm, cm - cc, fmaf(wp, cp - cc, Wf * cc));
+            } else {
+                float wm = __builtin_amdgcn_fmed3f(hm1 - d, 0.0f, Lv);        // row mc-1
+                float w0 = __builtin_amdgcn_fmed3f(hv - fabsf(d), 0.0f, Lv);  // row mc
+                float wp = __builtin_amdgcn_fmed3f(hm1 + d, 0.0f, Lv);        // row mc+1
+                if (POW == 2) {
+                    w0 *= w0;
+                    wp *= wp;
+                    wm *= wm;
+                }
+                sum = w0 * cc;
+                sum = fmaf(wp, cp, sum);
+                sum = fmaf(wm, cm, sum);
+            }
         } else {
             sum = 0.0f;
+            const float *q = colp + mi;
 #pragma unroll
             for (int j = 0; j <= 2 * RAD; ++j) {
                 float wj = __builtin_amdgcn_fmed3f(hv - fabsf(d - (float)(j - RAD)), 0.0f, Lv);
                 if (POW == 2) wj *= wj;
-                const float cj = FIT ? colp[mi + j] : col[min(max(mi - shift + j, 0), n_rows - 1)];
+                const float cj = FIT ? q[j] : col[min(max(mi - shift + j, 0), n_rows - 1)];
                 sum = fmaf(wj, cj, sum);
             }
         }
@@ -436,7 +450,7 @@ __device__ __forceinline__ void back_vertical(const float *col, int shift, int n
 template <int GEOM, int KPL, int RAD, typename ET, int POW, bool WGT, bool TAIL>
 __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *__restrict__ rec,
                                                 const ET *__restrict__ sino, const float *__restrict__ w, int vd,
-                                                int k0, int kend, int cap, float *col, int lane,
+                                                int k0, int kend, int cap, int unit, float *col, int lane,
                                                 float (&acc)[KPL]) {
     const int nc = __float_as_int(rec[0]);
     float Aw[2 * RAD + 1];
@@ -506,10 +520,22 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *_
         wave_sync();
         const int shift = RAD + mstart - ai;
         const float kr0 = (float)(k0 + lane - g.kref), krl = (float)(kend - 1 - g.kref);
-        if (fit)
-            back_vertical<KPL, RAD, POW, TAIL, true>(col, shift, n_rows, acd, Wd, hv, Lv, c0, c1, kr0, krl, acc);
-        else
-            back_vertical<KPL, RAD, POW, TAIL, false>(col, shift, n_rows, acd, Wd, hv, Lv, c0, c1, kr0, krl, acc);
+        const float Wf = rec[7];
+        constexpr bool CAN_UNIT = RAD == 1 && POW == 1;
+        if (CAN_UNIT && unit) {  // wave-uniform
+            if (fit)
+                back_vertical<KPL, RAD, POW, TAIL, true, CAN_UNIT>(col, shift, n_rows, acd, Wd, Wf, hv, Lv, c0, c1,
+                                                                   kr0, krl, acc);
+            else
+                back_vertical<KPL, RAD, POW, TAIL, false, CAN_UNIT>(col, shift, n_rows, acd, Wd, Wf, hv, Lv, c0, c1,
+                                                                    kr0, krl, acc);
+        } else if (fit) {
+            back_vertical<KPL, RAD, POW, TAIL, true, false>(col, shift, n_rows, acd, Wd, Wf, hv, Lv, c0, c1, kr0, krl,
+                                                            acc);
+        } else {
+            back_vertical<KPL, RAD, POW, TAIL, false, false>(col, shift, n_rows, acd, Wd, Wf, hv, Lv, c0, c1, kr0,
+                                                             krl, acc);
+        }
         wave_sync();
     } else {
 #pragma unroll
@@ -532,7 +558,7 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *_
 template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN, bool WGT, bool TAIL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void back_kernel(
     DevGeom g, const float *__restrict__ T, const ET *__restrict__ sino, const float *__restrict__ w, int npix,
-    const int32_t *__restrict__ views, int nviews, int nsplit, int chunk0, int nchunk, int cap, float scale,
+    const int32_t *__restrict__ views, int nviews, int nsplit, int chunk0, int nchunk, int cap, int unit, float scale,
     int accumulate, float *__restrict__ out, float *__restrict__ part) {
     constexpr int PPW = KAHAN ? 1 : 2;  // pixels per wave
     extern __shared__ float lds[];
@@ -563,9 +589,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
         const int vd = views ? views[vi] : vi;
         const float *recA = T + ((size_t)vi * npix + sA) * kRec;
         const float *recB = T + ((size_t)vi * npix + sB) * kRec;
-        back_pixel_view<GEOM, KPL, RAD, ET, POW, WGT, TAIL>(g, recA, sino, w, vd, k0, kend, cap, colA, lane, accA);
+        back_pixel_view<GEOM, KPL, RAD, ET, POW, WGT, TAIL>(g, recA, sino, w, vd, k0, kend, cap, unit, colA, lane,
+                                                            accA);
         if (hasB)
-            back_pixel_view<GEOM, KPL, RAD, ET, POW, WGT, TAIL>(g, recB, sino, w, vd, k0, kend, cap, colB, lane,
+            back_pixel_view<GEOM, KPL, RAD, ET, POW, WGT, TAIL>(g, recB, sino, w, vd, k0, kend, cap, unit, colB, lane,
                                                                 accB);
         if (KAHAN && ((vi - vb) % kKahanViews == kKahanViews - 1 || vi == ve - 1)) {  // wave-uniform
 #pragma unroll
@@ -623,8 +650,8 @@ static void launch_back_range(const DevGeom &g, const BackPlan &pl, const float 
     const int64_t blocks = groups * nch * pl.nsplit;
     const size_t lds = GEOM == MBX_GEOM_CONE ? 4 * PPW * (size_t)pl.cap * sizeof(float) : 0;
     hipLaunchKernelGGL((back_kernel<GEOM, KPL, RAD, ET, POW, KAHAN, WGT, TAIL>), dim3((unsigned)blocks), dim3(256), lds,
-                       st, g, T, sino, w, npix, views, nviews, pl.nsplit, chunk0, nch, pl.cap, scale, accumulate, out,
-                       part);
+                       st, g, T, sino, w, npix, views, nviews, pl.nsplit, chunk0, nch, pl.cap, pl.unit, scale,
+                       accumulate, out, part);
 }
 
 // Full slice chunks in one launch, a partial last chunk in a second one.
