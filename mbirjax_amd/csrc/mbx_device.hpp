@@ -108,6 +108,7 @@ __device__ __forceinline__ void store4_nt(bf16 *p, const float v[4]) {
 typedef int i32x4_t __attribute__((ext_vector_type(4)));
 __device__ float raw_load_f32(i32x4_t rsrc, int voffset, int soffset, int aux) __asm("llvm.amdgcn.raw.buffer.load.f32");
 __device__ short raw_load_i16(i32x4_t rsrc, int voffset, int soffset, int aux) __asm("llvm.amdgcn.raw.buffer.load.i16");
+__device__ mbx_f32x4 raw_load_f32x4(i32x4_t rsrc, int voffset, int soffset, int aux) __asm("llvm.amdgcn.raw.buffer.load.v4f32");
 
 __device__ __forceinline__ i32x4_t make_rsrc(const void *p) {
     const uint64_t a = (uint64_t)p;

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cstdio>
 #include <string>
 
@@ -312,7 +313,7 @@ BackPlan plan_back(const DevGeom &g, int npix, int nviews, int ppw) {
     // taps and margins), padded to whole 64-row chunks; at least the detector
     // height + 2 plus a chunk (the clamped fallback layout).
     const int round = kWave;
-    p.cap = g.R + 2 + round;
+    p.cap = g.R + 8 + round;  // the clamped layout: R + 2 rows, up to 3 more for 4-row alignment, a chunk
     p.unit = 0;
     if (g.kind == MBX_GEOM_CONE) {  // (translation: the clamped layout covers every magnification)
         const double rmax = g.dv * std::sqrt((double)g.img_row_center * g.img_row_center +
@@ -447,10 +448,14 @@ __device__ __forceinline__ void back_vertical(const float *col, int shift, int n
     }
 }
 
+// back_kernel flags
+constexpr int kBackUnit = 1;  // the unit-sum row weights apply (every W <= 2, radius-1 row taps)
+constexpr int kBackVec4 = 2;  // float32 sinogram, R % 4 == 0, 16-byte aligned: 16-byte column loads
+
 template <int GEOM, int KPL, int RAD, typename ET, int POW, bool WGT, bool TAIL>
 __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *__restrict__ rec,
                                                 const ET *__restrict__ sino, const float *__restrict__ w, int vd,
-                                                int k0, int kend, int cap, int unit, float *col, int lane,
+                                                int k0, int kend, int cap, int flags, float *col, int lane,
                                                 float (&acc)[KPL]) {
     const int nc = __float_as_int(rec[0]);
     float Aw[2 * RAD + 1];
@@ -480,15 +485,41 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *_
         // One zero row of margin on each side: a lane's slice position may
         // round to one row beyond the chunk's first or last centre row.
         const int count = mlast - mfirst + 3;
-        const bool fit = count + kWave <= cap;
-        const int mstart = fit ? mfirst - 1 : mlo - 1;
-        const int n_rows = fit ? count : mhi - mlo + 3;
-        // ---- detector column col[i] = row mstart + i, gathered in 64-row
-        // chunks (loads of up to kFill chunks in flight).  Rows are clamped
-        // to the detector for the loads; rows off it are zeroed afterwards.
+        const bool fit = count + kWave + 3 <= cap;
+        int mstart = fit ? mfirst - 1 : mlo - 1;
+        int n_rows = fit ? count : mhi - mlo + 3;
+        // 16-byte column loads (float32, R % 4 == 0, 16-byte aligned
+        // sinogram): the column starts on a multiple of 4 rows, so no 4-row
+        // group straddles row 0 or row R
+        constexpr bool VEC_OK = std::is_same<ET, float>::value && !WGT;
+        const bool v4 = VEC_OK && (flags & kBackVec4);
+        if (v4) {
+            n_rows += mstart & 3;
+            mstart &= ~3;
+        }
+        // ---- detector column col[i] = row mstart + i: whole 256-row groups
+        // 4 rows per lane (v4), the rest in 64-row chunks (loads of up to
+        // kFill chunks in flight).  Rows are clamped to the detector for the
+        // loads; rows off it are zeroed afterwards.
+        const int ng = v4 ? n_rows >> 8 : 0;
+        for (int gq = 0; gq < ng; ++gq) {
+            const int r = min(max(mstart + gq * 4 * kWave + 4 * lane, 0), g.R - 4);
+            mbx_f32x4 ev[2 * RAD + 1];
+#pragma unroll
+            for (int j = 0; j <= 2 * RAD; ++j) ev[j] = raw_load_f32x4(src.rs, r * 4, src.soff[j], 0);
+            mbx_f32x4 c;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float cq = 0.0f;
+#pragma unroll
+                for (int j = 0; j <= 2 * RAD; ++j) cq = fmaf(src.cv[j] ? Aw[j] : 0.0f, ev[j][q], cq);
+                c[q] = cq;
+            }
+            *(mbx_f32x4 *)(col + gq * 4 * kWave + 4 * lane) = c;  // 16-byte aligned
+        }
         constexpr int FILL = kBackFill;
         const int nch = (n_rows + kWave - 1) / kWave;
-        for (int c0 = 0; c0 < nch; c0 += FILL) {
+        for (int c0 = 4 * ng; c0 < nch; c0 += FILL) {
             float ev[FILL][2 * RAD + 1];
 #pragma unroll
             for (int u = 0; u < FILL; ++u) {
@@ -522,7 +553,7 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *_
         const float kr0 = (float)(k0 + lane - g.kref), krl = (float)(kend - 1 - g.kref);
         const float Wf = rec[7];
         constexpr bool CAN_UNIT = RAD == 1 && POW == 1;
-        if (CAN_UNIT && unit) {  // wave-uniform
+        if (CAN_UNIT && (flags & kBackUnit)) {  // wave-uniform
             if (fit)
                 back_vertical<KPL, RAD, POW, TAIL, true, CAN_UNIT>(col, shift, n_rows, acd, Wd, Wf, hv, Lv, c0, c1,
                                                                    kr0, krl, acc);
@@ -558,7 +589,7 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *_
 template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN, bool WGT, bool TAIL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void back_kernel(
     DevGeom g, const float *__restrict__ T, const ET *__restrict__ sino, const float *__restrict__ w, int npix,
-    const int32_t *__restrict__ views, int nviews, int nsplit, int chunk0, int nchunk, int cap, int unit, float scale,
+    const int32_t *__restrict__ views, int nviews, int nsplit, int chunk0, int nchunk, int cap, int flags, float scale,
     int accumulate, float *__restrict__ out, float *__restrict__ part) {
     constexpr int PPW = KAHAN ? 1 : 2;  // pixels per wave
     extern __shared__ float lds[];
@@ -589,10 +620,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
         const int vd = views ? views[vi] : vi;
         const float *recA = T + ((size_t)vi * npix + sA) * kRec;
         const float *recB = T + ((size_t)vi * npix + sB) * kRec;
-        back_pixel_view<GEOM, KPL, RAD, ET, POW, WGT, TAIL>(g, recA, sino, w, vd, k0, kend, cap, unit, colA, lane,
+        back_pixel_view<GEOM, KPL, RAD, ET, POW, WGT, TAIL>(g, recA, sino, w, vd, k0, kend, cap, flags, colA, lane,
                                                             accA);
         if (hasB)
-            back_pixel_view<GEOM, KPL, RAD, ET, POW, WGT, TAIL>(g, recB, sino, w, vd, k0, kend, cap, unit, colB, lane,
+            back_pixel_view<GEOM, KPL, RAD, ET, POW, WGT, TAIL>(g, recB, sino, w, vd, k0, kend, cap, flags, colB, lane,
                                                                 accB);
         if (KAHAN && ((vi - vb) % kKahanViews == kKahanViews - 1 || vi == ve - 1)) {  // wave-uniform
 #pragma unroll
@@ -649,8 +680,10 @@ static void launch_back_range(const DevGeom &g, const BackPlan &pl, const float 
     const int64_t groups = ((int64_t)npix + 4 * PPW - 1) / (4 * PPW);  // 4 waves per block
     const int64_t blocks = groups * nch * pl.nsplit;
     const size_t lds = GEOM == MBX_GEOM_CONE ? 4 * PPW * (size_t)pl.cap * sizeof(float) : 0;
+    const bool vec4 = g.R % 4 == 0 && ((uintptr_t)sino & 15) == 0;
+    const int flags = (pl.unit ? kBackUnit : 0) | (vec4 ? kBackVec4 : 0);
     hipLaunchKernelGGL((back_kernel<GEOM, KPL, RAD, ET, POW, KAHAN, WGT, TAIL>), dim3((unsigned)blocks), dim3(256), lds,
-                       st, g, T, sino, w, npix, views, nviews, pl.nsplit, chunk0, nch, pl.cap, pl.unit, scale,
+                       st, g, T, sino, w, npix, views, nviews, pl.nsplit, chunk0, nch, pl.cap, flags, scale,
                        accumulate, out, part);
 }
 
