@@ -40,8 +40,9 @@ inline bool dtype_ok(int dt) { return dt == MBX_DTYPE_F32 || dt == MBX_DTYPE_BF1
 // Per-(view, pixel) geometry table shared by both projectors.
 int table_chunk(const DevGeom &g, int npix, int nviews);  // pixels per table
 int64_t table_bytes(int chunk, int nviews);                // records + n_c array
+// Slot s of the table is pixel pix[s * pstride].
 int launch_table(const DevGeom &g, const float *angles, const int32_t *pix, int npix, const int32_t *views, int nviews,
-                 float *T, int *NC, hipStream_t st);
+                 float *T, int *NC, hipStream_t st, int pstride = 1);
 inline int *table_nc(float *T, int chunk, int nviews) {
     return (int *)((char *)T + (((int64_t)chunk * nviews * 16 * 4 + 255) & ~255ll));
 }
@@ -89,9 +90,10 @@ int launch_fwd(const DevGeom &g, const float *angles, const float *vox, const in
 int launch_back_tab(const DevGeom &g, const float *T, int npix, const void *sino, int sino_dtype,
                     const float *weights, const int32_t *views, int nviews, int coeff_power, int kahan, float scale,
                     int accumulate, float *out, float *part, hipStream_t st);
+// The cylinder of table slot s is vox + s * pstride * Nz.
 int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float *vox, int npix, int nviews, void *sino,
                    int sino_dtype, int layout, int accumulate, const float *weights, void *aux, int chunk,
-                   double *sumsq, hipStream_t st);
+                   double *sumsq, hipStream_t st, int pstride = 1);
 // Sum `n` doubles in fixed order into out[0] (one block).
 int launch_sum_partials(const double *partials, int64_t n, double *out, int accumulate, hipStream_t st);
 

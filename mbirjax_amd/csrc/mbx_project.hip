@@ -150,7 +150,7 @@ int64_t table_bytes(int chunk, int nviews) {
 
 template <int GEOM, int RAD>
 __global__ __launch_bounds__(256) void pv_table_kernel(DevGeom g, const float *__restrict__ angles,
-                                                       const int32_t *__restrict__ pix, int npix,
+                                                       const int32_t *__restrict__ pix, int npix, int pstride,
                                                        const int32_t *__restrict__ views, float *__restrict__ T,
                                                        int *__restrict__ NC) {
     const int vi = blockIdx.y;
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(256) void pv_table_kernel(DevGeom g, const float *_
     if (s >= npix) return;
     const mbx_geom &h = g.h;
     const int va = views ? views[vi] : vi;
-    const int p = pix[s];
+    const int p = pix[(size_t)s * pstride];
     const int i = p / g.cols, j = p - (p / g.cols) * g.cols;
     const double dv = h.delta_voxel, dc = h.delta_det_channel;
     // Wc: voxel width on the detector in channels; sc: channel weight per unit
@@ -258,30 +258,30 @@ __device__ __forceinline__ int rec_mlo(float w) { return __float_as_int(w) & 0xf
 __device__ __forceinline__ int rec_mhi(float w) { return __float_as_int(w) >> 16; }
 
 int launch_table(const DevGeom &g, const float *angles, const int32_t *pix, int npix, const int32_t *views, int nviews,
-                 float *T, int *NC, hipStream_t st) {
+                 float *T, int *NC, hipStream_t st, int pstride) {
     if (npix == 0 || nviews == 0) return MBX_OK;
     dim3 grid((npix + 255) / 256, nviews);
     if (g.kind == MBX_GEOM_TRANSLATION) {
         if (g.rh == 1)
             hipLaunchKernelGGL((pv_table_kernel<MBX_GEOM_TRANSLATION, 1>), grid, dim3(256), 0, st, g, angles, pix, npix,
-                               views, T, NC);
+                               pstride, views, T, NC);
         else
             hipLaunchKernelGGL((pv_table_kernel<MBX_GEOM_TRANSLATION, 2>), grid, dim3(256), 0, st, g, angles, pix, npix,
-                               views, T, NC);
+                               pstride, views, T, NC);
     } else if (g.kind == MBX_GEOM_CONE) {
         if (g.rh == 1)
-            hipLaunchKernelGGL((pv_table_kernel<MBX_GEOM_CONE, 1>), grid, dim3(256), 0, st, g, angles, pix, npix, views,
-                               T, NC);
+            hipLaunchKernelGGL((pv_table_kernel<MBX_GEOM_CONE, 1>), grid, dim3(256), 0, st, g, angles, pix, npix,
+                               pstride, views, T, NC);
         else
-            hipLaunchKernelGGL((pv_table_kernel<MBX_GEOM_CONE, 2>), grid, dim3(256), 0, st, g, angles, pix, npix, views,
-                               T, NC);
+            hipLaunchKernelGGL((pv_table_kernel<MBX_GEOM_CONE, 2>), grid, dim3(256), 0, st, g, angles, pix, npix,
+                               pstride, views, T, NC);
     } else {
         if (g.rh == 1)
             hipLaunchKernelGGL((pv_table_kernel<MBX_GEOM_PARALLEL, 1>), grid, dim3(256), 0, st, g, angles, pix, npix,
-                               views, T, NC);
+                               pstride, views, T, NC);
         else
             hipLaunchKernelGGL((pv_table_kernel<MBX_GEOM_PARALLEL, 2>), grid, dim3(256), 0, st, g, angles, pix, npix,
-                               views, T, NC);
+                               pstride, views, T, NC);
     }
     MBX_LAUNCH_CHECK();
     return MBX_OK;
@@ -928,73 +928,88 @@ __device__ __forceinline__ T block_sum(T v, T *scratch) {
 // Per view: the subset pixels sorted by centre channel n_c (bucket n_c + r),
 // ascending pixel slot within a bucket, the bucket starts, and the forward
 // records in that order.  Pixels whose taps miss the detector are dropped.
-// Counting sort with LDS counters, then every bucket is put in pixel order,
-// so the result (and the forward summation order) does not depend on atomic
-// ordering.
+// Stable counting sort: wave s (s < nseg) owns the contiguous slot range
+// [s*npix/nseg, (s+1)*npix/nseg); pass 1 counts its pixels per bucket, a scan
+// over (bucket, segment) gives every segment its cursor in each bucket, and
+// pass 2 walks the range 64 slots at a time in order, ranking the lanes of a
+// bucket by lane (ballots), so the order -- and the forward summation order --
+// is ascending slot within each bucket whatever the atomics' timing.
 constexpr int kBinThreads = 1024;
-constexpr size_t kBinLds = 64 * 1024;  // counters and, when they fit, the view's list
-// sort_in_lds: the list is sorted in LDS (after the counters) and then copied out.
+constexpr int kBinWaves = kBinThreads / 64;
+constexpr size_t kBinLds = 64 * 1024;  // counters [nseg][nbk] and, when it fits, the view's list
+// list_in_lds: the list is built in LDS (after the counters) and then copied out.
 __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const float *__restrict__ T,
                                                               const int *__restrict__ NC, int npix, int nbk,
-                                                              int vmode, int sort_in_lds, int *__restrict__ list,
-                                                              int *__restrict__ cstart, float *__restrict__ FR) {
-    extern __shared__ int cnt[];  // [nbk] (+ [npix])
+                                                              int vmode, int nseg, int list_in_lds,
+                                                              int *__restrict__ list, int *__restrict__ cstart,
+                                                              float *__restrict__ FR) {
+    extern __shared__ int cnt[];  // [nseg][nbk] (+ [npix])
     const int rad = g.rh;
     const int v = blockIdx.x, tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
     const int *ncv = NC + (size_t)v * npix;
     int *lg = list + (size_t)v * npix;
-    int *lv = sort_in_lds ? cnt + nbk : lg;
+    int *lv = list_in_lds ? cnt + (size_t)nseg * nbk : lg;
     int *cs = cstart + (size_t)v * (nbk + 1);
-    for (int b = tid; b < nbk; b += blockDim.x) cnt[b] = 0;
+    for (int i = tid; i < nseg * nbk; i += blockDim.x) cnt[i] = 0;
     __syncthreads();
-    for (int s = tid; s < npix; s += blockDim.x) {
-        const int b = ncv[s] + rad;
-        if (b >= 0 && b < nbk) atomicAdd(&cnt[b], 1);
+    const int sb = (int)((int64_t)npix * wave / nseg), se = (int)((int64_t)npix * (wave + 1) / nseg);
+    int *cw = cnt + (size_t)wave * nbk;  // this wave's counters (wave < nseg)
+    if (wave < nseg) {  // pass 1
+        for (int s = sb + lane; s < se; s += 64) {
+            const int b = ncv[s] + rad;
+            if (b >= 0 && b < nbk) atomicAdd(&cw[b], 1);
+        }
     }
     __syncthreads();
-    // exclusive scan: each thread owns a contiguous run of buckets
+    // exclusive scan over (bucket, segment): each thread owns a contiguous run of buckets
     const int per = (nbk + blockDim.x - 1) / blockDim.x;
     const int b0 = tid * per, b1 = min(nbk, b0 + per);
     int run = 0;
-    for (int b = b0; b < b1; ++b) run += cnt[b];
-    __shared__ int wsum[kBinThreads / 64];
+    for (int b = b0; b < b1; ++b)
+        for (int q = 0; q < nseg; ++q) run += cnt[q * nbk + b];
+    __shared__ int wsum[kBinWaves];
     __shared__ int total;
     int incl = run;
     for (int o = 1; o < 64; o <<= 1) {
         const int y = __shfl_up(incl, o, 64);
-        if ((tid & 63) >= o) incl += y;
+        if (lane >= o) incl += y;
     }
-    if ((tid & 63) == 63) wsum[tid >> 6] = incl;
+    if (lane == 63) wsum[wave] = incl;
     __syncthreads();
     int base = incl - run;
-    for (int w = 0; w < (tid >> 6); ++w) base += wsum[w];
+    for (int w = 0; w < wave; ++w) base += wsum[w];
     for (int b = b0; b < b1; ++b) {
-        const int c = cnt[b];
         cs[b] = base;
-        cnt[b] = base;  // placement cursor
-        base += c;
+        for (int q = 0; q < nseg; ++q) {  // segment q's cursor in bucket b
+            const int c = cnt[q * nbk + b];
+            cnt[q * nbk + b] = base;
+            base += c;
+        }
     }
     if (tid == blockDim.x - 1) {
         cs[nbk] = base;
         total = base;
     }
     __syncthreads();
-    for (int s = tid; s < npix; s += blockDim.x) {
-        const int b = ncv[s] + rad;
-        if (b >= 0 && b < nbk) lv[atomicAdd(&cnt[b], 1)] = s;
-    }
-    __syncthreads();
-    __threadfence_block();
-    for (int b = tid; b < nbk; b += blockDim.x) {  // insertion sort of each bucket
-        const int lo = b == 0 ? 0 : cnt[b - 1], hi = cnt[b];  // the cursors now hold the bucket ends
-        for (int i = lo + 1; i < hi; ++i) {
-            const int x = lv[i];
-            int j = i - 1;
-            while (j >= lo && lv[j] > x) {
-                lv[j + 1] = lv[j];
-                --j;
+    if (wave < nseg) {  // pass 2: place in slot order
+        for (int s0 = sb; s0 < se; s0 += 64) {
+            const int s = s0 + lane;
+            const int b = s < se ? ncv[s] + rad : -1;
+            const bool ok = b >= 0 && b < nbk;
+            uint64_t todo = __ballot(ok);
+            const uint64_t below = (1ull << lane) - 1;  // lanes before this one
+            int pos = 0;
+            while (todo) {  // one bucket of this round at a time
+                const int lead = __builtin_ctzll(todo);
+                const int bl = __builtin_amdgcn_readlane(b, lead);
+                const uint64_t same = __ballot(ok && b == bl);
+                const int cur = cw[bl];  // uniform: every lane reads the cursor
+                if (ok && b == bl) pos = cur + __builtin_popcountll(same & below);
+                if (lane == lead) cw[bl] = cur + __builtin_popcountll(same);
+                todo &= ~same;
             }
-            lv[j + 1] = x;
+            if (ok) lv[pos] = s;
         }
     }
     __syncthreads();
@@ -1003,7 +1018,7 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
     const int n = total;
     for (int i = tid; i < n; i += blockDim.x) {
         const int s = lv[i];
-        if (sort_in_lds) lg[i] = s;
+        if (list_in_lds) lg[i] = s;
         const float4 *r4 = (const float4 *)(T + ((size_t)v * npix + s) * kRec);
         const float4 R0 = r4[0], R1 = r4[1], R2 = r4[2], R3 = r4[3];
         float o[kFRec];
@@ -1094,8 +1109,8 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
 template <int GEOM, int RAD, int RB, int VM, typename OT, int LAYOUT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void fwd_kernel(
     DevGeom g, const float *__restrict__ T, const float *__restrict__ FR, const int *__restrict__ list,
-    const int *__restrict__ cstart, const float *__restrict__ vox, int npix, int ncb, int nrb, int nbatch,
-    int cstride, OT *__restrict__ sino, int accumulate, const float *__restrict__ w,
+    const int *__restrict__ cstart, const float *__restrict__ vox, int64_t vstride, int npix, int ncb, int nrb,
+    int nbatch, int cstride, OT *__restrict__ sino, int accumulate, const float *__restrict__ w,
     double *__restrict__ part_sumsq) {
     // (views, channels, rows) output: a sliding window of 2*RAD+1 channels per row
     constexpr bool SW = LAYOUT == MBX_LAYOUT_VCR;
@@ -1177,7 +1192,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             const int sc = __builtin_amdgcn_readlane(q.sc, slot);
             // slices past the cylinder end read 0 (num_records bounds the buffer)
             const __amdgpu_buffer_rsrc_t rs =
-                __builtin_amdgcn_make_buffer_rsrc((void *)(vox + (size_t)sc * g.Nz + kf), 0, (g.Nz - kf) * 4,
+                __builtin_amdgcn_make_buffer_rsrc((void *)(vox + sc * vstride + kf), 0, (g.Nz - kf) * 4,
                                                   0x00020000);
             float *dst = buf + (size_t)slot * cstride + P;  // 16-byte aligned
             // whole 256-slice blocks 16 bytes per lane, the rest 64 slices at a time
@@ -1496,30 +1511,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
 
 template <int GEOM, int RAD, int RB, int VM, typename OT, int LAYOUT>
 static void launch_fwd_t(const DevGeom &g, const FwdPlan &pl, const float *T, const float *FR, const int *list,
-                         const int *cstart, const float *vox, int npix, OT *sino, int accumulate, const float *w,
-                         double *part, hipStream_t st) {
+                         const int *cstart, const float *vox, int64_t vstride, int npix, OT *sino, int accumulate,
+                         const float *w, double *part, hipStream_t st) {
     hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, VM, OT, LAYOUT>), dim3((unsigned)pl.nblocks), dim3(64 * pl.nw),
-                       pl.lds, st, g, T, FR, list, cstart, vox, npix, pl.ncb, pl.nrb, pl.nb, pl.cstride, sino,
-                       accumulate, w, part);
+                       pl.lds, st, g, T, FR, list, cstart, vox, vstride, npix, pl.ncb, pl.nrb, pl.nb, pl.cstride,
+                       sino, accumulate, w, part);
 }
 
 template <int GEOM, int RAD, int RB, int VM>
 static void dispatch_fwd_2(const DevGeom &g, const FwdPlan &pl, const float *T, const float *FR, const int *list,
-                           const int *cstart, const float *vox, int npix, void *sino, int dt, int layout,
-                           int accumulate, const float *w, double *part, hipStream_t st) {
+                           const int *cstart, const float *vox, int64_t vstride, int npix, void *sino, int dt,
+                           int layout, int accumulate, const float *w, double *part, hipStream_t st) {
     if (dt == MBX_DTYPE_F32) {
         if (layout == MBX_LAYOUT_VCR)
-            launch_fwd_t<GEOM, RAD, RB, VM, float, MBX_LAYOUT_VCR>(g, pl, T, FR, list, cstart, vox, npix,
+            launch_fwd_t<GEOM, RAD, RB, VM, float, MBX_LAYOUT_VCR>(g, pl, T, FR, list, cstart, vox, vstride, npix,
                                                                    (float *)sino, accumulate, w, part, st);
         else
-            launch_fwd_t<GEOM, RAD, RB, VM, float, MBX_LAYOUT_VRC>(g, pl, T, FR, list, cstart, vox, npix,
+            launch_fwd_t<GEOM, RAD, RB, VM, float, MBX_LAYOUT_VRC>(g, pl, T, FR, list, cstart, vox, vstride, npix,
                                                                    (float *)sino, accumulate, w, part, st);
     } else {
         if (layout == MBX_LAYOUT_VCR)
-            launch_fwd_t<GEOM, RAD, RB, VM, bf16, MBX_LAYOUT_VCR>(g, pl, T, FR, list, cstart, vox, npix,
+            launch_fwd_t<GEOM, RAD, RB, VM, bf16, MBX_LAYOUT_VCR>(g, pl, T, FR, list, cstart, vox, vstride, npix,
                                                                   (bf16 *)sino, accumulate, w, part, st);
         else
-            launch_fwd_t<GEOM, RAD, RB, VM, bf16, MBX_LAYOUT_VRC>(g, pl, T, FR, list, cstart, vox, npix,
+            launch_fwd_t<GEOM, RAD, RB, VM, bf16, MBX_LAYOUT_VRC>(g, pl, T, FR, list, cstart, vox, vstride, npix,
                                                                   (bf16 *)sino, accumulate, w, part, st);
     }
 }
@@ -1528,7 +1543,8 @@ static void dispatch_fwd_2(const DevGeom &g, const FwdPlan &pl, const float *T, 
 // fwd_aux_bytes(g, chunk >= npix, nviews) bytes; sumsq may be NULL.
 int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float *vox, int npix, int nviews, void *sino,
                    int sino_dtype, int layout, int accumulate, const float *weights, void *aux, int chunk,
-                   double *sumsq, hipStream_t st) {
+                   double *sumsq, hipStream_t st, int pstride) {
+    const int64_t vstride = (int64_t)pstride * g.Nz;
     const FwdPlan pl = plan_fwd(g, nviews, layout);
     if (pl.lds > 160 * 1024)
         return fail(MBX_EUNSUPPORTED, "forward projection LDS footprint too large (num_recon_slices too big)");
@@ -1543,16 +1559,20 @@ int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float 
     float *FR = (float *)((char *)cstart + al256((size_t)nviews * (nbk + 1) * sizeof(int)));
     const int vm = fwd_vert_mode(g);
     if (nviews > 0) {
-        const bool in_lds = (size_t)(nbk + npix) * sizeof(int) <= kBinLds;
-        const size_t lds = (size_t)(nbk + (in_lds ? npix : 0)) * sizeof(int);
-        hipLaunchKernelGGL(fwd_bin_kernel, dim3(nviews), dim3(kBinThreads), lds, st, g, T, NC, npix, nbk, vm,
+        // as many pass segments as the counters fit, up to one per wave; the
+        // list in LDS too when it fits beside all sixteen
+        const int cap = (int)(kBinLds / sizeof(int));
+        const bool in_lds = (int64_t)kBinWaves * nbk + npix <= cap;
+        const int nseg = std::min(kBinWaves, (cap - (in_lds ? npix : 0)) / nbk);
+        const size_t lds = ((size_t)nseg * nbk + (in_lds ? npix : 0)) * sizeof(int);
+        hipLaunchKernelGGL(fwd_bin_kernel, dim3(nviews), dim3(kBinThreads), lds, st, g, T, NC, npix, nbk, vm, nseg,
                            (int)in_lds, list, cstart, FR);
         MBX_LAUNCH_CHECK();
     }
     double *p = sumsq ? part : nullptr;
 #define MBX_FWD(G, RA, RB_, VM_)                                                                                      \
-    dispatch_fwd_2<G, RA, RB_, VM_>(g, pl, T, FR, list, cstart, vox, npix, sino, sino_dtype, layout, accumulate,      \
-                                    weights, p, st)
+    dispatch_fwd_2<G, RA, RB_, VM_>(g, pl, T, FR, list, cstart, vox, vstride, npix, sino, sino_dtype, layout,        \
+                                    accumulate, weights, p, st)
     if (is_fan(g)) {
         if (vm == kVertTwo) {
             if (g.rh == 1)
@@ -1602,13 +1622,16 @@ int launch_fwd(const DevGeom &g, const float *angles, const float *vox, const in
         return launch_fwd_tab(g, T, NC, vox, 0, nviews, sino, sino_dtype, layout, accumulate, weights, aux, chunk,
                               sumsq, st);
     }
-    for (int c0 = 0; c0 < npix; c0 += chunk) {
-        const int n = std::min(chunk, npix - c0);
-        const bool last = c0 + n >= npix;
-        int rc = launch_table(g, angles, pix + c0, n, views, nviews, T, NC, st);
+    // Interleaved chunks (chunk c holds pixels c, c + nch, c + 2 nch, ...): a
+    // run of whole image rows would put most of a view's candidates into a
+    // few channel blocks, one chunk pixel spread does not.
+    const int nch = (npix + chunk - 1) / chunk;
+    for (int c = 0; c < nch; ++c) {
+        const int n = (npix - c + nch - 1) / nch;
+        int rc = launch_table(g, angles, pix + c, n, views, nviews, T, NC, st, nch);
         if (rc != MBX_OK) return rc;
-        rc = launch_fwd_tab(g, T, NC, vox + (size_t)c0 * g.Nz, n, nviews, sino, sino_dtype, layout,
-                            c0 == 0 ? accumulate : 1, weights, aux, chunk, last ? sumsq : nullptr, st);
+        rc = launch_fwd_tab(g, T, NC, vox + (size_t)c * g.Nz, n, nviews, sino, sino_dtype, layout,
+                            c == 0 ? accumulate : 1, weights, aux, chunk, c == nch - 1 ? sumsq : nullptr, st, nch);
         if (rc != MBX_OK) return rc;
     }
     return MBX_OK;

@@ -467,16 +467,18 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
         if (rc != MBX_OK) return rc;
     }
     // forward projection of delta over the subset (into dsino), optionally with forward_quadratic
+    // (interleaved chunks: chunk c holds pixels c, c + nch, ..., so that every
+    // chunk spreads over the image and the forward's channel blocks stay balanced)
     auto project_delta = [&](double *sumsq, const float *w) -> int {
-        for (int c0 = 0; c0 < npix; c0 += L.chunk) {
-            const int n = std::min(L.chunk, npix - c0);
-            const bool last = c0 + n >= npix;
+        const int nch = (npix + L.chunk - 1) / L.chunk;
+        for (int c = 0; c < nch; ++c) {
+            const int n = (npix - c + nch - 1) / nch;
             if (!one_table) {
-                int r = launch_table(d, angles, pix + c0, n, nullptr, d.V, T, NC, st);
+                int r = launch_table(d, angles, pix + c, n, nullptr, d.V, T, NC, st, nch);
                 if (r != MBX_OK) return r;
             }
-            int r = launch_fwd_tab(d, T, NC, delta + (size_t)c0 * d.Nz, n, d.V, dsino, delta_dtype, MBX_LAYOUT_VCR,
-                                   c0 > 0, w, fpart, L.chunk, last ? sumsq : nullptr, st);
+            int r = launch_fwd_tab(d, T, NC, delta + (size_t)c * d.Nz, n, d.V, dsino, delta_dtype, MBX_LAYOUT_VCR,
+                                   c > 0, w, fpart, L.chunk, c == nch - 1 ? sumsq : nullptr, st, nch);
             if (r != MBX_OK) return r;
         }
         return MBX_OK;
