@@ -136,6 +136,43 @@ def run_precision(model, y, part_np, precision, steps, warmup):
     return res
 
 
+def run_full(model, y, iterations=20, precision="fp32"):
+    """The default multi-granular schedule as recon() runs it (partitions K = 1,
+    4, 16, 64, then 128 for the remaining iterations, subsets in random order):
+    seconds for vcd_setup (Hessian diagonal, error sinogram) and for the
+    iterations, timed on the stream."""
+    import torch
+    import mbirjax_amd as mj
+    from mbirjax_amd import vcd_utils
+    recon_shape, granularity = model.get_params(['recon_shape', 'granularity'])
+    partitions = vcd_utils.gen_set_of_pixel_partitions(recon_shape, granularity)
+    seq = vcd_utils.gen_partition_sequence(model.get_params('partition_sequence'), iterations)
+    dev = y.device
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    state = model.vcd_setup(y, init_recon=0, precision=precision,
+                            max_subset_size=max(int(p.shape[1]) for p in partitions), channel_major=True)
+    parts = [torch.as_tensor(np.ascontiguousarray(p, dtype=np.int32), device=dev) for p in partitions]
+    ev[1].record(stream)
+    rng = np.random.RandomState(11)
+    for i in range(iterations):
+        part = parts[int(seq[i])]
+        state.iterate(part, rng.permutation(part.shape[0]))
+    ev[2].record(stream)
+    ev[2].synchronize()
+    t_setup = ev[0].elapsed_time(ev[1]) / 1e3
+    t_iter = ev[1].elapsed_time(ev[2]) / 1e3
+    res = dict(precision=precision, iterations=iterations, subsets_per_iteration=[int(parts[int(k)].shape[0])
+                                                                                  for k in seq[:iterations]],
+               setup_s=t_setup, iterations_s=t_iter, iter_per_s=iterations / t_iter,
+               iter_per_s_with_setup=iterations / (t_setup + t_iter), fm_rmse=state.forward_model_rmse())
+    del state, parts
+    torch.cuda.empty_cache()
+    return res
+
+
 def cpu_baseline(model, part_np, view_sample=72):
     """float32 oracle port (oracle/liboracle_f32.so, OpenMP over the host cores)
     on one K=128 subset, back + forward over a sample of views, scaled to one
@@ -199,6 +236,8 @@ def main():
     ap.add_argument("--precision", default="all", choices=["fp32", "mixed", "bf16", "all"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-full-run", action="store_true",
+                    help="skip the 20-iteration default-schedule run (K = 1, 4, 16, 64, 128, ...)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (views would shard across "
@@ -253,6 +292,9 @@ def main():
         "precisions": {k: {kk: vv for kk, vv in v.items() if kk not in ("b_back", "b_fwd")}
                        for k, v in results.items()},
     }
+    if not args.no_full_run and "fp32" in results:
+        line["full_run"] = run_full(model, y)
+        log("full_run", json.dumps(line["full_run"]))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(model, part)
     if args.copy_calibration:

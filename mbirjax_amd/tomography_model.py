@@ -441,7 +441,8 @@ class TomographyModel(ParameterHandler):
         if init_recon is None:
             self._log('info', 'Starting direct recon for initial reconstruction')
             init_recon = self.direct_recon(self.from_channel_major(y))
-        elif isinstance(init_recon, int) and init_recon == 0:
+        zero_init = isinstance(init_recon, int) and init_recon == 0
+        if zero_init:
             init_recon = torch.zeros(recon_shape, dtype=torch.float32, device=dev)
         init_recon = self._tensor(init_recon)
         if tuple(init_recon.shape) != tuple(recon_shape):
@@ -453,15 +454,17 @@ class TomographyModel(ParameterHandler):
         self._log('info', 'Initializing error sinogram')
         full = vcd_utils.gen_full_indices(recon_shape, use_ror_mask=self.use_ror_mask)
         full_d = torch.as_tensor(full, device=dev)
-        Ax = self._fwd(init_recon.reshape(-1, nz)[full_d.long()].contiguous(), full_d, layout=_lib.LAYOUT_VCR)
-        nrm = self._dot(Ax, None, w)
-        alpha0 = self._dot(Ax, y, w) / nrm if nrm > 0 else 1.0
         e_dtype = _BF16 if precision == 'bf16' else _F32
-        alpha_dev = torch.tensor([float(np.float32(alpha0))], dtype=torch.float64, device=dev)
         e32 = y.clone()
-        _lib.check(_lib.lib().mbx_sino_axpy(_lib.ptr(e32), _F32, _lib.ptr(Ax), _F32, _lib.ptr(alpha_dev),
-                                            int(e32.numel()), _lib.stream()))
-        del Ax
+        alpha0 = 1.0
+        if not zero_init:  # (A 0 = 0: alpha0 = 1 and e = y)
+            Ax = self._fwd(init_recon.reshape(-1, nz)[full_d.long()].contiguous(), full_d, layout=_lib.LAYOUT_VCR)
+            nrm = self._dot(Ax, None, w)
+            alpha0 = self._dot(Ax, y, w) / nrm if nrm > 0 else 1.0
+            alpha_dev = torch.tensor([float(np.float32(alpha0))], dtype=torch.float64, device=dev)
+            _lib.check(_lib.lib().mbx_sino_axpy(_lib.ptr(e32), _F32, _lib.ptr(Ax), _F32, _lib.ptr(alpha_dev),
+                                                int(e32.numel()), _lib.stream()))
+            del Ax
         if e_dtype == _BF16:
             e = torch.empty(e32.shape, dtype=torch.bfloat16, device=dev)
             _lib.check(_lib.lib().mbx_sino_transpose(_lib.ptr(e32), _F32, _lib.LAYOUT_VCR, _lib.ptr(e), _BF16,
