@@ -109,6 +109,25 @@ typedef int i32x4_t __attribute__((ext_vector_type(4)));
 __device__ float raw_load_f32(i32x4_t rsrc, int voffset, int soffset, int aux) __asm("llvm.amdgcn.raw.buffer.load.f32");
 __device__ short raw_load_i16(i32x4_t rsrc, int voffset, int soffset, int aux) __asm("llvm.amdgcn.raw.buffer.load.i16");
 __device__ mbx_f32x4 raw_load_f32x4(i32x4_t rsrc, int voffset, int soffset, int aux) __asm("llvm.amdgcn.raw.buffer.load.v4f32");
+__device__ mbx_u32x2 raw_load_u32x2(i32x4_t rsrc, int voffset, int soffset, int aux) __asm("llvm.amdgcn.raw.buffer.load.v2i32");
+
+// Four consecutive elements at a byte offset (float32: 16 bytes, bfloat16: 8 bytes).
+template <typename T>
+__device__ __forceinline__ mbx_f32x4 buf_load4(i32x4_t rsrc, int voffset, int soffset);
+template <>
+__device__ __forceinline__ mbx_f32x4 buf_load4<float>(i32x4_t rsrc, int voffset, int soffset) {
+    return raw_load_f32x4(rsrc, voffset, soffset, 0);
+}
+template <>
+__device__ __forceinline__ mbx_f32x4 buf_load4<bf16>(i32x4_t rsrc, int voffset, int soffset) {
+    const mbx_u32x2 u = raw_load_u32x2(rsrc, voffset, soffset, 0);
+    mbx_f32x4 r;
+    r.x = __uint_as_float(u.x << 16);
+    r.y = __uint_as_float(u.x & 0xffff0000u);
+    r.z = __uint_as_float(u.y << 16);
+    r.w = __uint_as_float(u.y & 0xffff0000u);
+    return r;
+}
 
 __device__ __forceinline__ i32x4_t make_rsrc(const void *p) {
     const uint64_t a = (uint64_t)p;

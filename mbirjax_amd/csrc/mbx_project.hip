@@ -450,7 +450,7 @@ __device__ __forceinline__ void back_vertical(const float *col, int shift, int n
 
 // back_kernel flags
 constexpr int kBackUnit = 1;  // the unit-sum row weights apply (every W <= 2, radius-1 row taps)
-constexpr int kBackVec4 = 2;  // float32 sinogram, R % 4 == 0, 16-byte aligned: 16-byte column loads
+constexpr int kBackVec4 = 2;  // R % 4 == 0, sinogram aligned to 4 elements: 4-row column loads
 
 template <int GEOM, int KPL, int RAD, typename ET, int POW, bool WGT, bool TAIL>
 __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *__restrict__ rec,
@@ -488,10 +488,10 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *_
         const bool fit = count + kWave + 3 <= cap;
         int mstart = fit ? mfirst - 1 : mlo - 1;
         int n_rows = fit ? count : mhi - mlo + 3;
-        // 16-byte column loads (float32, R % 4 == 0, 16-byte aligned
-        // sinogram): the column starts on a multiple of 4 rows, so no 4-row
-        // group straddles row 0 or row R
-        constexpr bool VEC_OK = std::is_same<ET, float>::value && !WGT;
+        // 4-row column loads (16 bytes float32, 8 bytes bfloat16; R % 4 == 0,
+        // aligned sinogram): the column starts on a multiple of 4 rows, so no
+        // 4-row group straddles row 0 or row R
+        constexpr bool VEC_OK = !WGT;
         const bool v4 = VEC_OK && (flags & kBackVec4);
         if (v4) {
             n_rows += mstart & 3;
@@ -506,7 +506,7 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *_
             const int r = min(max(mstart + gq * 4 * kWave + 4 * lane, 0), g.R - 4);
             mbx_f32x4 ev[2 * RAD + 1];
 #pragma unroll
-            for (int j = 0; j <= 2 * RAD; ++j) ev[j] = raw_load_f32x4(src.rs, r * 4, src.soff[j], 0);
+            for (int j = 0; j <= 2 * RAD; ++j) ev[j] = buf_load4<ET>(src.rs, r * (int)sizeof(ET), src.soff[j]);
             mbx_f32x4 c;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -680,7 +680,7 @@ static void launch_back_range(const DevGeom &g, const BackPlan &pl, const float 
     const int64_t groups = ((int64_t)npix + 4 * PPW - 1) / (4 * PPW);  // 4 waves per block
     const int64_t blocks = groups * nch * pl.nsplit;
     const size_t lds = GEOM == MBX_GEOM_CONE ? 4 * PPW * (size_t)pl.cap * sizeof(float) : 0;
-    const bool vec4 = g.R % 4 == 0 && ((uintptr_t)sino & 15) == 0;
+    const bool vec4 = g.R % 4 == 0 && ((uintptr_t)sino & (4 * sizeof(ET) - 1)) == 0;
     const int flags = (pl.unit ? kBackUnit : 0) | (vec4 ? kBackVec4 : 0);
     hipLaunchKernelGGL((back_kernel<GEOM, KPL, RAD, ET, POW, KAHAN, WGT, TAIL>), dim3((unsigned)blocks), dim3(256), lds,
                        st, g, T, sino, w, npix, views, nviews, pl.nsplit, chunk0, nch, pl.cap, flags, scale,

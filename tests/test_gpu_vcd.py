@@ -186,7 +186,7 @@ def test_c4_subproblem_mixed_and_fp32_vs_oracle(gpu):
     2 pi, 128 x 192 detector, same magnification (sdd 768, sid 384), 128^3
     recon; 10 iterations in each precision mode against the FP64 oracle under
     the same partitions and subset order.  'fp32' and 'mixed' (float32 error
-    sinogram, bfloat16 delta sinogram, Kahan back projection) meet NRMSE <=
+    sinogram, bfloat16 delta sinogram) meet NRMSE <=
     1e-3.  'bf16' also stores the error sinogram in bfloat16; late updates
     smaller than half a bfloat16 ulp of the residual are rounded away, which
     leaves it at ~3.4e-3 of the oracle (its error against the phantom is the
