@@ -35,14 +35,9 @@ __device__ __forceinline__ float from_float<float>(float x) {
 }
 template <>
 __device__ __forceinline__ bf16 from_float<bf16>(float x) {
-    uint32_t u = __float_as_uint(x);
+    // round to nearest even (NaN stays a quiet NaN): one v_cvt_pk_bf16_f32 on gfx950
     bf16 r;
-    if ((u & 0x7fffffffu) > 0x7f800000u) {  // NaN: keep it quiet
-        r.bits = (uint16_t)((u >> 16) | 0x40u);
-        return r;
-    }
-    u += 0x7fffu + ((u >> 16) & 1u);
-    r.bits = (uint16_t)(u >> 16);
+    r.bits = __builtin_bit_cast(uint16_t, (__bf16)x);
     return r;
 }
 
