@@ -189,8 +189,10 @@ def test_hessian_matches_impulse(gpu):
     assert rel(H, Href) <= TOL
 
 
-def test_bf16_and_kahan_back_projection(gpu):
-    m, g = CASES["cone_m2"]()
+@pytest.mark.parametrize("case", ["cone_m2", "cone_multichunk"])  # (1300 rows: whole 256-row column groups)
+def test_bf16_and_kahan_back_projection(gpu, case):
+    print("case", case, flush=True)
+    m, g = CASES[case]()
     rows, cols, nz = m.get_params('recon_shape')
     pix = torch.arange(rows * cols, dtype=torch.int32, device='cuda')
     rng = np.random.default_rng(14)
