@@ -240,8 +240,9 @@ def main():
                     help="skip the 20-iteration default-schedule run (K = 1, 4, 16, 64, 128, ...)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (views would shard across "
-              "GPUs with one all-reduce per subset; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path (views would shard across "
+              "GPUs with one all-reduce of the subset's back projection per subset). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     torch.manual_seed(0)
