@@ -213,18 +213,23 @@ def copy_calibration(nbytes=1 << 30):
     return dict(bytes=2 * nbytes, GBps=2 * nbytes / t / 1e9)
 
 
-def profiled_traffic(kernel, workload, precision):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
-    (FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy), or None."""
+def profiled_value(key, kernel, workload, precision):
+    """A per-launch figure of `kernel` from the committed rocprofv3 summaries
+    (a line '<key> <kernel> <workload> <precision>: <value>'), or None."""
     import glob
     import re
-    pat = re.compile(r"traffic_bytes_per_launch %s %s %s: ([0-9.eE+]+)" % (kernel, workload, precision))
+    pat = re.compile(r"%s %s %s %s: ([0-9.eE+]+)" % (key, kernel, workload, precision))
     for f in sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "*.md"))):
         with open(f) as fh:
             m = pat.search(fh.read())
         if m:
             return float(m.group(1)), os.path.relpath(f, os.path.dirname(os.path.abspath(__file__)))
     return None, None
+
+
+def profiled_traffic(kernel, workload, precision):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy), or None."""
+    return profiled_value("traffic_bytes_per_launch", kernel, workload, precision)
 
 
 def main():
@@ -288,7 +293,11 @@ def main():
                          "forward" if dominant == "fwd" else "back",
                          " + binning" if dominant == "fwd" else "", kname)),
                      "bytes_per_launch": b_dom, "avg_launch_ms": t_dom * 1e3,
-                     "traffic_source": traffic_src},
+                     "traffic_source": traffic_src,
+                     # the bound the kernel actually meets: non-packed FP32 issues one
+                     # wave-instruction per CU-cycle (SQ_INSTS_VALU / (CUs x cycles))
+                     "valu_issue_per_cu_cycle": profiled_value("valu_issue_per_cu_cycle", kname, args.workload,
+                                                               main_p)[0]},
         "projector_GBps": r["proj_GBps"],
         "precisions": {k: {kk: vv for kk, vv in v.items() if kk not in ("b_back", "b_fwd")}
                        for k, v in results.items()},
