@@ -142,7 +142,6 @@ def run_full(model, y, iterations=20, precision="fp32"):
     seconds for vcd_setup (Hessian diagonal, error sinogram) and for the
     iterations, timed on the stream."""
     import torch
-    import mbirjax_amd as mj
     from mbirjax_amd import vcd_utils
     recon_shape, granularity = model.get_params(['recon_shape', 'granularity'])
     partitions = vcd_utils.gen_set_of_pixel_partitions(recon_shape, granularity)
