@@ -931,25 +931,24 @@ __device__ __forceinline__ T block_sum(T v, T *scratch) {
 // Stable counting sort: wave s (s < nseg) owns the contiguous slot range
 // [s*npix/nseg, (s+1)*npix/nseg); pass 1 counts its pixels per bucket, a scan
 // over (bucket, segment) gives every segment its cursor in each bucket, and
-// pass 2 walks the range 64 slots at a time in order, ranking the lanes of a
-// bucket by lane (ballots), so the order -- and the forward summation order --
-// is ascending slot within each bucket whatever the atomics' timing.
+// pass 2 walks the range 64 slots at a time in order, sorting the round's
+// (bucket, lane) keys across the wave and ranking each slot within its bucket,
+// so the order -- and the forward summation order -- is ascending slot within
+// each bucket whatever the atomics' timing; each lane then writes one slot's
+// list entry and forward record at its position.
 constexpr int kBinThreads = 1024;
 constexpr int kBinWaves = kBinThreads / 64;
-constexpr size_t kBinLds = 64 * 1024;  // counters [nseg][nbk] and, when it fits, the view's list
-// list_in_lds: the list is built in LDS (after the counters) and then copied out.
+constexpr size_t kBinLds = 64 * 1024;  // counters [nseg][nbk]
 __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const float *__restrict__ T,
                                                               const int *__restrict__ NC, int npix, int nbk,
-                                                              int vmode, int nseg, int list_in_lds,
-                                                              int *__restrict__ list, int *__restrict__ cstart,
-                                                              float *__restrict__ FR) {
-    extern __shared__ int cnt[];  // [nseg][nbk] (+ [npix])
+                                                              int vmode, int nseg, int *__restrict__ list,
+                                                              int *__restrict__ cstart, float *__restrict__ FR) {
+    extern __shared__ int cnt[];  // [nseg][nbk]
     const int rad = g.rh;
     const int v = blockIdx.x, tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int *ncv = NC + (size_t)v * npix;
     int *lg = list + (size_t)v * npix;
-    int *lv = list_in_lds ? cnt + (size_t)nseg * nbk : lg;
     int *cs = cstart + (size_t)v * (nbk + 1);
     for (int i = tid; i < nseg * nbk; i += blockDim.x) cnt[i] = 0;
     __syncthreads();
@@ -969,7 +968,6 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
     for (int b = b0; b < b1; ++b)
         for (int q = 0; q < nseg; ++q) run += cnt[q * nbk + b];
     __shared__ int wsum[kBinWaves];
-    __shared__ int total;
     int incl = run;
     for (int o = 1; o < 64; o <<= 1) {
         const int y = __shfl_up(incl, o, 64);
@@ -987,38 +985,10 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
             base += c;
         }
     }
-    if (tid == blockDim.x - 1) {
-        cs[nbk] = base;
-        total = base;
-    }
+    if (tid == blockDim.x - 1) cs[nbk] = base;
     __syncthreads();
-    if (wave < nseg) {  // pass 2: place in slot order
-        for (int s0 = sb; s0 < se; s0 += 64) {
-            const int s = s0 + lane;
-            const int b = s < se ? ncv[s] + rad : -1;
-            const bool ok = b >= 0 && b < nbk;
-            uint64_t todo = __ballot(ok);
-            const uint64_t below = (1ull << lane) - 1;  // lanes before this one
-            int pos = 0;
-            while (todo) {  // one bucket of this round at a time
-                const int lead = __builtin_ctzll(todo);
-                const int bl = __builtin_amdgcn_readlane(b, lead);
-                const uint64_t same = __ballot(ok && b == bl);
-                const int cur = cw[bl];  // uniform: every lane reads the cursor
-                if (ok && b == bl) pos = cur + __builtin_popcountll(same & below);
-                if (lane == lead) cw[bl] = cur + __builtin_popcountll(same);
-                todo &= ~same;
-            }
-            if (ok) lv[pos] = s;
-        }
-    }
-    __syncthreads();
-    __threadfence_block();
-    // forward records in binned order
-    const int n = total;
-    for (int i = tid; i < n; i += blockDim.x) {
-        const int s = lv[i];
-        if (list_in_lds) lg[i] = s;
+    // forward record of slot s at list position i
+    auto write_record = [&](int s, int i) {
         const float4 *r4 = (const float4 *)(T + ((size_t)v * npix + s) * kRec);
         const float4 R0 = r4[0], R1 = r4[1], R2 = r4[2], R3 = r4[3];
         float o[kFRec];
@@ -1086,6 +1056,45 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
         float4 *dst = (float4 *)(FR + ((size_t)v * npix + i) * kFRec);
 #pragma unroll
         for (int q = 0; q < 4; ++q) dst[q] = make_float4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    };
+    if (wave < nseg) {  // pass 2: place in slot order; each lane writes one slot's record
+        for (int s0 = sb; s0 < se; s0 += 64) {
+            const int s = s0 + lane;
+            const int b = s < se ? ncv[s] + rad : -1;
+            const bool ok = b >= 0 && b < nbk;
+            // sort the round's (bucket, lane) keys across the wave (bitonic
+            // network over lane shuffles): equal buckets become adjacent, in lane
+            // (= slot) order; slots off the detector sort last
+            int key = ((ok ? b : nbk) << 6) | lane;
+#pragma unroll
+            for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    const int other = __shfl_xor(key, j, 64);
+                    const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+                    key = keep_min ? min(key, other) : max(key, other);
+                }
+            }
+            const int kb = key >> 6;
+            const int ks = s0 + (key & 63);
+            // rank within the bucket: lanes since the bucket's first lane (max-scan of heads)
+            const int prev = __shfl_up(kb, 1, 64);
+            int first = (lane == 0 || prev != kb) ? lane : 0;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int t = __shfl_up(first, d, 64);
+                if (lane >= d) first = max(first, t);
+            }
+            const int next = __shfl_down(kb, 1, 64);
+            const bool last = lane == 63 || next != kb;
+            if (kb < nbk) {
+                const int cur = cw[kb];  // read by every lane of the bucket before its last lane moves it
+                const int pos = cur + lane - first;
+                if (last) cw[kb] = pos + 1;
+                lg[pos] = ks;
+                write_record(ks, pos);
+            }
+        }
     }
 }
 
@@ -1559,14 +1568,11 @@ int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float 
     float *FR = (float *)((char *)cstart + al256((size_t)nviews * (nbk + 1) * sizeof(int)));
     const int vm = fwd_vert_mode(g);
     if (nviews > 0) {
-        // as many pass segments as the counters fit, up to one per wave; the
-        // list in LDS too when it fits beside all sixteen
-        const int cap = (int)(kBinLds / sizeof(int));
-        const bool in_lds = (int64_t)kBinWaves * nbk + npix <= cap;
-        const int nseg = std::min(kBinWaves, (cap - (in_lds ? npix : 0)) / nbk);
-        const size_t lds = ((size_t)nseg * nbk + (in_lds ? npix : 0)) * sizeof(int);
+        // as many pass segments as the counters fit, up to one per wave
+        const int nseg = std::min<int>(kBinWaves, (int)(kBinLds / sizeof(int)) / nbk);
+        const size_t lds = (size_t)nseg * nbk * sizeof(int);
         hipLaunchKernelGGL(fwd_bin_kernel, dim3(nviews), dim3(kBinThreads), lds, st, g, T, NC, npix, nbk, vm, nseg,
-                           (int)in_lds, list, cstart, FR);
+                           list, cstart, FR);
         MBX_LAUNCH_CHECK();
     }
     double *p = sumsq ? part : nullptr;
