@@ -934,8 +934,9 @@ __device__ __forceinline__ T block_sum(T v, T *scratch) {
 // pass 2 walks the range 64 slots at a time in order, sorting the round's
 // (bucket, lane) keys across the wave and ranking each slot within its bucket,
 // so the order -- and the forward summation order -- is ascending slot within
-// each bucket whatever the atomics' timing; each lane then writes one slot's
-// list entry and forward record at its position.
+// each bucket whatever the atomics' timing; the position goes back to the
+// slot's own lane, which loaded the slot's table record while the round was
+// sorted and writes its forward record there.
 constexpr int kBinThreads = 1024;
 constexpr int kBinWaves = kBinThreads / 64;
 constexpr size_t kBinLds = 64 * 1024;  // counters [nseg][nbk]
@@ -988,9 +989,7 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
     if (tid == blockDim.x - 1) cs[nbk] = base;
     __syncthreads();
     // forward record of slot s at list position i
-    auto write_record = [&](int s, int i) {
-        const float4 *r4 = (const float4 *)(T + ((size_t)v * npix + s) * kRec);
-        const float4 R0 = r4[0], R1 = r4[1], R2 = r4[2], R3 = r4[3];
+    auto write_record = [&](int i, const float4 &R0, const float4 &R1, const float4 &R2, const float4 &R3) {
         float o[kFRec];
         o[0] = R0.z;
         o[1] = R0.w;
@@ -1060,7 +1059,11 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
     if (wave < nseg) {  // pass 2: place in slot order; each lane writes one slot's record
         for (int s0 = sb; s0 < se; s0 += 64) {
             const int s = s0 + lane;
-            const int b = s < se ? ncv[s] + rad : -1;
+            const int sc = min(s, se - 1);  // (lanes past the range load a valid slot, unused)
+            // this lane's slot: its table record is loaded now, while the round is sorted
+            const float4 *r4 = (const float4 *)(T + ((size_t)v * npix + sc) * kRec);
+            const float4 R0 = r4[0], R1 = r4[1], R2 = r4[2], R3 = r4[3];
+            const int b = s < se ? ncv[sc] + rad : -1;
             const bool ok = b >= 0 && b < nbk;
             // sort the round's (bucket, lane) keys across the wave (bitonic
             // network over lane shuffles): equal buckets become adjacent, in lane
@@ -1087,13 +1090,16 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
             }
             const int next = __shfl_down(kb, 1, 64);
             const bool last = lane == 63 || next != kb;
+            int pos = 0;
             if (kb < nbk) {
                 const int cur = cw[kb];  // read by every lane of the bucket before its last lane moves it
-                const int pos = cur + lane - first;
+                pos = cur + lane - first;
                 if (last) cw[kb] = pos + 1;
                 lg[pos] = ks;
-                write_record(ks, pos);
             }
+            // back to the slot's own lane (ds_permute: lane j sends to lane key & 63)
+            const int mypos = __builtin_amdgcn_ds_permute((key & 63) << 2, pos);
+            if (ok) write_record(mypos, R0, R1, R2, R3);
         }
     }
 }
