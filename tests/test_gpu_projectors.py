@@ -77,6 +77,12 @@ CASES = {
     "translation_grid_offsets": lambda: make_trans(20, 24, 40, 160.0, 80.0, (8, 44, 30), 1.0,
                                                    tv=mj.utilities.gen_translation_vectors(5, 4, 4.0, 3.0),
                                                    det_row_offset=0.8, det_channel_offset=-1.1),
+    # tap radii 2: a five-channel horizontal window (large voxels), a five-slice
+    # vertical window in the general mode (small voxels)
+    "cone_psf2": lambda: make_cone(12, 16, 24, 96.0, 48.0, delta_voxel=1.0, recon_shape=(8, 8, 6)),
+    "cone_bp2": lambda: make_cone(12, 24, 24, 96.0, 48.0, delta_voxel=0.2, recon_shape=(20, 20, 40)),
+    "parallel_psf2": lambda: make_par(10, 8, 20, delta_voxel=2.5, recon_shape=(8, 8, 8)),
+    "translation_psf2": lambda: make_trans(12, 20, 36, 144.0, 72.0, (6, 30, 24), 1.5, delta_voxel=1.2),
 }
 
 
