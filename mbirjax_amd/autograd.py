@@ -49,6 +49,11 @@ def _layout_code(layout):
     raise ValueError("layout must be 'vrc' (views, rows, channels) or 'vcr' (views, channels, rows)")
 
 
+def _project(model, vox, pix, layout):
+    code = _layout_code(layout)
+    return model._fwd_rows(vox, pix) if code == _lib.LAYOUT_VRC else model._fwd(vox, pix, layout=code)
+
+
 class ForwardProjector(torch.autograd.Function):
     """y = A x for cylinders x (npix, Nz) at pixels pix; backward g -> A^T g."""
 
@@ -56,7 +61,7 @@ class ForwardProjector(torch.autograd.Function):
     def forward(ctx, vox, model, pix, layout):
         ctx.model, ctx.layout = model, layout
         ctx.save_for_backward(pix)
-        return model._fwd(vox.detach().to(torch.float32).contiguous(), pix, layout=_layout_code(layout))
+        return _project(model, vox.detach().to(torch.float32).contiguous(), pix, layout)
 
     @staticmethod
     def backward(ctx, g):
@@ -83,7 +88,7 @@ class BackProjector(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g):
         (pix,) = ctx.saved_tensors
-        out = ctx.model._fwd(g.detach().to(torch.float32).contiguous(), pix, layout=_layout_code(ctx.layout))
+        out = _project(ctx.model, g.detach().to(torch.float32).contiguous(), pix, ctx.layout)
         return out, None, None, None
 
 

@@ -197,6 +197,12 @@ class TomographyModel(ParameterHandler):
             _lib.ptr(ws), wsb, _lib.stream()))
         return (out, sumsq) if want_sumsq else out
 
+    def _fwd_rows(self, vox, pix, views=None):
+        """(V', R, C) float32 projection: the channel-major forward, then one
+        transpose (bitwise the same sums; the channel-major kernel's sliding
+        window beats the row-major register tile by more than the transpose costs)."""
+        return self.from_channel_major(self._fwd(vox, pix, views, layout=_lib.LAYOUT_VCR))
+
     def _back(self, sino_vcr, pix, views=None, coeff_power=1, weights=None, kahan=False, scale=1.0, out=None,
               accumulate=False):
         """Back project a channel-major sinogram (f32 or bf16) to cylinders (npix, Nz)."""
@@ -234,7 +240,7 @@ class TomographyModel(ParameterHandler):
             raise ValueError('voxel_values must have shape[0:2] = (num_indices, num_slices)')
         V = self.get_params('sinogram_shape')[0]
         views = None if view_indices is None else self._indices(view_indices, V, 'view_indices')
-        return self._output(self._fwd(vox, pix, views), output_device)
+        return self._output(self._fwd_rows(vox, pix, views), output_device)
 
     def sparse_back_project(self, sinogram, pixel_indices, view_indices=None, coeff_power=1, output_device=None):
         """(npix, Nz) back projection; view_indices select rows of the passed sinogram
