@@ -1104,23 +1104,26 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
     }
 }
 
-// One workgroup per (view, 16-channel block, row block); lane l of wave w owns
-// detector rows m0 + 128w + l and m0 + 128w + 64 + l and keeps their 16
-// channels in registers.  The block's candidate pixels are a contiguous run
-// of the view's binned records (sorted by centre channel, then pixel) and are
-// handled in batches, double-buffered in LDS:
-//   staging: the slices of each candidate's cylinder that reach the detector
-//            are copied global -> LDS asynchronously (buffer loads into LDS)
-//            while the previous batch is being used, then scaled in place by
-//            1/cos(phi_k) with zero pads;
-//   update:  for each centre channel (a compile-time loop, so every tap lands
-//            in a fixed register) every wave resamples that channel's staged
-//            cylinders onto its rows (row-driven vertical fan,
-//            mbirjax/cone_beam.py:347-372), weights them with the horizontal
-//            taps (mbirjax/cone_beam.py:276-282) and adds them to its tile.
-// Each output element has one writer and a fixed summation order; the tile is
-// written to HBM once.  At most four waves per SIMD are resident (the LDS of
-// four blocks per CU).
+// One workgroup per (view, channel block, row block).  The block's candidate
+// pixels are a contiguous run of the view's binned records (sorted by centre
+// channel, then pixel) and are handled in batches, double-buffered in LDS:
+//   staging: the slices of each candidate's cylinder that reach the block's
+//            rows are copied global -> LDS asynchronously (buffer loads into
+//            LDS) while the previous batch is being used, then scaled in place
+//            by 1/cos(phi_k) with zero pads;
+//   update:  every wave resamples the staged cylinders onto its rows
+//            (row-driven vertical fan, mbirjax/cone_beam.py:347-372) and weights
+//            them with the horizontal taps (mbirjax/cone_beam.py:276-282).
+// Channel-major output (views, channels, rows): 32-channel blocks of 2 waves,
+// lane l of wave w owning rows m0 + 256w + l + 64r (r < 4); the candidates are
+// walked in list order and a sliding window of 2*RAD+1 channel accumulators
+// per row follows their buckets, streaming each channel out as soon as it is
+// complete.  Row-major output (views, rows, channels): 16-channel blocks of 4
+// waves, lane l of wave w owning rows m0 + 128w + l and + 64, with the 16
+// channels in registers (a compile-time loop over the centre-channel buckets
+// puts every tap in a fixed register) and the tile written once.
+// Each output element has one writer and a fixed summation order (the list
+// order of its candidates, the same in both schemes).
 template <int GEOM, int RAD, int RB, int VM, typename OT, int LAYOUT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void fwd_kernel(
     DevGeom g, const float *__restrict__ T, const float *__restrict__ FR, const int *__restrict__ list,
