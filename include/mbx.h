@@ -139,6 +139,24 @@ int mbx_forward_project(const mbx_geom *g, const float *angles, const float *vox
                         int32_t sino_layout, int32_t accumulate, const float *weights, double *sumsq,
                         void *workspace, int64_t workspace_bytes, void *stream);
 
+/* The forward projection in two steps, for repeated projections of one pixel
+ * set (no counterpart in mbirjax, whose projector recomputes its geometry on
+ * every call; for callers that apply A to the same subset many times, such as
+ * unrolled or autograd loops, and for timing the projection kernel alone).
+ * mbx_forward_plan builds the geometry table, the centre-channel bins and the
+ * forward records of (pix, views) into plan, which holds
+ * mbx_forward_workspace_bytes(g, npix, nviews) bytes; the pixel set must fit
+ * one geometry-table chunk (about 512 MB of table: 10965 pixels at 720 views),
+ * else MBX_EUNSUPPORTED.  mbx_forward_apply then projects cylinders vox of
+ * those pixels exactly as mbx_forward_project would (same arguments; views
+ * only fixes the view count).  The plan stays valid while the geometry,
+ * angles, pix and views are unchanged; applies on one stream may share it. */
+int mbx_forward_plan(const mbx_geom *g, const float *angles, const int32_t *pix, int32_t npix, const int32_t *views,
+                     int32_t nviews, void *plan, int64_t plan_bytes, void *stream);
+int mbx_forward_apply(const mbx_geom *g, const void *plan, const float *vox, int32_t npix, const int32_t *views,
+                      int32_t nviews, void *sino, int32_t sino_dtype, int32_t sino_layout, int32_t accumulate,
+                      const float *weights, double *sumsq, void *stream);
+
 /* Sparse back projection (mbirjax/projectors.py:131-209, tomography_model.py:624-677).
  * sino: (num_sino_views, C, R) in sino_dtype, MBX_LAYOUT_VCR.  views: NULL =
  * views 0..num_sino_views-1 (then num_sino_views must equal g->num_views), else

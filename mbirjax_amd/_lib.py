@@ -30,7 +30,8 @@ LAYOUT_VRC = 1
 # Every exported symbol of include/mbx.h.
 EXPORTS = (
     "mbx_version", "mbx_last_error", "mbx_back_workspace_bytes", "mbx_forward_workspace_bytes",
-    "mbx_forward_project", "mbx_back_project", "mbx_sino_transpose", "mbx_qggmrf_grad_hess",
+    "mbx_forward_project", "mbx_forward_plan", "mbx_forward_apply", "mbx_back_project", "mbx_sino_transpose",
+    "mbx_qggmrf_grad_hess",
     "mbx_dot_workspace_bytes", "mbx_dot", "mbx_sino_axpy", "mbx_vcd_workspace_bytes", "mbx_vcd_subset_update",
     "mbx_denoise_workspace_bytes", "mbx_denoise_subset_update", "mbx_median_filter3d",
 )
@@ -87,6 +88,8 @@ def lib():
             "mbx_back_workspace_bytes": (i64, [G, i32, i32]),
             "mbx_forward_workspace_bytes": (i64, [G, i32, i32]),
             "mbx_forward_project": (i32, [G, vp, vp, vp, i32, vp, i32, vp, i32, i32, i32, vp, vp, vp, i64, vp]),
+            "mbx_forward_plan": (i32, [G, vp, vp, i32, vp, i32, vp, i64, vp]),
+            "mbx_forward_apply": (i32, [G, vp, vp, i32, vp, i32, vp, i32, i32, i32, vp, vp, vp]),
             "mbx_back_project": (i32, [G, vp, vp, i32, i32, vp, vp, i32, vp, i32, i32, i32, f32, i32, vp, vp, i64,
                                        vp]),
             "mbx_sino_transpose": (i32, [vp, i32, i32, vp, i32, i32, i32, i32, i32, vp]),

@@ -90,10 +90,13 @@ int launch_fwd(const DevGeom &g, const float *angles, const float *vox, const in
 int launch_back_tab(const DevGeom &g, const float *T, int npix, const void *sino, int sino_dtype,
                     const float *weights, const int32_t *views, int nviews, int coeff_power, int kahan, float scale,
                     int accumulate, float *out, float *part, hipStream_t st);
-// The cylinder of table slot s is vox + s * pstride * Nz.
+// The cylinder of table slot s is vox + s * pstride * Nz.  steps: kFwdBin
+// (bins and forward records from the table), kFwdProject (the projection
+// from bins and records already in aux), or both.
+constexpr int kFwdBin = 1, kFwdProject = 2;
 int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float *vox, int npix, int nviews, void *sino,
                    int sino_dtype, int layout, int accumulate, const float *weights, void *aux, int chunk,
-                   double *sumsq, hipStream_t st, int pstride = 1);
+                   double *sumsq, hipStream_t st, int pstride = 1, int steps = kFwdBin | kFwdProject);
 // Sum `n` doubles in fixed order into out[0] (one block).
 int launch_sum_partials(const double *partials, int64_t n, double *out, int accumulate, hipStream_t st);
 

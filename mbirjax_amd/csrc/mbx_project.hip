@@ -1561,7 +1561,7 @@ static void dispatch_fwd_2(const DevGeom &g, const FwdPlan &pl, const float *T, 
 // fwd_aux_bytes(g, chunk >= npix, nviews) bytes; sumsq may be NULL.
 int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float *vox, int npix, int nviews, void *sino,
                    int sino_dtype, int layout, int accumulate, const float *weights, void *aux, int chunk,
-                   double *sumsq, hipStream_t st, int pstride) {
+                   double *sumsq, hipStream_t st, int pstride, int steps) {
     const int64_t vstride = (int64_t)pstride * g.Nz;
     const FwdPlan pl = plan_fwd(g, nviews, layout);
     if (pl.lds > 160 * 1024)
@@ -1576,7 +1576,7 @@ int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float 
     int *cstart = (int *)((char *)list + al256((size_t)nviews * chunk * sizeof(int)));
     float *FR = (float *)((char *)cstart + al256((size_t)nviews * (nbk + 1) * sizeof(int)));
     const int vm = fwd_vert_mode(g);
-    if (nviews > 0) {
+    if ((steps & kFwdBin) && nviews > 0) {
         // as many pass segments as the counters fit, up to one per wave
         const int nseg = std::min<int>(kBinWaves, (int)(kBinLds / sizeof(int)) / nbk);
         const size_t lds = (size_t)nseg * nbk * sizeof(int);
@@ -1584,6 +1584,7 @@ int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float 
                            list, cstart, FR);
         MBX_LAUNCH_CHECK();
     }
+    if (!(steps & kFwdProject)) return MBX_OK;
     double *p = sumsq ? part : nullptr;
 #define MBX_FWD(G, RA, RB_, VM_)                                                                                      \
     dispatch_fwd_2<G, RA, RB_, VM_>(g, pl, T, FR, list, cstart, vox, vstride, npix, sino, sino_dtype, layout,        \
@@ -1736,6 +1737,62 @@ int mbx_forward_project(const mbx_geom *g, const float *angles, const float *vox
         return fail(MBX_EWORKSPACE, "forward workspace too small: need " + std::to_string(need) + " bytes");
     return launch_fwd(d, angles, vox, pix, npix, views, nviews, sino, sino_dtype, sino_layout, accumulate, weights,
                       sumsq, workspace, (hipStream_t)stream);
+}
+
+int mbx_forward_plan(const mbx_geom *g, const float *angles, const int32_t *pix, int32_t npix, const int32_t *views,
+                     int32_t nviews, void *plan, int64_t plan_bytes, void *stream) {
+    DevGeom d;
+    int rc = make_dev_geom(g, &d);
+    if (rc != MBX_OK) return rc;
+    if (npix < 0) return fail(MBX_EINVAL, "npix must be >= 0");
+    if (!views) nviews = d.V;
+    if (nviews < 0) return fail(MBX_EINVAL, "nviews must be >= 0");
+    const int chunk = table_chunk(d, std::max(npix, 1), std::max(nviews, 1));
+    if (npix > chunk)
+        return fail(MBX_EUNSUPPORTED, "forward plan: " + std::to_string(npix) + " pixels exceed one table chunk (" +
+                                          std::to_string(chunk) + "); use mbx_forward_project");
+    if (!angles || (npix > 0 && !pix)) return fail(MBX_EINVAL, "NULL device pointer (angles or pix)");
+    const int64_t need = fwd_workspace_bytes(d, npix, nviews);
+    if (plan_bytes < need || (need > 0 && !plan))
+        return fail(MBX_EWORKSPACE, "forward plan too small: need " + std::to_string(need) + " bytes");
+    if (nviews == 0) return MBX_OK;
+    hipStream_t st = (hipStream_t)stream;
+    float *T = (float *)plan;
+    int *NC = table_nc(T, chunk, nviews);
+    void *aux = (char *)plan + table_bytes(chunk, nviews);
+    if (npix > 0) {
+        rc = launch_table(d, angles, pix, npix, views, nviews, T, NC, st);
+        if (rc != MBX_OK) return rc;
+    }
+    return launch_fwd_tab(d, T, NC, nullptr, npix, nviews, nullptr, MBX_DTYPE_F32, MBX_LAYOUT_VCR, 0, nullptr, aux,
+                          chunk, nullptr, st, 1, kFwdBin);
+}
+
+int mbx_forward_apply(const mbx_geom *g, const void *plan, const float *vox, int32_t npix, const int32_t *views,
+                      int32_t nviews, void *sino, int32_t sino_dtype, int32_t sino_layout, int32_t accumulate,
+                      const float *weights, double *sumsq, void *stream) {
+    DevGeom d;
+    int rc = make_dev_geom(g, &d);
+    if (rc != MBX_OK) return rc;
+    if (npix < 0) return fail(MBX_EINVAL, "npix must be >= 0");
+    if (!views) nviews = d.V;
+    if (nviews < 0) return fail(MBX_EINVAL, "nviews must be >= 0");
+    if (!dtype_ok(sino_dtype)) return fail(MBX_EINVAL, "unknown sinogram dtype");
+    if (sino_layout != MBX_LAYOUT_VCR && sino_layout != MBX_LAYOUT_VRC)
+        return fail(MBX_EINVAL, "unknown sinogram layout");
+    const int chunk = table_chunk(d, std::max(npix, 1), std::max(nviews, 1));
+    if (npix > chunk) return fail(MBX_EUNSUPPORTED, "forward apply: the pixel set exceeds one table chunk");
+    if (!plan || !sino || (npix > 0 && !vox)) return fail(MBX_EINVAL, "NULL device pointer (plan, sino or vox)");
+    hipStream_t st = (hipStream_t)stream;
+    if (nviews == 0) {
+        if (sumsq) return launch_sum_partials(nullptr, 0, sumsq, 0, st);  // writes 0
+        return MBX_OK;
+    }
+    const float *T = (const float *)plan;
+    const int *NC = table_nc((float *)plan, chunk, nviews);
+    void *aux = (char *)plan + table_bytes(chunk, nviews);
+    return launch_fwd_tab(d, T, NC, vox, npix, nviews, sino, sino_dtype, sino_layout, accumulate, weights, aux, chunk,
+                          sumsq, st, 1, kFwdProject);
 }
 
 int mbx_back_project(const mbx_geom *g, const float *angles, const void *sino, int32_t sino_dtype,

@@ -197,6 +197,29 @@ class TomographyModel(ParameterHandler):
             _lib.ptr(ws), wsb, _lib.stream()))
         return (out, sumsq) if want_sumsq else out
 
+    def _fwd_plan(self, pix, views=None):
+        """Geometry table, bins and forward records of the pixel set pix (device
+        int32), reusable by _fwd_apply while pix and views are unchanged."""
+        V = self.get_params('sinogram_shape')[0]
+        nv = V if views is None else int(views.numel())
+        nb = int(_lib.lib().mbx_forward_workspace_bytes(self._geom, int(pix.numel()), nv))
+        plan = torch.empty(max(nb, 256), dtype=torch.uint8, device=pix.device)
+        _lib.check(_lib.lib().mbx_forward_plan(self._geom, _lib.ptr(self._angles()), _lib.ptr(pix), int(pix.numel()),
+                                               _lib.ptr(views), nv, _lib.ptr(plan), nb, _lib.stream()))
+        return plan
+
+    def _fwd_apply(self, plan, vox, views=None, layout=_lib.LAYOUT_VCR, out_dtype=_F32, out=None, accumulate=False):
+        """Forward projection of cylinders vox (npix, Nz) with a plan from _fwd_plan."""
+        V, R, C = self.get_params('sinogram_shape')
+        nv = V if views is None else int(views.numel())
+        if out is None:
+            shape = (nv, C, R) if layout == _lib.LAYOUT_VCR else (nv, R, C)
+            out = torch.empty(shape, dtype=_torch_dtype(out_dtype), device=vox.device)
+        _lib.check(_lib.lib().mbx_forward_apply(self._geom, _lib.ptr(plan), _lib.ptr(vox), int(vox.shape[0]),
+                                                _lib.ptr(views), nv, _lib.ptr(out), out_dtype, layout, int(accumulate),
+                                                None, None, _lib.stream()))
+        return out
+
     def _fwd_rows(self, vox, pix, views=None):
         """(V', R, C) float32 projection: the channel-major forward, then one
         transpose (bitwise the same sums; the channel-major kernel's sliding

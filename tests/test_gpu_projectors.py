@@ -271,3 +271,32 @@ def test_chunked_full_projection_matches_parts():
     bp = m._back(full, p)
     bp_parts = torch.cat([m._back(full, p[:h].contiguous()), m._back(full, p[h:].contiguous())])
     assert torch.equal(bp, bp_parts)
+
+
+@pytest.mark.parametrize("case", ["cone_m2", "cone_bp2", "cone_psf2", "parallel", "translation"])
+def test_forward_plan_apply_matches_project(gpu, case):
+    """mbx_forward_plan + mbx_forward_apply give bitwise what mbx_forward_project
+    gives, over repeated applications of one plan (both layouts, accumulate,
+    bfloat16 output, a view subset)."""
+    from mbirjax_amd import _lib
+    print("case", case, flush=True)
+    m, g = CASES[case]()
+    rows, cols, nz = m.get_params('recon_shape')
+    rng = np.random.default_rng(21)
+    pix = torch.as_tensor(np.sort(rng.choice(rows * cols, size=max(1, rows * cols // 2), replace=False))
+                          .astype(np.int32), device='cuda')
+    x1 = torch.as_tensor(rng.standard_normal((pix.numel(), nz)).astype(np.float32), device='cuda')
+    x2 = torch.as_tensor(rng.standard_normal((pix.numel(), nz)).astype(np.float32), device='cuda')
+    plan = m._fwd_plan(pix)
+    for x in (x1, x2):
+        assert torch.equal(m._fwd_apply(plan, x), m._fwd(x, pix, layout=_lib.LAYOUT_VCR))
+        assert torch.equal(m._fwd_apply(plan, x, layout=_lib.LAYOUT_VRC), m._fwd(x, pix, layout=_lib.LAYOUT_VRC))
+    base = m._fwd(x1, pix, layout=_lib.LAYOUT_VCR)
+    acc = m._fwd_apply(plan, x2, out=base.clone(), accumulate=True)
+    assert torch.equal(acc, m._fwd(x2, pix, layout=_lib.LAYOUT_VCR, out=base.clone(), accumulate=True))
+    assert torch.equal(m._fwd_apply(plan, x1, out_dtype=_lib.DTYPE_BF16),
+                       m._fwd(x1, pix, layout=_lib.LAYOUT_VCR, out_dtype=_lib.DTYPE_BF16))
+    V = g.sinogram_shape[0]
+    views = torch.as_tensor(np.arange(1, V, 3, dtype=np.int32), device='cuda')
+    vplan = m._fwd_plan(pix, views)
+    assert torch.equal(m._fwd_apply(vplan, x1, views), m._fwd(x1, pix, views, layout=_lib.LAYOUT_VCR))

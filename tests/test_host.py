@@ -74,6 +74,15 @@ def test_validation_before_any_hip_call():
                               None) == _lib.MBX_EWORKSPACE
     assert L.mbx_forward_project(geom(), fake, fake, fake, 1, None, 0, fake, 0, 0, 0, None, None, fake,
                                  L.mbx_forward_workspace_bytes(geom(), 1, 8) - 1, None) == _lib.MBX_EWORKSPACE
+    # forward plan / apply: more pixels than one table chunk, small plan, bad layout
+    big = geom(sinogram_shape=(720, 4, 6))
+    chunk_px = 20000  # > the 512 MB table chunk at 720 views (10965 pixels)
+    assert L.mbx_forward_plan(big, fake, fake, chunk_px, None, 0, fake, 1 << 40, None) == _lib.MBX_EUNSUPPORTED
+    assert b"table chunk" in L.mbx_last_error()
+    assert L.mbx_forward_plan(geom(), fake, fake, 4, None, 0, fake, L.mbx_forward_workspace_bytes(geom(), 4, 8) - 1,
+                              None) == _lib.MBX_EWORKSPACE
+    assert L.mbx_forward_apply(geom(), fake, fake, 4, None, 0, fake, 0, 7, 0, None, None, None) == _lib.MBX_EINVAL
+    assert L.mbx_forward_apply(geom(), None, fake, 4, None, 0, fake, 0, 0, 0, None, None, None) == _lib.MBX_EINVAL
     # transpose aliasing, dot and axpy arguments
     assert L.mbx_sino_transpose(fake, 0, 0, fake, 0, 1, 2, 3, 4, None) == _lib.MBX_EINVAL
     assert L.mbx_dot(fake, 0, None, 0, None, -1, fake, fake, 8192, None) == _lib.MBX_EINVAL
