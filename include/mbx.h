@@ -44,6 +44,8 @@
  *   mbx_median_filter3d  <- median_filter3d (mbirjax/denoising.py:414-496)
  *   mbx_dot              <- the jnp.sum reductions of vcd_recon / get_forward_model_loss
  *                           (mbirjax/tomography_model.py:1139-1152, 1633-1658)
+ *   mbx_forward_plan / mbx_forward_apply: no counterpart (mbx_forward_project
+ *                           split into its setup and its projection kernel)
  *
  * Python binds these through ctypes (mbirjax_amd/_lib.py).
  */
