@@ -469,6 +469,8 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
     // forward projection of delta over the subset (into dsino), optionally with forward_quadratic
     // (interleaved chunks: chunk c holds pixels c, c + nch, ..., so that every
     // chunk spreads over the image and the forward's channel blocks stay balanced)
+    // (a second projection of one table chunk reuses the bins and records of the first)
+    bool binned = false;
     auto project_delta = [&](double *sumsq, const float *w) -> int {
         const int nch = (npix + L.chunk - 1) / L.chunk;
         for (int c = 0; c < nch; ++c) {
@@ -478,9 +480,11 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
                 if (r != MBX_OK) return r;
             }
             int r = launch_fwd_tab(d, T, NC, delta + (size_t)c * d.Nz, n, d.V, dsino, delta_dtype, MBX_LAYOUT_VCR,
-                                   c > 0, w, fpart, L.chunk, c == nch - 1 ? sumsq : nullptr, st, nch);
+                                   c > 0, w, fpart, L.chunk, c == nch - 1 ? sumsq : nullptr, st, nch,
+                                   binned ? kFwdProject : kFwdBin | kFwdProject);
             if (r != MBX_OK) return r;
         }
+        binned = one_table;
         return MBX_OK;
     };
     // (a) prior gradient and Hessian
