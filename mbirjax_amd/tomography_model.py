@@ -378,8 +378,16 @@ class TomographyModel(ParameterHandler):
     @staticmethod
     def _ramp_conv_rows(sino, recon_filter):
         """'valid' convolution of every detector row (last axis, length C) with
-        the (2C-1)-tap filter: out[i] = sum_j row[j] h(i - j), via rFFT."""
+        the (2C-1)-tap filter: out[i] = sum_j row[j] h(i - j).  Up to 2048
+        channels this is one float32 GEMM with the C x C Toeplitz matrix
+        H[j, i] = h(i - j) (512^3 benchmark: 4 ms, against 27 ms for the rFFT
+        route and 2.4 s for the FFT library's first call in a process);
+        beyond that, or when float32 GEMMs may use TF32, an rFFT."""
         C = sino.shape[-1]
+        if C <= 2048 and not torch.backends.cuda.matmul.allow_tf32:
+            i = torch.arange(C, device=sino.device)
+            H = recon_filter[(i[None, :] - i[:, None]) + (C - 1)].to(sino.dtype).contiguous()
+            return torch.matmul(sino, H)
         L = 1 << int(math.ceil(math.log2(3 * C - 2)))
         fr = torch.fft.rfft(sino, n=L, dim=-1)
         fh = torch.fft.rfft(recon_filter, n=L)

@@ -300,3 +300,28 @@ def test_forward_plan_apply_matches_project(gpu, case):
     views = torch.as_tensor(np.arange(1, V, 3, dtype=np.int32), device='cuda')
     vplan = m._fwd_plan(pix, views)
     assert torch.equal(m._fwd_apply(vplan, x1, views), m._fwd(x1, pix, views, layout=_lib.LAYOUT_VCR))
+
+
+@pytest.mark.parametrize("case", ["cone_offsets_odd", "cone_m2", "parallel", "parallel_offset_tall"])
+def test_direct_filters_match_oracle(gpu, case):
+    """FDK / FBP filtering (cosine weight, ramp over 2C-1 taps, scale) against
+    the FP64 oracle's exact Toeplitz restatement."""
+    from oracle import filters as ofilt
+    print("case", case, flush=True)
+    m, g = CASES[case]()
+    y = np.random.default_rng(31).standard_normal(g.sinogram_shape).astype(np.float32)
+    got = m.direct_filter(y)
+    ref = ofilt.fdk_filter(g, y) if case.startswith("cone") else ofilt.fbp_filter(g, y)
+    assert rel(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("C", [48, 2100])  # the Toeplitz-GEMM route, then the rFFT route
+def test_ramp_row_convolution_routes(gpu, C):
+    from oracle import filters as ofilt
+    rng = np.random.default_rng(32)
+    rows = rng.standard_normal((3, 5, C)).astype(np.float32)
+    h = ofilt.ramp_filter(C)
+    got = mj.TomographyModel._ramp_conv_rows(torch.as_tensor(rows, device='cuda'),
+                                             torch.as_tensor(h.astype(np.float32), device='cuda'))
+    ref = np.stack([[np.convolve(r.astype(np.float64), h, mode='valid') for r in v] for v in rows])
+    assert rel(got, ref) <= TOL
