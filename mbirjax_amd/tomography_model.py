@@ -401,10 +401,10 @@ class TomographyModel(ParameterHandler):
         """MBIR reconstruction with multi-granular VCD (mbirjax/tomography_model.py:966-1084).
 
         precision: 'fp32' (plain); 'mixed' (error sinogram accumulated in float32,
-        delta sinogram A*delta stored in bfloat16, Kahan-compensated back
-        projection, FP64 step-size and cost reductions); or 'bf16' (as 'mixed'
-        with the error sinogram also stored in bfloat16: least memory traffic,
-        but round-to-nearest drops updates below half a bfloat16 ulp).
+        delta sinogram A*delta stored in bfloat16, FP64 step-size and cost
+        reductions); or 'bf16' (as 'mixed' with the error sinogram also stored in
+        bfloat16: least memory traffic, but round-to-nearest drops updates below
+        half a bfloat16 ulp).
         """
         if num_iterations is not None:
             warnings.warn('num_iterations has been deprecated and will be removed in a future release.\n'
