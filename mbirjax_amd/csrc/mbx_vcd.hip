@@ -53,36 +53,44 @@ __device__ __forceinline__ float two_b_tilde(float delta, float bval, const QP &
     return bval * (q.scale * num / den);
 }
 
+// Surrogate gradient and Hessian of voxel (pixel p, slice k) over its six
+// neighbours.
+__device__ __forceinline__ void qggmrf_gh(const float *__restrict__ x, int rows, int cols, int Nz, int p, int k,
+                                          const QP &q, float &g, float &h) {
+    const int i = p / cols, j = p - i * cols;
+    const float *xc = x + (size_t)p * Nz;
+    const float v = xc[k];
+    // cylinder neighbours with reflected (zero-difference) ends (qggmrf.py:151-188)
+    const float dplus = (k + 1 < Nz) ? xc[k + 1] - v : 0.0f;
+    const float dminus = (k > 0) ? v - xc[k - 1] : 0.0f;
+    const float tp = two_b_tilde(dplus, 1.0f, q);
+    const float tm = two_b_tilde(dminus, 1.0f, q);
+    g = -q.b[4] * (tp * dplus);
+    h = q.b[4] * tp;
+    g += q.b[5] * (tm * dminus);
+    h += q.b[5] * tm;
+    // in-plane neighbours [row+1, row-1, col+1, col-1], clipped at the borders
+    // (ravel_multi_index(..., mode='clip'), qggmrf.py:191-247)
+    const int nb[4] = {min(i + 1, rows - 1) * cols + j, max(i - 1, 0) * cols + j, i * cols + min(j + 1, cols - 1),
+                       i * cols + max(j - 1, 0)};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const float d = v - x[(size_t)nb[t] * Nz + k];
+        const float bt = two_b_tilde(d, q.b[t], q);
+        g += bt * d;
+        h += bt;
+    }
+}
+
 __global__ __launch_bounds__(256) void qggmrf_kernel(const float *__restrict__ x, int rows, int cols, int Nz,
                                                      const int32_t *__restrict__ pix, int npix, QP q,
                                                      float *__restrict__ grad, float *__restrict__ hess) {
     const int s = blockIdx.x;
     if (s >= npix) return;
     const int p = pix[s];
-    const int i = p / cols, j = p - i * cols;
-    // clipped in-plane neighbours: ravel_multi_index(..., mode='clip') (qggmrf.py:235-236)
-    const int nb[4] = {min(i + 1, rows - 1) * cols + j, max(i - 1, 0) * cols + j, i * cols + min(j + 1, cols - 1),
-                       i * cols + max(j - 1, 0)};
-    const float *xc = x + (size_t)p * Nz;
     for (int k = threadIdx.x; k < Nz; k += blockDim.x) {
-        const float v = xc[k];
-        // cylinder neighbours with reflected (zero-difference) ends (qggmrf.py:151-188)
-        const float dplus = (k + 1 < Nz) ? xc[k + 1] - v : 0.0f;
-        const float dminus = (k > 0) ? v - xc[k - 1] : 0.0f;
-        const float tp = two_b_tilde(dplus, 1.0f, q);
-        const float tm = two_b_tilde(dminus, 1.0f, q);
-        float g = -q.b[4] * (tp * dplus);
-        float h = q.b[4] * tp;
-        g += q.b[5] * (tm * dminus);
-        h += q.b[5] * tm;
-        // in-plane neighbours [row+1, row-1, col+1, col-1] (qggmrf.py:191-247)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const float d = v - x[(size_t)nb[t] * Nz + k];
-            const float bt = two_b_tilde(d, q.b[t], q);
-            g += bt * d;
-            h += bt;
-        }
+        float g, h;
+        qggmrf_gh(x, rows, cols, Nz, p, k, q, g, h);
         grad[(size_t)s * Nz + k] = g;
         hess[(size_t)s * Nz + k] = h;
     }
@@ -207,8 +215,11 @@ static int launch_axpy(void *e, int edt, const void *d, int ddt, const double *a
 
 // delta = -((fg + pg) / (fh + ph)) with fg = -fmc*bp, fh = fmc*H[p];
 // partials: [0] prior_linear = sum pg*delta, [1] sum ph*delta^2, [2] sum bp*delta
+// QG: the prior gradient and Hessian are the qGGMRF surrogate's, computed here
+// from the recon x (else pg is read and ph = ph_const, the proximal map's).
+template <bool QG>
 __global__ __launch_bounds__(256) void delta_kernel(const float *__restrict__ bp, const float *__restrict__ pg,
-                                                    const float *__restrict__ ph, float ph_const,
+                                                    float ph_const, const float *__restrict__ x, int rows, int cols, QP q,
                                                     const float *__restrict__ fm_hess, const int32_t *__restrict__ pix,
                                                     int npix, int Nz, float fmc, float *__restrict__ delta,
                                                     double *__restrict__ part) {
@@ -219,8 +230,13 @@ __global__ __launch_bounds__(256) void delta_kernel(const float *__restrict__ bp
         const int s = (int)(t / Nz), k = (int)(t - (int64_t)s * Nz);
         const float fg = -(fmc * bp[t]);
         const float fh = fmc * fm_hess[(size_t)pix[s] * Nz + k];
-        const float h = ph ? ph[t] : ph_const;
-        const float g = pg[t];
+        float g, h;
+        if (QG) {
+            qggmrf_gh(x, rows, cols, Nz, pix[s], k, q, g, h);
+        } else {
+            h = ph_const;
+            g = pg[t];
+        }
         const float dl = -((fg + g) / (fh + h));
         delta[t] = dl;
         pl += (double)g * dl;
@@ -311,7 +327,7 @@ __global__ __launch_bounds__(256) void axpy_scal_kernel(TE *__restrict__ e, cons
 
 struct VcdLayout {
     int chunk;  // pixels per geometry table
-    size_t table, bpart, fpart, bp, pg, ph, delta, dsino, dpart, l1part, scal, total;
+    size_t table, bpart, fpart, bp, pg, delta, dsino, dpart, l1part, scal, total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -330,9 +346,7 @@ static VcdLayout vcd_layout(const DevGeom &g, int npix, int ddt) {
     off += align256((size_t)fwd_aux_bytes(g, L.chunk, g.V));
     L.bp = off;
     off += align256(cyl);
-    L.pg = off;
-    off += align256(cyl);
-    L.ph = off;
+    L.pg = off;  // proximal-map prior gradient
     off += align256(cyl);
     L.delta = off;
     off += align256(cyl);
@@ -441,7 +455,6 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
     char *ws = (char *)workspace;
     float *bp = (float *)(ws + L.bp);
     float *pg = (float *)(ws + L.pg);
-    float *ph = (float *)(ws + L.ph);
     float *delta = (float *)(ws + L.delta);
     void *dsino = (void *)(ws + L.dsino);
     double *dpart = (double *)(ws + L.dpart);
@@ -487,22 +500,20 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
         binned = one_table;
         return MBX_OK;
     };
-    // (a) prior gradient and Hessian
-    float ph_const = 0.0f;
-    const float *php = ph;
+    // (a) prior gradient and Hessian (the qGGMRF surrogate's inside the delta
+    // kernel) and (d,e,f) delta and FP64 partials
     if (prox_input) {
         hipLaunchKernelGGL(prox_grad_kernel, dim3(nb), dim3(256), 0, st, flat_recon, prox_input, pix, npix, d.Nz,
                            (float)(1.0 / (sigma_prox * sigma_prox)), pg);
         MBX_LAUNCH_CHECK();
-        ph_const = (float)(sigma_prox * sigma_prox);
-        php = nullptr;
+        hipLaunchKernelGGL((delta_kernel<false>), dim3(nb), dim3(256), 0, st, bp, pg,
+                           (float)(sigma_prox * sigma_prox), nullptr, d.rows, d.cols, QP{}, fm_hessian, pix, npix,
+                           d.Nz, (float)fm_constant, delta, dpart);
     } else {
-        rc = mbx_qggmrf_grad_hess(flat_recon, d.rows, d.cols, d.Nz, pix, npix, qparams, pg, ph, stream);
-        if (rc != MBX_OK) return rc;
+        hipLaunchKernelGGL((delta_kernel<true>), dim3(nb), dim3(256), 0, st, bp, nullptr, 0.0f, flat_recon,
+                           d.rows, d.cols, make_qp(qparams), fm_hessian, pix, npix, d.Nz, (float)fm_constant, delta,
+                           dpart);
     }
-    // (d,e,f) delta and FP64 partials
-    hipLaunchKernelGGL(delta_kernel, dim3(nb), dim3(256), 0, st, bp, pg, php, ph_const, fm_hessian, pix, npix, d.Nz,
-                       (float)fm_constant, delta, dpart);
     MBX_LAUNCH_CHECK();
     // (g,h) delta sinogram and forward_quadratic
     rc = project_delta(scal + 0, weights);
