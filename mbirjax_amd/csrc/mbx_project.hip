@@ -294,20 +294,27 @@ int launch_table(const DevGeom &g, const float *angles, const int32_t *pix, int 
 BackPlan plan_back(const DevGeom &g, int npix, int nviews, int ppw) {
     BackPlan p;
     const int per_lane = (g.Nz + kWave - 1) / kWave;
-    p.kpl = per_lane <= 4 ? 4 : per_lane <= 8 ? 8 : 16;
+    p.kpl = per_lane <= 2 ? 2 : per_lane <= 4 ? 4 : per_lane <= 8 ? 8 : 16;
     p.nchunk = (g.Nz + kWave * p.kpl - 1) / (kWave * p.kpl);
     p.ppw = ppw;
     const int64_t units = ((int64_t)npix + ppw - 1) / ppw * p.nchunk;  // waves per view group
     // Small subsets: split the views over the XCDs so that each XCD's L2
     // streams only its share of the sinogram, keeping every wave of the pass
-    // resident at once (7 waves per SIMD); large passes are VALU-bound.
+    // resident at once (7 waves per SIMD); large passes are VALU-bound.  With
+    // two pixels per wave the split (a multiple of the 8 XCDs) grows while the
+    // pass has fewer than 4096 waves, every group keeps 4 views or more and
+    // the partials stay within 64 MiB.
     const int64_t resident = 7LL * 4 * 256;
-    if (nviews < 8 || units > 16384)
+    if (nviews < 8 || units > 16384) {
         p.nsplit = 1;
-    else if (ppw == 2)
+    } else if (ppw == 2) {
         p.nsplit = 8;
-    else
+        while (units * p.nsplit < 4096 && p.nsplit < 64 && nviews >= 8 * p.nsplit &&
+               2 * p.nsplit * (int64_t)npix * g.Nz * (int64_t)sizeof(float) <= (64ll << 20))
+            p.nsplit *= 2;
+    } else {
         p.nsplit = units * 8 <= resident ? 8 : units * 4 <= resident ? 4 : 2;
+    }
     // LDS detector-column capacity per pixel: the rows one slice chunk can
     // reach, from the largest pixel magnification in the recon disc (plus
     // taps and margins), padded to whole 64-row chunks; at least the detector
@@ -759,6 +766,9 @@ static int dispatch_back_1(const DevGeom &g, const BackPlan &pl, const float *T,
         case 4:
             return dispatch_back_2<GEOM, 4>(g, pl, T, sino, dt, w, npix, views, nviews, pw, kahan, scale, accumulate,
                                             out, part, st);
+        case 2:
+            return dispatch_back_2<GEOM, 2>(g, pl, T, sino, dt, w, npix, views, nviews, pw, kahan, scale, accumulate,
+                                            out, part, st);
         case 8:
             return dispatch_back_2<GEOM, 8>(g, pl, T, sino, dt, w, npix, views, nviews, pw, kahan, scale, accumulate,
                                             out, part, st);
@@ -822,6 +832,8 @@ int launch_back(const DevGeom &g, const float *angles, const void *sino, int sin
 
 constexpr int kFwdCB = 16;        // channels per block, (views, rows, channels) output: the per-lane register tile
 constexpr int kFwdCBW = 32;       // channels per block, (views, channels, rows) output: the sliding window's sweep
+constexpr int kFwdCBWMin = 8;     // ... narrowed down to this on small launches
+constexpr int64_t kFwdMinWaves = 4096;  // four waves per SIMD
 constexpr int kFwdRPL = 2;        // detector rows per lane (64 apart), (views, rows, channels) output
 constexpr int kFwdRPLW = 4;       // detector rows per lane, (views, channels, rows) output
 constexpr int kFwdPad = 8;        // zero slices either side of a staged cylinder (>= 2*bp_psf_radius + 1;
@@ -839,6 +851,13 @@ FwdPlan plan_fwd(const DevGeom &g, int nviews, int layout) {
     p.nrb = (g.R + p.rows_pb - 1) / p.rows_pb;
     p.cb = layout == MBX_LAYOUT_VCR ? kFwdCBW : kFwdCB;
     p.ncb = (g.C + p.cb - 1) / p.cb;
+    // channel-major output: narrower channel blocks while the launch has
+    // fewer than kFwdMinWaves waves, each block then walking fewer candidates
+    while (layout == MBX_LAYOUT_VCR && p.cb > kFwdCBWMin &&
+           (int64_t)nviews * p.ncb * p.nrb * p.nw < kFwdMinWaves) {
+        p.cb /= 2;
+        p.ncb = (g.C + p.cb - 1) / p.cb;
+    }
     p.list = 0;
     // A staged cylinder holds the slices the block's rows reach (at most all
     // slices) in whole 64-slice chunks, and the zero pads.
@@ -1114,7 +1133,8 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
 //   update:  every wave resamples the staged cylinders onto its rows
 //            (row-driven vertical fan, mbirjax/cone_beam.py:347-372) and weights
 //            them with the horizontal taps (mbirjax/cone_beam.py:276-282).
-// Channel-major output (views, channels, rows): 32-channel blocks of 2 waves,
+// Channel-major output (views, channels, rows): 32-channel blocks (8 or 16 on
+// launches of fewer than 4096 waves) of 2 waves,
 // lane l of wave w owning rows m0 + 256w + l + 64r (r < 4); the candidates are
 // walked in list order and a sliding window of 2*RAD+1 channel accumulators
 // per row follows their buckets, streaming each channel out as soon as it is
@@ -1128,13 +1148,17 @@ template <int GEOM, int RAD, int RB, int VM, typename OT, int LAYOUT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void fwd_kernel(
     DevGeom g, const float *__restrict__ T, const float *__restrict__ FR, const int *__restrict__ list,
     const int *__restrict__ cstart, const float *__restrict__ vox, int64_t vstride, int npix, int ncb, int nrb,
-    int nbatch, int cstride, OT *__restrict__ sino, int accumulate, const float *__restrict__ w,
+    int cbw, int nbatch, int cstride, OT *__restrict__ sino, int accumulate, const float *__restrict__ w,
     double *__restrict__ part_sumsq) {
     // (views, channels, rows) output: a sliding window of 2*RAD+1 channels per row
     constexpr bool SW = LAYOUT == MBX_LAYOUT_VCR;
-    constexpr int CB = SW ? kFwdCBW : kFwdCB, P = kFwdPad, RPL = SW ? kFwdRPLW : kFwdRPL;
-    constexpr int ND = CB + 2 * RAD;  // centre-channel buckets n0 - RAD .. n0 + CB - 1 + RAD
-    static_assert(!SW || ND < kWave, "the bucket starts are held one per lane");
+    constexpr int CB = kFwdCB, P = kFwdPad, RPL = SW ? kFwdRPLW : kFwdRPL;
+    // channels per block: the register tile's CB, or the window's cbw (at most
+    // kFwdCBW, so that the bucket starts fit one per lane)
+    const int cbs = SW ? cbw : CB;
+    constexpr int NDT = CB + 2 * RAD;  // register tile: centre-channel buckets n0 - RAD .. n0 + CB - 1 + RAD
+    const int ND = cbs + 2 * RAD;      // buckets of the block
+    static_assert(kFwdCBW + 2 * RAD < kWave, "the bucket starts are held one per lane");
     extern __shared__ float lds[];
     const int nw = blockDim.x >> 6;
     float *bufs = lds;                                                  // [2][nbatch][cstride]
@@ -1146,7 +1170,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     const int vi = b / (ncb * nrb);
     const int rem = b - vi * (ncb * nrb);
     const int cbi = rem / nrb, rbi = rem - (rem / nrb) * nrb;
-    const int n0 = cbi * CB, n1 = min(g.C, n0 + CB);
+    const int n0 = cbi * cbs, n1 = min(g.C, n0 + cbs);
     const int rows_pb = nw * kWave * RPL;
     const int m0 = rbi * rows_pb, m1 = min(g.R, m0 + rows_pb);
     const int wm0 = m0 + wave * kWave * RPL;  // this wave's rows: wm0 + lane + 64 r
@@ -1158,7 +1182,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     const int *csv = cstart + (size_t)vi * (nbk + 1);
     // bucket starts of this block: bs[D] (or lane D of bsl) is the start of
     // centre channel n0 + D - RAD
-    int bs[SW ? 1 : ND + 1];
+    int bs[SW ? 1 : NDT + 1];
     int bsl = 0, lo, hi;
     if constexpr (SW) {
         bsl = csv[min(n0 + min(lane, ND), nbk)];
@@ -1166,9 +1190,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         hi = __builtin_amdgcn_readlane(bsl, ND);
     } else {
 #pragma unroll
-        for (int D = 0; D <= ND; ++D) bs[D] = csv[min(n0 + D, nbk)];
+        for (int D = 0; D <= NDT; ++D) bs[D] = csv[min(n0 + D, nbk)];
         lo = bs[0];
-        hi = bs[ND];
+        hi = bs[NDT];
     }
     const int nbat = (hi - lo + nbatch - 1) / nbatch;
 
@@ -1483,7 +1507,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             }
         } else if (wm0 < m1) {  // wave-uniform
 #pragma unroll
-            for (int D = 0; D < ND; ++D) {
+            for (int D = 0; D < NDT; ++D) {
                 const int slo = max(bs[D], b0) - b0;
                 const int shi = min(bs[D + 1], b0 + nb) - b0;
                 const float4 *rp = (const float4 *)(FRv + (size_t)(b0 + slo) * kFRec);
@@ -1532,7 +1556,8 @@ static void launch_fwd_t(const DevGeom &g, const FwdPlan &pl, const float *T, co
                          const int *cstart, const float *vox, int64_t vstride, int npix, OT *sino, int accumulate,
                          const float *w, double *part, hipStream_t st) {
     hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, VM, OT, LAYOUT>), dim3((unsigned)pl.nblocks), dim3(64 * pl.nw),
-                       pl.lds, st, g, T, FR, list, cstart, vox, vstride, npix, pl.ncb, pl.nrb, pl.nb, pl.cstride,
+                       pl.lds, st, g, T, FR, list, cstart, vox, vstride, npix, pl.ncb, pl.nrb, pl.cb, pl.nb,
+                       pl.cstride,
                        sino, accumulate, w, part);
 }
 
