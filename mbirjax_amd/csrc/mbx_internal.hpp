@@ -87,13 +87,20 @@ int launch_back(const DevGeom &g, const float *angles, const void *sino, int sin
 int launch_fwd(const DevGeom &g, const float *angles, const float *vox, const int32_t *pix, int npix,
                const int32_t *views, int nviews, void *sino, int sino_dtype, int layout, int accumulate,
                const float *weights, double *sumsq, void *ws, hipStream_t st);
+// defer (may be NULL): when the views are split into groups, leave the nsplit
+// partials (npix * Nz floats each, in part) unreduced and set *defer = nsplit
+// (0 when out is written).  The reduction is the float sum over the groups in
+// order, times scale.
 int launch_back_tab(const DevGeom &g, const float *T, int npix, const void *sino, int sino_dtype,
                     const float *weights, const int32_t *views, int nviews, int coeff_power, int kahan, float scale,
-                    int accumulate, float *out, float *part, hipStream_t st);
+                    int accumulate, float *out, float *part, hipStream_t st, int *defer = nullptr);
 // The cylinder of table slot s is vox + s * pstride * Nz.  steps: kFwdBin
 // (bins and forward records from the table), kFwdProject (the projection
-// from bins and records already in aux), or both.
-constexpr int kFwdBin = 1, kFwdProject = 2;
+// from bins and records already in aux), or both; kFwdPartials: the
+// fwd_blocks() FP64 partials of the weighted squared norm are written to the
+// start of aux (and not reduced unless sumsq is given).
+constexpr int kFwdBin = 1, kFwdProject = 2, kFwdPartials = 4;
+int64_t fwd_blocks(const DevGeom &g, int nviews, int layout);
 int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float *vox, int npix, int nviews, void *sino,
                    int sino_dtype, int layout, int accumulate, const float *weights, void *aux, int chunk,
                    double *sumsq, hipStream_t st, int pstride = 1, int steps = kFwdBin | kFwdProject);

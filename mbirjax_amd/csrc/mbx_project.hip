@@ -781,7 +781,8 @@ static int dispatch_back_1(const DevGeom &g, const BackPlan &pl, const float *T,
 // Back projection of pixels whose table T (nviews x npix records) is built.
 int launch_back_tab(const DevGeom &g, const float *T, int npix, const void *sino, int sino_dtype,
                     const float *weights, const int32_t *views, int nviews, int coeff_power, int kahan, float scale,
-                    int accumulate, float *out, float *part, hipStream_t st) {
+                    int accumulate, float *out, float *part, hipStream_t st, int *defer) {
+    if (defer) *defer = 0;
     if (npix == 0) return MBX_OK;
     // Kahan keeps a compensation per slice: one pixel per wave keeps it in registers
     const BackPlan pl = plan_back(g, npix, nviews, kahan ? 1 : 2);
@@ -795,6 +796,10 @@ int launch_back_tab(const DevGeom &g, const float *T, int npix, const void *sino
                                                 kahan, scale, accumulate, out, part, st);
     if (rc != MBX_OK) return rc;
     MBX_LAUNCH_CHECK();
+    if (pl.nsplit > 1 && defer) {
+        *defer = pl.nsplit;  // the caller sums the partials
+        return MBX_OK;
+    }
     if (pl.nsplit > 1) {
         const int64_t n = (int64_t)npix * g.Nz;
         const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
@@ -876,6 +881,7 @@ static int64_t fwd_part_blocks(const DevGeom &g, int nviews) {
     return std::max(plan_fwd(g, nviews, MBX_LAYOUT_VCR).nblocks, plan_fwd(g, nviews, MBX_LAYOUT_VRC).nblocks);
 }
 static int fwd_nbk(const DevGeom &g) { return g.C + 2 * g.rh; }  // centre-channel buckets n_c + r
+int64_t fwd_blocks(const DevGeom &g, int nviews, int layout) { return plan_fwd(g, nviews, layout).nblocks; }
 
 // Forward record of one (view, pixel) pair, stored in the view's binned order
 // (centre channel, then pixel): what the forward update reads per candidate
@@ -1610,7 +1616,7 @@ int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float 
         MBX_LAUNCH_CHECK();
     }
     if (!(steps & kFwdProject)) return MBX_OK;
-    double *p = sumsq ? part : nullptr;
+    double *p = (sumsq || (steps & kFwdPartials)) ? part : nullptr;
 #define MBX_FWD(G, RA, RB_, VM_)                                                                                      \
     dispatch_fwd_2<G, RA, RB_, VM_>(g, pl, T, FR, list, cstart, vox, vstride, npix, sino, sino_dtype, layout,        \
                                     accumulate, weights, p, st)

@@ -217,8 +217,11 @@ static int launch_axpy(void *e, int edt, const void *d, int ddt, const double *a
 // partials: [0] prior_linear = sum pg*delta, [1] sum ph*delta^2, [2] sum bp*delta
 // QG: the prior gradient and Hessian are the qGGMRF surrogate's, computed here
 // from the recon x (else pg is read and ph = ph_const, the proximal map's).
+// bsplit > 0: bp is the float sum, in order, of the back projection's bsplit
+// view-group partials in bpart (each npix * Nz).
 template <bool QG>
-__global__ __launch_bounds__(256) void delta_kernel(const float *__restrict__ bp, const float *__restrict__ pg,
+__global__ __launch_bounds__(256) void delta_kernel(const float *__restrict__ bp, const float *__restrict__ bpart,
+                                                    int bsplit, const float *__restrict__ pg,
                                                     float ph_const, const float *__restrict__ x, int rows, int cols, QP q,
                                                     const float *__restrict__ fm_hess, const int32_t *__restrict__ pix,
                                                     int npix, int Nz, float fmc, float *__restrict__ delta,
@@ -228,7 +231,14 @@ __global__ __launch_bounds__(256) void delta_kernel(const float *__restrict__ bp
     double pl = 0.0, pq = 0.0, fl = 0.0;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
         const int s = (int)(t / Nz), k = (int)(t - (int64_t)s * Nz);
-        const float fg = -(fmc * bp[t]);
+        float b;
+        if (bsplit > 0) {
+            b = 0.0f;
+            for (int j = 0; j < bsplit; ++j) b += bpart[(size_t)j * n + t];
+        } else {
+            b = bp[t];
+        }
+        const float fg = -(fmc * b);
         const float fh = fmc * fm_hess[(size_t)pix[s] * Nz + k];
         float g, h;
         if (QG) {
@@ -241,7 +251,7 @@ __global__ __launch_bounds__(256) void delta_kernel(const float *__restrict__ bp
         delta[t] = dl;
         pl += (double)g * dl;
         pq += (double)h * dl * dl;
-        fl += (double)bp[t] * dl;
+        fl += (double)b * dl;
     }
     const double r0 = block_reduce<double>(pl, scratch);
     const double r1 = block_reduce<double>(pq, scratch);
@@ -253,22 +263,27 @@ __global__ __launch_bounds__(256) void delta_kernel(const float *__restrict__ bp
     }
 }
 
-// scal[0] = fq_raw (sum of delta_sino^2 * w), written by the forward projection.
+// fq_raw = sum of delta_sino^2 * w, from the forward projection's nfq
+// per-block partials fqpart (kept in scal[0]).
 // alpha = clip((fl - pl) / (fq + pq + eps32), eps32, 1.5)   (tomography_model.py:1467-1471)
-__global__ __launch_bounds__(256) void alpha_kernel(const double *__restrict__ part, int nblk, double fmc,
+__global__ __launch_bounds__(256) void alpha_kernel(const double *__restrict__ part, int nblk,
+                                                    const double *__restrict__ fqpart, int64_t nfq, double fmc,
                                                     double *__restrict__ scal, double *__restrict__ stats) {
     __shared__ double scratch[8];
-    double a = 0.0, b = 0.0, c = 0.0;
+    double a = 0.0, b = 0.0, c = 0.0, f = 0.0;
     for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
         a += part[i];
         b += part[kRedBlocks + i];
         c += part[2 * kRedBlocks + i];
     }
+    for (int64_t i = threadIdx.x; i < nfq; i += blockDim.x) f += fqpart[i];
     const double pl = block_reduce<double>(a, scratch);
     const double pq = 0.5 * block_reduce<double>(b, scratch);
     const double fl = fmc * block_reduce<double>(c, scratch);
+    const double fq_raw = block_reduce<double>(f, scratch);
     if (threadIdx.x == 0) {
-        const double fq = fmc * scal[0];
+        scal[0] = fq_raw;
+        const double fq = fmc * fq_raw;
         const double eps = (double)kEps32;
         double alpha = (fl - pl) / (fq + pq + eps);
         alpha = alpha < eps ? eps : (alpha > 1.5 ? 1.5 : alpha);
@@ -470,21 +485,25 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
     void *fpart = ws + L.fpart;
     const bool one_table = npix <= L.chunk;  // build the table once for all projections
 
-    // (c) back projection of the weighted error sinogram
+    // (c) back projection of the weighted error sinogram (with one table, the
+    // delta kernel sums its view-group partials)
+    int bsplit = 0;
     for (int c0 = 0; c0 < npix; c0 += L.chunk) {
         const int n = std::min(L.chunk, npix - c0);
         rc = launch_table(d, angles, pix + c0, n, nullptr, d.V, T, NC, st);
         if (rc != MBX_OK) return rc;
         rc = launch_back_tab(d, T, n, e, e_dtype, weights, nullptr, d.V, 1, kahan, 1.0f, 0, bp + (size_t)c0 * d.Nz,
-                             bpart, st);
+                             bpart, st, one_table ? &bsplit : nullptr);
         if (rc != MBX_OK) return rc;
     }
     // forward projection of delta over the subset (into dsino), optionally with forward_quadratic
     // (interleaved chunks: chunk c holds pixels c, c + nch, ..., so that every
     // chunk spreads over the image and the forward's channel blocks stay balanced)
-    // (a second projection of one table chunk reuses the bins and records of the first)
+    // (a second projection of one table chunk reuses the bins and records of the
+    // first; with partials, the last chunk leaves the FP64 partials of
+    // sum w * (A delta)^2 at the start of fpart)
     bool binned = false;
-    auto project_delta = [&](double *sumsq, const float *w) -> int {
+    auto project_delta = [&](bool partials, const float *w) -> int {
         const int nch = (npix + L.chunk - 1) / L.chunk;
         for (int c = 0; c < nch; ++c) {
             const int n = (npix - c + nch - 1) / nch;
@@ -493,8 +512,9 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
                 if (r != MBX_OK) return r;
             }
             int r = launch_fwd_tab(d, T, NC, delta + (size_t)c * d.Nz, n, d.V, dsino, delta_dtype, MBX_LAYOUT_VCR,
-                                   c > 0, w, fpart, L.chunk, c == nch - 1 ? sumsq : nullptr, st, nch,
-                                   binned ? kFwdProject : kFwdBin | kFwdProject);
+                                   c > 0, w, fpart, L.chunk, nullptr, st, nch,
+                                   (binned ? kFwdProject : kFwdBin | kFwdProject) |
+                                       (partials && c == nch - 1 ? kFwdPartials : 0));
             if (r != MBX_OK) return r;
         }
         binned = one_table;
@@ -506,26 +526,27 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
         hipLaunchKernelGGL(prox_grad_kernel, dim3(nb), dim3(256), 0, st, flat_recon, prox_input, pix, npix, d.Nz,
                            (float)(1.0 / (sigma_prox * sigma_prox)), pg);
         MBX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((delta_kernel<false>), dim3(nb), dim3(256), 0, st, bp, pg,
+        hipLaunchKernelGGL((delta_kernel<false>), dim3(nb), dim3(256), 0, st, bp, bpart, bsplit, pg,
                            (float)(sigma_prox * sigma_prox), nullptr, d.rows, d.cols, QP{}, fm_hessian, pix, npix,
                            d.Nz, (float)fm_constant, delta, dpart);
     } else {
-        hipLaunchKernelGGL((delta_kernel<true>), dim3(nb), dim3(256), 0, st, bp, nullptr, 0.0f, flat_recon,
-                           d.rows, d.cols, make_qp(qparams), fm_hessian, pix, npix, d.Nz, (float)fm_constant, delta,
-                           dpart);
+        hipLaunchKernelGGL((delta_kernel<true>), dim3(nb), dim3(256), 0, st, bp, bpart, bsplit, nullptr, 0.0f,
+                           flat_recon, d.rows, d.cols, make_qp(qparams), fm_hessian, pix, npix, d.Nz,
+                           (float)fm_constant, delta, dpart);
     }
     MBX_LAUNCH_CHECK();
     // (g,h) delta sinogram and forward_quadratic
-    rc = project_delta(scal + 0, weights);
+    rc = project_delta(true, weights);
     if (rc != MBX_OK) return rc;
     // (i) step size on the device
-    hipLaunchKernelGGL(alpha_kernel, dim3(1), dim3(256), 0, st, dpart, nb, fm_constant, scal, stats);
+    hipLaunchKernelGGL(alpha_kernel, dim3(1), dim3(256), 0, st, dpart, nb, (const double *)fpart,
+                       fwd_blocks(d, d.V, MBX_LAYOUT_VCR), fm_constant, scal, stats);
     MBX_LAUNCH_CHECK();
     // (j) positivity
     if (positivity) {
         hipLaunchKernelGGL(positivity_kernel, dim3(nb), dim3(256), 0, st, flat_recon, pix, npix, d.Nz, scal, delta);
         MBX_LAUNCH_CHECK();
-        rc = project_delta(nullptr, nullptr);
+        rc = project_delta(false, nullptr);
         if (rc != MBX_OK) return rc;
     }
     // (k,m) recon update and ell1
