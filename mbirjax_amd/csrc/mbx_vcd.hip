@@ -234,7 +234,8 @@ __global__ __launch_bounds__(256) void delta_kernel(const float *__restrict__ bp
         float b;
         if (bsplit > 0) {
             b = 0.0f;
-            for (int j = 0; j < bsplit; ++j) b += bpart[(size_t)j * n + t];
+#pragma unroll 8
+            for (int j = 0; j < bsplit; ++j) b += bpart[(size_t)j * n + t];  // loads in flight, adds in order
         } else {
             b = bp[t];
         }
@@ -276,6 +277,7 @@ __global__ __launch_bounds__(256) void alpha_kernel(const double *__restrict__ p
         b += part[kRedBlocks + i];
         c += part[2 * kRedBlocks + i];
     }
+#pragma unroll 8
     for (int64_t i = threadIdx.x; i < nfq; i += blockDim.x) f += fqpart[i];
     const double pl = block_reduce<double>(a, scratch);
     const double pq = 0.5 * block_reduce<double>(b, scratch);
