@@ -90,10 +90,14 @@ int launch_fwd(const DevGeom &g, const float *angles, const float *vox, const in
 // defer (may be NULL): when the views are split into groups, leave the nsplit
 // partials (npix * Nz floats each, in part) unreduced and set *defer = nsplit
 // (0 when out is written).  The reduction is the float sum over the groups in
-// order, times scale.
+// order, times scale.  plan_npix: plan the launch as if it had
+// max(npix, plan_npix) pixels -- a chunk of a larger call passes the chunk
+// size that back_part_bytes() sized part for (the split grows as the pixel
+// count falls, so a smaller last chunk planned on its own could need more).
 int launch_back_tab(const DevGeom &g, const float *T, int npix, const void *sino, int sino_dtype,
                     const float *weights, const int32_t *views, int nviews, int coeff_power, int kahan, float scale,
-                    int accumulate, float *out, float *part, hipStream_t st, int *defer = nullptr);
+                    int accumulate, float *out, float *part, hipStream_t st, int *defer = nullptr,
+                    int plan_npix = 0);
 // The cylinder of table slot s is vox + s * pstride * Nz.  steps: kFwdBin
 // (bins and forward records from the table), kFwdProject (the projection
 // from bins and records already in aux), or both; kFwdPartials: the

@@ -781,11 +781,11 @@ static int dispatch_back_1(const DevGeom &g, const BackPlan &pl, const float *T,
 // Back projection of pixels whose table T (nviews x npix records) is built.
 int launch_back_tab(const DevGeom &g, const float *T, int npix, const void *sino, int sino_dtype,
                     const float *weights, const int32_t *views, int nviews, int coeff_power, int kahan, float scale,
-                    int accumulate, float *out, float *part, hipStream_t st, int *defer) {
+                    int accumulate, float *out, float *part, hipStream_t st, int *defer, int plan_npix) {
     if (defer) *defer = 0;
     if (npix == 0) return MBX_OK;
     // Kahan keeps a compensation per slice: one pixel per wave keeps it in registers
-    const BackPlan pl = plan_back(g, npix, nviews, kahan ? 1 : 2);
+    const BackPlan pl = plan_back(g, std::max(npix, plan_npix), nviews, kahan ? 1 : 2);
     if (pl.nsplit > 1 && !part) return fail(MBX_EWORKSPACE, "back projection needs a partials workspace");
     int rc;
     if (is_fan(g))
@@ -824,8 +824,9 @@ int launch_back(const DevGeom &g, const float *angles, const void *sino, int sin
         const int n = std::min(chunk, npix - c0);
         int rc = launch_table(g, angles, pix + c0, n, views, nviews, T, NC, st);
         if (rc != MBX_OK) return rc;
+        // every chunk planned as a full one: the partials were sized for it
         rc = launch_back_tab(g, T, n, sino, sino_dtype, weights, views, nviews, coeff_power, kahan, scale, accumulate,
-                             out + (size_t)c0 * g.Nz, part, st);
+                             out + (size_t)c0 * g.Nz, part, st, nullptr, chunk);
         if (rc != MBX_OK) return rc;
     }
     return MBX_OK;

@@ -495,7 +495,7 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
         rc = launch_table(d, angles, pix + c0, n, nullptr, d.V, T, NC, st);
         if (rc != MBX_OK) return rc;
         rc = launch_back_tab(d, T, n, e, e_dtype, weights, nullptr, d.V, 1, kahan, 1.0f, 0, bp + (size_t)c0 * d.Nz,
-                             bpart, st, one_table ? &bsplit : nullptr);
+                             bpart, st, one_table ? &bsplit : nullptr, L.chunk);
         if (rc != MBX_OK) return rc;
     }
     // forward projection of delta over the subset (into dsino), optionally with forward_quadratic

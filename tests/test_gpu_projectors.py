@@ -273,6 +273,33 @@ def test_chunked_full_projection_matches_parts():
     assert torch.equal(bp, bp_parts)
 
 
+def test_chunked_back_projection_small_last_chunk():
+    """A back projection over more pixels than one geometry-table chunk (128
+    views: 61681 pixels per chunk) whose last chunk is small: every chunk is
+    planned as a full one, so the view-group partials stay inside the workspace
+    sized for it (a small chunk planned on its own splits its views, which a
+    full chunk of this size does not).  The result must match the back
+    projections of the two parts, each within one chunk."""
+    from mbirjax_amd import vcd_utils
+    V, R, C = 128, 4, 416
+    m = mj.ParallelBeamModel((V, R, C), np.linspace(0, np.pi, V, endpoint=False))
+    m.set_params(recon_shape=(288, 288, 4))
+    pix = np.flatnonzero(vcd_utils.get_2d_ror_mask((288, 288, 4))).astype(np.int32)
+    chunk = 61681
+    assert chunk < pix.size < chunk + 8192, pix.size
+    print("pixels", pix.size, flush=True)
+    dev = torch.device("cuda")
+    g = torch.Generator(device=dev)
+    g.manual_seed(9)
+    e = torch.randn((V, C, R), device=dev, generator=g)
+    p = torch.as_tensor(pix, device=dev)
+    full = m._back(e, p)
+    parts = torch.cat([m._back(e, p[:chunk].contiguous()), m._back(e, p[chunk:].contiguous())])
+    d = float((full.double() - parts.double()).norm() / parts.double().norm())
+    assert d <= 1e-6, d
+    assert torch.equal(full[:chunk], parts[:chunk])  # same plan for the first chunk
+
+
 @pytest.mark.parametrize("case", ["cone_m2", "cone_bp2", "cone_psf2", "parallel", "translation"])
 def test_forward_plan_apply_matches_project(gpu, case):
     """mbx_forward_plan + mbx_forward_apply give bitwise what mbx_forward_project
