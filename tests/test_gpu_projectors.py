@@ -273,6 +273,29 @@ def test_chunked_full_projection_matches_parts():
     assert torch.equal(bp, bp_parts)
 
 
+def test_forward_channel_block_widths_agree():
+    """The channel-major forward narrows its channel blocks (32 -> 16 -> 8)
+    while a launch has fewer than 4096 waves.  Here all 512 views run with
+    32-channel blocks (512 views x 8 blocks x 1 wave) and an 8-view selection
+    with 8-channel blocks; each output element keeps its summation order, so
+    the selected views must agree bitwise, and both match the oracle."""
+    from mbirjax_amd import _lib
+    m, g = make_cone(512, 64, 256, 1024.0, 512.0)
+    rs = m.get_params('recon_shape')
+    np.random.seed(4)
+    sub = mj.gen_pixel_partition(rs, 64)[0]
+    rng = np.random.default_rng(14)
+    x = rng.standard_normal((sub.size, rs[2])).astype(np.float32)
+    views = np.array([0, 77, 128, 255, 256, 301, 444, 511], dtype=np.int32)
+    xd = torch.as_tensor(x, device='cuda')
+    pd = torch.as_tensor(sub, device='cuda')
+    full = m._fwd(xd, pd, layout=_lib.LAYOUT_VCR)
+    part = m._fwd(xd, pd, views=torch.as_tensor(views, device='cuda'), layout=_lib.LAYOUT_VCR)
+    assert torch.equal(full[torch.as_tensor(views, device='cuda').long()], part)
+    ref = op.forward(g, x, sub, views=views)  # (views, R, C)
+    assert rel(part.permute(0, 2, 1), ref) <= TOL
+
+
 def test_chunked_back_projection_small_last_chunk():
     """A back projection over more pixels than one geometry-table chunk (128
     views: 61681 pixels per chunk) whose last chunk is small: every chunk is
