@@ -19,11 +19,41 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "mbx_internal.hpp"
 
 namespace mbx {
+
+// A second stream per device (default priority) that runs the forward binning
+// of a subset while the back projection runs on the caller's stream, with the
+// two events of the fork and the join.  Created on first use; the mutex
+// serialises the enqueueing of concurrent calls, which share the events.
+struct SideStream {
+    std::mutex mu;
+    bool tried = false;
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+static SideStream g_side[64];
+
+static SideStream *side_stream() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    SideStream &ss = g_side[dev];
+    std::lock_guard<std::mutex> lk(ss.mu);
+    if (!ss.tried) {
+        ss.tried = true;
+        if (hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ss.join, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            ss.s = nullptr;  // then the binning stays on the caller's stream
+        }
+    }
+    return ss.s ? &ss : nullptr;
+}
 
 constexpr float kEps32 = 1.1920928955078125e-07f;  // jnp.finfo(jnp.float32).eps
 constexpr int kRedBlocks = 1024;
@@ -488,12 +518,26 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
     const bool one_table = npix <= L.chunk;  // build the table once for all projections
 
     // (c) back projection of the weighted error sinogram (with one table, the
-    // delta kernel sums its view-group partials)
+    // delta kernel sums its view-group partials, and the forward binning runs
+    // on the side stream meanwhile when it is long enough to pay for the fork
+    // and join: 2^18 (view, pixel) records or more)
+    const bool side_bin = one_table && (int64_t)npix * d.V >= (1 << 18);
     int bsplit = 0;
+    SideStream *side = nullptr;
+    std::unique_lock<std::mutex> side_lock;
     for (int c0 = 0; c0 < npix; c0 += L.chunk) {
         const int n = std::min(L.chunk, npix - c0);
         rc = launch_table(d, angles, pix + c0, n, nullptr, d.V, T, NC, st);
         if (rc != MBX_OK) return rc;
+        if (side_bin && (side = side_stream()) != nullptr) {
+            side_lock = std::unique_lock<std::mutex>(side->mu);
+            MBX_HIP_CHECK(hipEventRecord(side->fork, st));
+            MBX_HIP_CHECK(hipStreamWaitEvent(side->s, side->fork, 0));
+            rc = launch_fwd_tab(d, T, NC, delta, n, d.V, dsino, delta_dtype, MBX_LAYOUT_VCR, 0, weights, fpart,
+                                L.chunk, nullptr, side->s, 1, kFwdBin);
+            if (rc != MBX_OK) return rc;
+            MBX_HIP_CHECK(hipEventRecord(side->join, side->s));
+        }
         rc = launch_back_tab(d, T, n, e, e_dtype, weights, nullptr, d.V, 1, kahan, 1.0f, 0, bp + (size_t)c0 * d.Nz,
                              bpart, st, one_table ? &bsplit : nullptr, L.chunk);
         if (rc != MBX_OK) return rc;
@@ -537,6 +581,11 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
                            (float)fm_constant, delta, dpart);
     }
     MBX_LAUNCH_CHECK();
+    if (side) {  // join: the bins and records are in fpart
+        MBX_HIP_CHECK(hipStreamWaitEvent(st, side->join, 0));
+        side_lock.unlock();
+        binned = true;
+    }
     // (g,h) delta sinogram and forward_quadratic
     rc = project_delta(true, weights);
     if (rc != MBX_OK) return rc;
