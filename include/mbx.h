@@ -203,7 +203,11 @@ int mbx_sino_axpy(void *e, int32_t e_dtype, const void *d, int32_t d_dtype, cons
  * prox_input: NULL = qGGMRF prior, else (rows*cols, Nz) f32 proximal-map input
  * with prior Hessian sigma_prox^2.  delta_dtype: storage of the delta sinogram.
  * stats (device double[4]) accumulates {sum |alpha*delta|, alpha, 1, 0} for
- * per-iteration statistics.  workspace: mbx_vcd_workspace_bytes() bytes. */
+ * per-iteration statistics.  workspace: mbx_vcd_workspace_bytes() bytes.
+ * Subsets of 2^18 (view, pixel) pairs or more bin the forward records on a
+ * second stream of the current device (default priority, created on first use
+ * with two events and kept until the process ends), forked from and joined
+ * back into `stream` inside the call: the call stays stream-ordered. */
 int64_t mbx_vcd_workspace_bytes(const mbx_geom *g, int32_t npix, int32_t delta_dtype);
 int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_recon, void *e, int32_t e_dtype,
                           const float *weights, const float *fm_hessian, const int32_t *pix, int32_t npix,
