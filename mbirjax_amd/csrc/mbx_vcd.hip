@@ -38,16 +38,26 @@ struct SideStream {
 };
 static SideStream g_side[64];
 
-static SideStream *side_stream() {
+// The side stream of the device that runs st (the current device for the
+// null stream), or NULL when there is none.
+static SideStream *side_stream(hipStream_t st) {
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    const hipError_t e = st ? hipStreamGetDevice(st, &dev) : hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
     SideStream &ss = g_side[dev];
     std::lock_guard<std::mutex> lk(ss.mu);
     if (!ss.tried) {
         ss.tried = true;
-        if (hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&ss.join, hipEventDisableTiming) != hipSuccess) {
+        int cur = dev;
+        bool ok = hipGetDevice(&cur) == hipSuccess && (cur == dev || hipSetDevice(dev) == hipSuccess);
+        ok = ok && hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&ss.join, hipEventDisableTiming) == hipSuccess;
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (!ok) {
             (void)hipGetLastError();
             ss.s = nullptr;  // then the binning stays on the caller's stream
         }
@@ -529,7 +539,7 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
         const int n = std::min(L.chunk, npix - c0);
         rc = launch_table(d, angles, pix + c0, n, nullptr, d.V, T, NC, st);
         if (rc != MBX_OK) return rc;
-        if (side_bin && (side = side_stream()) != nullptr) {
+        if (side_bin && (side = side_stream(st)) != nullptr) {
             side_lock = std::unique_lock<std::mutex>(side->mu);
             MBX_HIP_CHECK(hipEventRecord(side->fork, st));
             MBX_HIP_CHECK(hipStreamWaitEvent(side->s, side->fork, 0));
