@@ -61,7 +61,8 @@ int64_t back_part_bytes(const DevGeom &g, int chunk, int nviews);  // view-group
 int64_t back_workspace_bytes(const DevGeom &g, int npix, int nviews);
 
 struct FwdPlan {
-    int nw;       // waves per block (one 64-row slab each)
+    int nw;       // waves per block (one slab of 64 * rpl rows each)
+    int rpl;      // detector rows per lane (64 apart)
     int rows_pb;  // detector rows per block
     int nrb;      // row blocks
     int cb;       // channels per block

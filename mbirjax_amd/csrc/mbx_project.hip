@@ -841,7 +841,8 @@ constexpr int kFwdCBW = 32;       // channels per block, (views, channels, rows)
 constexpr int kFwdCBWMin = 8;     // ... narrowed down to this on small launches
 constexpr int64_t kFwdMinWaves = 4096;  // four waves per SIMD
 constexpr int kFwdRPL = 2;        // detector rows per lane (64 apart), (views, rows, channels) output
-constexpr int kFwdRPLW = 4;       // detector rows per lane, (views, channels, rows) output
+constexpr int kFwdRPLW = 4;       // detector rows per lane, (views, channels, rows) output (2 on
+                                  // detectors of at most 128 rows)
 constexpr int kFwdPad = 8;        // zero slices either side of a staged cylinder (>= 2*bp_psf_radius + 1;
                                   // a multiple of 4 keeps every staged cylinder 16-byte aligned)
 constexpr int kFwdMaxBatch = 16;  // cylinders staged per batch
@@ -851,7 +852,8 @@ constexpr int kBinMax = 16384;    // channel buckets the binning kernel holds in
 
 FwdPlan plan_fwd(const DevGeom &g, int nviews, int layout) {
     FwdPlan p;
-    const int slab = kWave * (layout == MBX_LAYOUT_VCR ? kFwdRPLW : kFwdRPL);  // rows per wave
+    p.rpl = layout == MBX_LAYOUT_VCR ? (g.R <= 2 * kWave ? 2 : kFwdRPLW) : kFwdRPL;
+    const int slab = kWave * p.rpl;  // rows per wave
     p.nw = (std::min(g.R, 4 * slab) + slab - 1) / slab;
     p.rows_pb = p.nw * slab;
     p.nrb = (g.R + p.rows_pb - 1) / p.rows_pb;
@@ -1151,7 +1153,7 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
 // puts every tap in a fixed register) and the tile written once.
 // Each output element has one writer and a fixed summation order (the list
 // order of its candidates, the same in both schemes).
-template <int GEOM, int RAD, int RB, int VM, typename OT, int LAYOUT>
+template <int GEOM, int RAD, int RB, int VM, typename OT, int LAYOUT, int RPLW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void fwd_kernel(
     DevGeom g, const float *__restrict__ T, const float *__restrict__ FR, const int *__restrict__ list,
     const int *__restrict__ cstart, const float *__restrict__ vox, int64_t vstride, int npix, int ncb, int nrb,
@@ -1159,7 +1161,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     double *__restrict__ part_sumsq) {
     // (views, channels, rows) output: a sliding window of 2*RAD+1 channels per row
     constexpr bool SW = LAYOUT == MBX_LAYOUT_VCR;
-    constexpr int CB = kFwdCB, P = kFwdPad, RPL = SW ? kFwdRPLW : kFwdRPL;
+    constexpr int CB = kFwdCB, P = kFwdPad, RPL = SW ? RPLW : kFwdRPL;
+    static_assert(RPL % 2 == 0, "rows per lane cover whole 128-row groups of the record masks");
     // channels per block: the register tile's CB, or the window's cbw (at most
     // kFwdCBW, so that the bucket starts fit one per lane)
     const int cbs = SW ? cbw : CB;
@@ -1562,10 +1565,18 @@ template <int GEOM, int RAD, int RB, int VM, typename OT, int LAYOUT>
 static void launch_fwd_t(const DevGeom &g, const FwdPlan &pl, const float *T, const float *FR, const int *list,
                          const int *cstart, const float *vox, int64_t vstride, int npix, OT *sino, int accumulate,
                          const float *w, double *part, hipStream_t st) {
-    hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, VM, OT, LAYOUT>), dim3((unsigned)pl.nblocks), dim3(64 * pl.nw),
-                       pl.lds, st, g, T, FR, list, cstart, vox, vstride, npix, pl.ncb, pl.nrb, pl.cb, pl.nb,
-                       pl.cstride,
-                       sino, accumulate, w, part);
+    // rows per lane of the window kernel: the plan's (the row-major kernel has its own)
+    if constexpr (LAYOUT == MBX_LAYOUT_VCR) {
+        if (pl.rpl == 2) {
+            hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, VM, OT, LAYOUT, 2>), dim3((unsigned)pl.nblocks),
+                               dim3(64 * pl.nw), pl.lds, st, g, T, FR, list, cstart, vox, vstride, npix, pl.ncb,
+                               pl.nrb, pl.cb, pl.nb, pl.cstride, sino, accumulate, w, part);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, VM, OT, LAYOUT, kFwdRPLW>), dim3((unsigned)pl.nblocks),
+                       dim3(64 * pl.nw), pl.lds, st, g, T, FR, list, cstart, vox, vstride, npix, pl.ncb, pl.nrb, pl.cb,
+                       pl.nb, pl.cstride, sino, accumulate, w, part);
 }
 
 template <int GEOM, int RAD, int RB, int VM>
