@@ -871,10 +871,19 @@ FwdPlan plan_fwd(const DevGeom &g, int nviews, int layout) {
     // slices) in whole 64-slice chunks, and the zero pads.
     const int span_max = is_fan(g) ? g.Nz : p.rows_pb;
     p.cstride = ((span_max + kWave - 1) / kWave) * kWave + 2 * kFwdPad + 8;  // a multiple of 4
-    const size_t budget = layout == MBX_LAYOUT_VCR ? kFwdLdsW : kFwdLds;
-    p.nb = (int)std::max<size_t>(1, std::min<size_t>(kFwdMaxBatch, budget / (2 * (size_t)p.cstride * sizeof(float))));
-    p.lds = 2 * (size_t)p.nb * p.cstride * sizeof(float) + 64 * sizeof(int);
     p.nblocks = (int64_t)nviews * p.ncb * p.nrb;
+    size_t budget = layout == MBX_LAYOUT_VCR ? kFwdLdsW : kFwdLds;
+    // Launches of few waves (at most two rounds of kFwdMinWaves): a smaller
+    // staging batch if that lets every block be resident at once (160 KB of
+    // LDS per CU, 1 KB of allocation slack per block), down to 4 slots.
+    const size_t slot_bytes = 2 * (size_t)p.cstride * sizeof(float);
+    if (p.nblocks * p.nw <= 2 * kFwdMinWaves) {
+        const int64_t per_cu = (p.nblocks + 255) / 256;
+        const size_t fit = (size_t)(160 * 1024 / per_cu) - 1024 - 64 * sizeof(int);
+        if (fit < budget && fit >= 4 * slot_bytes) budget = fit;
+    }
+    p.nb = (int)std::max<size_t>(1, std::min<size_t>(kFwdMaxBatch, budget / slot_bytes));
+    p.lds = 2 * (size_t)p.nb * p.cstride * sizeof(float) + 64 * sizeof(int);
     return p;
 }
 
