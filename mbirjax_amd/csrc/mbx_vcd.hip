@@ -535,6 +535,16 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
     int bsplit = 0;
     SideStream *side = nullptr;
     std::unique_lock<std::mutex> side_lock;
+    // the caller's stream waits for the side stream's work on every return
+    // path once it is forked, so nothing of this call runs past its end
+    struct Join {
+        SideStream *&side;
+        hipStream_t st;
+        bool forked = false;
+        ~Join() {
+            if (forked) (void)hipStreamWaitEvent(st, side->join, 0);
+        }
+    } join{side, st};
     for (int c0 = 0; c0 < npix; c0 += L.chunk) {
         const int n = std::min(L.chunk, npix - c0);
         rc = launch_table(d, angles, pix + c0, n, nullptr, d.V, T, NC, st);
@@ -547,6 +557,7 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
                                 L.chunk, nullptr, side->s, 1, kFwdBin);
             if (rc != MBX_OK) return rc;
             MBX_HIP_CHECK(hipEventRecord(side->join, side->s));
+            join.forked = true;
         }
         rc = launch_back_tab(d, T, n, e, e_dtype, weights, nullptr, d.V, 1, kahan, 1.0f, 0, bp + (size_t)c0 * d.Nz,
                              bpart, st, one_table ? &bsplit : nullptr, L.chunk);
@@ -591,7 +602,8 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
                            (float)fm_constant, delta, dpart);
     }
     MBX_LAUNCH_CHECK();
-    if (side) {  // join: the bins and records are in fpart
+    if (join.forked) {  // join: the bins and records are in fpart
+        join.forked = false;
         MBX_HIP_CHECK(hipStreamWaitEvent(st, side->join, 0));
         side_lock.unlock();
         binned = true;
