@@ -842,7 +842,7 @@ constexpr int kFwdCBWMin = 8;     // ... narrowed down to this on small launches
 constexpr int64_t kFwdMinWaves = 4096;  // four waves per SIMD
 constexpr int kFwdRPL = 2;        // detector rows per lane (64 apart), (views, rows, channels) output
 constexpr int kFwdRPLW = 4;       // detector rows per lane, (views, channels, rows) output (2 on
-                                  // detectors of at most 128 rows)
+                                  // detectors of at most 256 rows)
 constexpr int kFwdPad = 8;        // zero slices either side of a staged cylinder (>= 2*bp_psf_radius + 1;
                                   // a multiple of 4 keeps every staged cylinder 16-byte aligned)
 constexpr int kFwdMaxBatch = 16;  // cylinders staged per batch
@@ -852,7 +852,7 @@ constexpr int kBinMax = 16384;    // channel buckets the binning kernel holds in
 
 FwdPlan plan_fwd(const DevGeom &g, int nviews, int layout) {
     FwdPlan p;
-    p.rpl = layout == MBX_LAYOUT_VCR ? (g.R <= 2 * kWave ? 2 : kFwdRPLW) : kFwdRPL;
+    p.rpl = layout == MBX_LAYOUT_VCR ? (g.R <= 4 * kWave ? 2 : kFwdRPLW) : kFwdRPL;
     const int slab = kWave * p.rpl;  // rows per wave
     p.nw = (std::min(g.R, 4 * slab) + slab - 1) / slab;
     p.rows_pb = p.nw * slab;
