@@ -3,11 +3,15 @@
 // mbx_vcd_subset_update restates vcd_subset_updater
 // (mbirjax/tomography_model.py:1343-1542) as a fixed kernel chain on one
 // stream with no host synchronisation:
-//   back projection of W*e            -> forward gradient (before -1/sigma_y^2)
-//   qGGMRF (or prox) gradient/Hessian -> prior terms
+//   geometry table of the subset
+//   back projection of W*e            -> forward gradient (before -1/sigma_y^2),
+//                                        as view-group partials
+//   [forward binning, on a side stream meanwhile for large subsets]
+//   delta kernel: sum of the partials, qGGMRF (or prox) gradient/Hessian,
 //   delta = -(fg + pg)/(fh + ph)      -> with FP64 partials of prior_linear,
 //                                        prior_quadratic and forward_linear
-//   forward projection of delta       -> delta sinogram, FP64 forward_quadratic
+//   forward projection of delta       -> delta sinogram, FP64 partials of
+//                                        forward_quadratic
 //   alpha = clip((fl - pl)/(fq + pq + eps32), eps32, 1.5)   (on the device)
 //   [positivity clamp + re-projection]
 //   x[S] += alpha*delta ; e -= alpha*delta_sino
