@@ -294,3 +294,33 @@ def test_c2_cone_recon_vs_oracle(gpu):
     print("C2 3-iteration recon NRMSE vs oracle %.3e, vs phantom %.3f" % (
         nrmse, np.linalg.norm(r - phantom) / np.linalg.norm(phantom)), flush=True)
     assert nrmse <= 1e-3
+
+
+def test_recon_on_a_side_torch_stream(gpu):
+    """A recon issued on a non-default torch stream equals the default-stream
+    one bitwise.  The first iteration's single subset (the whole region of
+    reconstruction over 64 views) has more than 2^18 view-pixel records, so the
+    subset update forks its forward binning onto the library's second stream of
+    the device that runs the caller's stream, and joins it back."""
+    V, R, C = 64, 64, 96
+    ang = np.linspace(0, 2 * np.pi, V, endpoint=False)
+    m = mj.ConeBeamModel((V, R, C), ang, source_detector_dist=4.0 * C, source_iso_dist=2.0 * C)
+    m.set_params(sharpness=0.0, verbose=0)
+    rs = m.get_params('recon_shape')
+    assert int(opart.get_2d_ror_mask(rs).sum()) * V >= 1 << 18
+    phantom = mj.utilities.gen_random_ellipsoid_phantom(rs, seed=3)
+    sino = m.forward_project(phantom).cpu().numpy()
+
+    def arr(r):
+        return r.detach().cpu().numpy() if torch.is_tensor(r) else np.asarray(r)
+
+    np.random.seed(5)
+    r1, _ = m.recon(sino, max_iterations=3, stop_threshold_change_pct=0, print_logs=False, logfile_path=None)
+    r1 = arr(r1)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        np.random.seed(5)
+        r2, _ = m.recon(sino, max_iterations=3, stop_threshold_change_pct=0, print_logs=False, logfile_path=None)
+        r2 = arr(r2)
+    torch.cuda.synchronize()
+    assert np.array_equal(r1, r2)
