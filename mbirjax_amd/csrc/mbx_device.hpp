@@ -171,8 +171,8 @@ struct DevGeom {
     mbx_geom h; // the caller's double-precision geometry (table kernel)
 };
 
-// Sum of v over the block (blockDim.x a multiple of 64, at most 8 waves),
-// in a fixed order; valid in thread 0.  scratch: 8 elements of shared memory.
+// Sum of v over the block (blockDim.x a multiple of 64), in a fixed order;
+// valid in thread 0.  scratch: one element of shared memory per wave.
 template <typename T>
 __device__ __forceinline__ T block_reduce(T v, T *scratch) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);

@@ -311,10 +311,10 @@ __global__ __launch_bounds__(256) void delta_kernel(const float *__restrict__ bp
 // fq_raw = sum of delta_sino^2 * w, from the forward projection's nfq
 // per-block partials fqpart (kept in scal[0]).
 // alpha = clip((fl - pl) / (fq + pq + eps32), eps32, 1.5)   (tomography_model.py:1467-1471)
-__global__ __launch_bounds__(256) void alpha_kernel(const double *__restrict__ part, int nblk,
+__global__ __launch_bounds__(1024) void alpha_kernel(const double *__restrict__ part, int nblk,
                                                     const double *__restrict__ fqpart, int64_t nfq, double fmc,
                                                     double *__restrict__ scal, double *__restrict__ stats) {
-    __shared__ double scratch[8];
+    __shared__ double scratch[16];  // one slot per wave of the 1024-thread block
     double a = 0.0, b = 0.0, c = 0.0, f = 0.0;
     for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
         a += part[i];
@@ -616,7 +616,7 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
     rc = project_delta(true, weights);
     if (rc != MBX_OK) return rc;
     // (i) step size on the device
-    hipLaunchKernelGGL(alpha_kernel, dim3(1), dim3(256), 0, st, dpart, nb, (const double *)fpart,
+    hipLaunchKernelGGL(alpha_kernel, dim3(1), dim3(1024), 0, st, dpart, nb, (const double *)fpart,
                        fwd_blocks(d, d.V, MBX_LAYOUT_VCR), fm_constant, scal, stats);
     MBX_LAUNCH_CHECK();
     // (j) positivity
