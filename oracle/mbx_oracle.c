@@ -50,7 +50,31 @@ typedef struct {
 
 typedef REAL real;
 
+/* Planted-mistake switch, for the tests that show the geometry pins catch
+ * plausible restatement mistakes (tests/test_oracle_geometry.py).  0 is the
+ * faithful restatement and the only value any other caller uses; each other
+ * value plants one mistake in the shared helpers, so forward and back stay
+ * adjoint and only the pins against the geometry can see it:
+ *   1 the 1/cos(phi) path factor dropped
+ *   2 both detector offsets entering with the opposite sign
+ *   3 the vertical footprint width W_r = dv/dr, without the magnification
+ *   4 recon_slice_offset entering with the opposite sign
+ *   5 cos(alpha) formed from theta + theta_p
+ *   6 the 1/cos(theta_p) factor of W_c dropped
+ *   7 the pixel magnification 1/(1/M + y/sdd)
+ *   8 the view rotation by -theta */
+static int g_mutation = 0;
+void oracle_set_mutation(int m) { g_mutation = m; }
+int oracle_get_mutation(void) { return g_mutation; }
+
 static real clipr(real x, real lo, real hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+/* detector offsets and the slice offset as they enter the geometry */
+static real chan_offset(const ogeom *g) { return (real)(g_mutation == 2 ? -g->det_channel_offset : g->det_channel_offset); }
+static real row_offset(const ogeom *g) { return (real)(g_mutation == 2 ? -g->det_row_offset : g->det_row_offset); }
+static real slice_offset(const ogeom *g) {
+    return (real)(g_mutation == 4 ? -g->recon_slice_offset : g->recon_slice_offset);
+}
 
 /* Horizontal data for pixel p at view parameters vp (angle, or translation
  * vector for kind 2): cone_beam.py:593-644, parallel_beam.py:223-263,
@@ -78,18 +102,19 @@ static void horizontal(const ogeom *g, int p, const double *vp, real *n_p, int *
     const real th = (real)vp[0];
     const real yt = dv * ((real)i - (real)(rows - 1) / 2);
     const real xt = dv * ((real)j - (real)(cols - 1) / 2);
-    const real c = cos(th), s = sin(th);
+    const real c = cos(th), s = g_mutation == 8 ? -sin(th) : sin(th);
     const real x = c * xt - s * yt;
     const real y = s * xt + c * yt;
     const real center = (real)(g->num_det_channels - 1) / 2;
     if (g->kind == 1) {
         const real sdd = (real)g->source_detector_dist;
-        const real mag = 1 / (1 / (real)g->magnification - y / sdd);
+        const real mag = 1 / (1 / (real)g->magnification + (g_mutation == 7 ? y : -y) / sdd);
         const real u = mag * x;
-        *n_p = (u + (real)g->det_channel_offset) / (real)g->delta_det_channel + center;
+        *n_p = (u + chan_offset(g)) / (real)g->delta_det_channel + center;
         const real theta_p = atan2(u, sdd);
-        *cos_a = fmax(fabs(cos(th - theta_p)), fabs(sin(th - theta_p)));
-        *W_c = mag * (dv / (real)g->delta_det_channel) * (*cos_a / cos(theta_p));
+        const real ta = g_mutation == 5 ? th + theta_p : th - theta_p;
+        *cos_a = fmax(fabs(cos(ta)), fabs(sin(ta)));
+        *W_c = mag * (dv / (real)g->delta_det_channel) * (*cos_a / (g_mutation == 6 ? 1 : cos(theta_p)));
         *mag_out = mag;
     } else {
         *n_p = (x + (real)g->det_channel_offset) / (real)g->delta_det_channel + center;
@@ -113,19 +138,25 @@ static real chan_w(const ogeom *g, real n_p, int n, real W_c, real cos_a) {
  * -t_z, no slice offset).  tz is the view's z translation (0 for the cone). */
 static real slice_z(const ogeom *g, int k, real tz) {
     const real z0 = (real)g->delta_voxel * ((real)k - (real)(g->num_recon_slices - 1) / 2);
-    return g->kind == 2 ? z0 - tz : z0 + (real)g->recon_slice_offset;
+    return g->kind == 2 ? z0 - tz : z0 + slice_offset(g);
 }
 
 /* detector row position of slice k of a pixel with magnification mag
  * (detector_uv_to_mn, cone_beam.py:684-707) */
 static real slice_row(const ogeom *g, real mag, int k, real tz) {
     const real v = mag * slice_z(g, k, tz);
-    return (v + (real)g->det_row_offset) / (real)g->delta_det_row + (real)(g->num_det_rows - 1) / 2;
+    return (v + row_offset(g)) / (real)g->delta_det_row + (real)(g->num_det_rows - 1) / 2;
 }
 
 static real inv_cos_phi(const ogeom *g, real mag, int k, real tz) {
+    if (g_mutation == 1) return 1;
     const real v = mag * slice_z(g, k, tz);
     return 1 / cos(atan2(v, (real)g->source_detector_dist));
+}
+
+/* vertical footprint width W_r = mag dv / dr (cone_beam.py:585-586) */
+static real vert_width(const ogeom *g, real mag) {
+    return (g_mutation == 3 ? 1 : mag) * (real)g->delta_voxel / (real)g->delta_det_row;
 }
 
 /* per-view parameter block: one angle, or a translation vector */
@@ -155,12 +186,12 @@ void oracle_forward(const ogeom *g, const double *vparams, const REAL *vox, cons
                  * slices k in round(k_m) +- bp_psf_radius (cone_beam.py:347-372), or
                  * +- psf_radius for the translation geometry (translation_model.py:348-377) */
                 const int rk = g->kind == 2 ? g->psf_radius : g->bp_psf_radius;
-                const real W = mag * (real)g->delta_voxel / (real)g->delta_det_row;
+                const real W = vert_width(g, mag);
                 const real Lmax = fmin((real)1, W);
                 for (int m = 0; m < R; ++m) {
-                    const real v_m = ((real)m - (real)(R - 1) / 2) * (real)g->delta_det_row - (real)g->det_row_offset;
+                    const real v_m = ((real)m - (real)(R - 1) / 2) * (real)g->delta_det_row - row_offset(g);
                     const real z_m = v_m / mag;
-                    const real zs = g->kind == 2 ? z_m + tz : z_m - (real)g->recon_slice_offset;
+                    const real zs = g->kind == 2 ? z_m + tz : z_m - slice_offset(g);
                     const real k_m = zs / (real)g->delta_voxel + (real)(Nz - 1) / 2;
                     const int kc = (int)rint(k_m);
                     real acc = 0;
@@ -216,7 +247,7 @@ void oracle_back(const ogeom *g, const double *vparams, const REAL *sino, const 
             }
             if (g->kind != 0) {
                 /* back_vertical_fan: voxel-driven, rows round(m_p) +- psf_radius */
-                const real W = mag * (real)g->delta_voxel / (real)g->delta_det_row;
+                const real W = vert_width(g, mag);
                 const real Lmax = fmin((real)1, W);
                 for (int k = 0; k < Nz; ++k) {
                     const real m_p = slice_row(g, mag, k, tz);

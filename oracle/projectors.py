@@ -13,9 +13,18 @@ and a restatement of the geometry helpers:
 Parity pins (tests/test_oracle.py): adjoint identity to ~1e-12, Hessian ==
 squared column norms of a brute-force dense A, parallel-beam closed forms at
 0 and 90 degrees, view-subset additivity, cone with sdd -> inf equal to
-parallel beam.  Projector values are not pinned by reference golden vectors
-(the reference holds none; JAX cannot run here), only by these invariants and
-closed forms.
+parallel beam.  Cone-beam geometry pins (tests/test_oracle_geometry.py), from
+the physical scanner and not from the projector's formulas: the footprint
+centroid equals the pinhole image of the voxel centre, the plateau equals the
+chord of the central ray through the voxel, the footprint mass equals the
+voxel volume times the Jacobian of the central projection (both to the
+separable model's second-order gap), and offset shifts move the image by
+whole cells; each of eight planted restatement mistakes (1/cos(phi), offset
+signs, magnification in W_r, slice-offset sign, cos(alpha), 1/cos(theta_p),
+magnification sign, rotation sense) fails at least one of them.  The
+reference's own FBP/FDK accuracy test (tests/test_fbp_fdk.py) passes at its
+tolerances.  Projector values are not pinned by reference golden vectors (the
+reference holds none; JAX cannot run here).
 """
 import ctypes
 import math
@@ -64,8 +73,29 @@ def _lib(real):
         p = ctypes.c_void_p
         lib.oracle_forward.argtypes = [ctypes.POINTER(OGeom), p, p, p, ctypes.c_int, p, ctypes.c_int, p]
         lib.oracle_back.argtypes = [ctypes.POINTER(OGeom), p, p, p, ctypes.c_int, p, ctypes.c_int, ctypes.c_int, p]
+        lib.oracle_set_mutation.argtypes = [ctypes.c_int]
+        lib.oracle_get_mutation.restype = ctypes.c_int
         _libs[key] = lib
     return _libs[key]
+
+
+class planted_mistake:
+    """Context manager that plants one numbered restatement mistake in the
+    FP64 oracle (see the list in mbx_oracle.c) and removes it on exit.  Only
+    the tests that show the geometry pins catch such mistakes use it."""
+
+    def __init__(self, number):
+        self.number = int(number)
+
+    def __enter__(self):
+        lib = _lib(np.float64)
+        assert lib.oracle_get_mutation() == 0
+        lib.oracle_set_mutation(self.number)
+        return self
+
+    def __exit__(self, *exc):
+        _lib(np.float64).oracle_set_mutation(0)
+        return False
 
 
 class Geometry:

@@ -21,7 +21,14 @@ forward_project/back_project, then one permutation per iteration).
 Pinned by (tests/test_oracle.py): the invariant e == y - A x after every
 subset, alpha in [eps32, 1.5], the prox fixed point (tests/test_prox.py:31-61)
 and the phantom tolerances of tests/test_vcd.py:21-22 at the reference's
-test geometry.
+test geometry; and (tests/test_oracle_vcd.py) with the quadratic prior
+(p = q = 2): the limit equals the direct solution of the MAP normal equations
+(A^T W A / sigma_y^2 + L) x = A^T W y / sigma_y^2, weighted and unweighted, the
+positivity limit equals the non-negative least-squares minimiser, and one
+iteration equals a dense-matrix replay (delta, alpha, recon, error sinogram);
+six planted mistakes of the update (PLANTED below) each fail one of these.
+The over-relaxation factor of the prior's quadratic term is pinned only by
+restatement.
 """
 import numpy as np
 
@@ -34,12 +41,26 @@ DEFAULTS = dict(sigma_y=1.0, sigma_x=1.0, sigma_prox=1.0, p=2.0, q=1.2, T=1.0, q
                 granularity=[1, 2, 4, 8, 16, 32, 64, 128, 256], partition_sequence=[0, 2, 4, 6, 7])
 
 
+# Planted restatement mistakes of the subset update, for the tests that show
+# the VCD pins catch them (tests/test_oracle_vcd.py); OracleModel.planted is
+# None everywhere else.
+PLANTED = {
+    "prior_sign": "prior gradient with the opposite sign",
+    "fm_constant_sigma": "forward-model constant 1/sigma_y instead of 1/sigma_y^2",
+    "no_overrelaxation": "prior quadratic term without the over-relaxation factor 1/2",
+    "error_update_sign": "error sinogram updated by + alpha A delta",
+    "positivity_no_alpha": "positivity clip delta >= -x instead of -x/alpha",
+    "hessian_unweighted": "forward Hessian diagonal without the weights",
+}
+
+
 class OracleModel:
     def __init__(self, geom, **params):
         self.geom = geom
         self.params = dict(DEFAULTS)
         self.params.update(params)
         self.check_invariant = False
+        self.planted = None
 
     # ---------------------------------------------------------- projections
     def forward_project(self, recon):
@@ -137,11 +158,12 @@ class OracleModel:
         alpha = np.sum(we * sinogram) / nrm if nrm > 0 else 1.0
         e = sinogram - alpha * e
         flat = (alpha * init_recon).reshape(-1, nz)
-        fm_hess = self.hessian_diagonal(None if const_w else w).reshape(-1, nz)
+        fm_hess = self.hessian_diagonal(None if const_w or self.planted == "hessian_unweighted"
+                                        else w).reshape(-1, nz)
         if prox_input is not None:
             prox_input = np.asarray(prox_input, np.float64).reshape(-1, nz)
         sigma_y = float(self.params["sigma_y"])
-        fmc = 1.0 / sigma_y ** 2
+        fmc = 1.0 / sigma_y ** (1 if self.planted == "fm_constant_sigma" else 2)
         qp = self.qparams()
         sp = float(self.params["sigma_prox"])
         self.y = sinogram
@@ -172,23 +194,26 @@ class OracleModel:
         else:
             ph = sigma_prox ** 2
             pg = qggmrf.prox_gradient_at_indices(flat, prox_input, S, sigma_prox)
+        if self.planted == "prior_sign":
+            pg = -pg
         we = e if const_w else w * e
         fg = -fmc * projectors.back(self.geom, we, S)
         fh = fmc * fm_hess[S]
         delta = -((fg + pg) / (fh + ph))
         pl = np.sum(pg * delta)
-        pq = 0.5 * np.sum(ph * delta ** 2)
+        pq = (1.0 if self.planted == "no_overrelaxation" else 0.5) * np.sum(ph * delta ** 2)
         ds = projectors.forward(self.geom, delta, S)
         fl = fmc * np.sum(we * ds)
         fq = fmc * np.sum(ds * ds * w)
         alpha = (fl - pl) / (fq + pq + EPS32)
         alpha = float(np.clip(alpha, EPS32, 1.5))
         if self.params["positivity_flag"]:
-            delta = np.maximum(-(1.0 / (alpha + EPS32)) * flat[S], delta)
+            pos = 1.0 if self.planted == "positivity_no_alpha" else 1.0 / (alpha + EPS32)
+            delta = np.maximum(-pos * flat[S], delta)
             ds = projectors.forward(self.geom, delta, S)
         flat = flat.copy()
         flat[S] += alpha * delta
-        e = e - alpha * ds
+        e = e + alpha * ds if self.planted == "error_update_sign" else e - alpha * ds
         if self.check_invariant:
             idx = np.flatnonzero(partitions.get_2d_ror_mask(self.geom.recon_shape))
             resid = self.y - projectors.forward(self.geom, flat[idx], idx)
