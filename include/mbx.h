@@ -151,13 +151,17 @@ int mbx_forward_project(const mbx_geom *g, const float *angles, const float *vox
  * one geometry-table chunk (about 512 MB of table: 10965 pixels at 720 views),
  * else MBX_EUNSUPPORTED.  mbx_forward_apply then projects cylinders vox of
  * those pixels exactly as mbx_forward_project would (same arguments; views
- * only fixes the view count).  The plan stays valid while the geometry,
- * angles, pix and views are unchanged; applies on one stream may share it. */
+ * only fixes the view count); plan_bytes must equal
+ * mbx_forward_workspace_bytes(g, npix, nviews) for the npix and nviews the
+ * plan was built with, else MBX_EINVAL (so an apply of more pixels or views
+ * than planned cannot read past the plan).  The plan stays valid while the
+ * geometry, angles, pix and views are unchanged; applies on one stream may
+ * share it. */
 int mbx_forward_plan(const mbx_geom *g, const float *angles, const int32_t *pix, int32_t npix, const int32_t *views,
                      int32_t nviews, void *plan, int64_t plan_bytes, void *stream);
-int mbx_forward_apply(const mbx_geom *g, const void *plan, const float *vox, int32_t npix, const int32_t *views,
-                      int32_t nviews, void *sino, int32_t sino_dtype, int32_t sino_layout, int32_t accumulate,
-                      const float *weights, double *sumsq, void *stream);
+int mbx_forward_apply(const mbx_geom *g, const void *plan, int64_t plan_bytes, const float *vox, int32_t npix,
+                      const int32_t *views, int32_t nviews, void *sino, int32_t sino_dtype, int32_t sino_layout,
+                      int32_t accumulate, const float *weights, double *sumsq, void *stream);
 
 /* Sparse back projection (mbirjax/projectors.py:131-209, tomography_model.py:624-677).
  * sino: (num_sino_views, C, R) in sino_dtype, MBX_LAYOUT_VCR.  views: NULL =
@@ -205,9 +209,13 @@ int mbx_sino_axpy(void *e, int32_t e_dtype, const void *d, int32_t d_dtype, cons
  * stats (device double[4]) accumulates {sum |alpha*delta|, alpha, 1, 0} for
  * per-iteration statistics.  workspace: mbx_vcd_workspace_bytes() bytes.
  * Subsets of 2^18 (view, pixel) pairs or more bin the forward records on a
- * second stream of the current device (default priority, created on first use
- * with two events and kept until the process ends), forked from and joined
- * back into `stream` inside the call: the call stays stream-ordered. */
+ * second stream of the device that runs `stream` (the current device for the
+ * NULL stream; default priority, created on first use with two events and
+ * kept until the process ends), forked from and joined back into `stream`
+ * inside the call: the call stays stream-ordered, also on an error return.
+ * The workspace size grows with npix: a workspace of
+ * mbx_vcd_workspace_bytes(g, N, dt) bytes serves every subset of up to N
+ * pixels. */
 int64_t mbx_vcd_workspace_bytes(const mbx_geom *g, int32_t npix, int32_t delta_dtype);
 int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_recon, void *e, int32_t e_dtype,
                           const float *weights, const float *fm_hessian, const int32_t *pix, int32_t npix,

@@ -89,7 +89,7 @@ def lib():
             "mbx_forward_workspace_bytes": (i64, [G, i32, i32]),
             "mbx_forward_project": (i32, [G, vp, vp, vp, i32, vp, i32, vp, i32, i32, i32, vp, vp, vp, i64, vp]),
             "mbx_forward_plan": (i32, [G, vp, vp, i32, vp, i32, vp, i64, vp]),
-            "mbx_forward_apply": (i32, [G, vp, vp, i32, vp, i32, vp, i32, i32, i32, vp, vp, vp]),
+            "mbx_forward_apply": (i32, [G, vp, i64, vp, i32, vp, i32, vp, i32, i32, i32, vp, vp, vp]),
             "mbx_back_project": (i32, [G, vp, vp, i32, i32, vp, vp, i32, vp, i32, i32, i32, f32, i32, vp, vp, i64,
                                        vp]),
             "mbx_sino_transpose": (i32, [vp, i32, i32, vp, i32, i32, i32, i32, i32, vp]),

@@ -58,6 +58,9 @@ struct BackPlan {
 };
 BackPlan plan_back(const DevGeom &g, int npix, int nviews, int ppw);
 int64_t back_part_bytes(const DevGeom &g, int chunk, int nviews);  // view-group partials
+// The most partials any pixel count up to n needs (the split grows as the
+// count falls, so a smaller subset can need more than a larger one).
+int64_t back_part_bound(const DevGeom &g, int n, int nviews);
 int64_t back_workspace_bytes(const DevGeom &g, int npix, int nviews);
 
 struct FwdPlan {

@@ -347,6 +347,27 @@ int64_t back_part_bytes(const DevGeom &g, int chunk, int nviews) {
     return (part + 255) & ~255ll;
 }
 
+// The largest back_part_bytes() of any pixel count 1..n.  A plan's view split
+// never grows with the pixel count (more waves stop the doubling earlier), so
+// on each run of equal splits the partials grow with the count and the
+// maximum sits at the run's last count: found by bisection for each split.
+int64_t back_part_bound(const DevGeom &g, int n, int nviews) {
+    int64_t best = 0;
+    for (int ppw = 1; ppw <= 2; ++ppw) {
+        for (int s = 2; s <= 64; s *= 2) {
+            if (n < 1 || plan_back(g, 1, nviews, ppw).nsplit < s) break;
+            int lo = 1, hi = n;  // largest count whose split is >= s
+            while (lo < hi) {
+                const int mid = lo + (hi - lo + 1) / 2;
+                if (plan_back(g, mid, nviews, ppw).nsplit >= s) lo = mid; else hi = mid - 1;
+            }
+            const BackPlan p = plan_back(g, lo, nviews, ppw);
+            best = std::max<int64_t>(best, (int64_t)p.nsplit * lo * g.Nz * (int64_t)sizeof(float));
+        }
+    }
+    return (best + 255) & ~255ll;
+}
+
 int64_t back_workspace_bytes(const DevGeom &g, int npix, int nviews) {
     if (npix == 0 || nviews == 0) return 0;
     const int chunk = table_chunk(g, npix, nviews);
@@ -1820,9 +1841,9 @@ int mbx_forward_plan(const mbx_geom *g, const float *angles, const int32_t *pix,
                           chunk, nullptr, st, 1, kFwdBin);
 }
 
-int mbx_forward_apply(const mbx_geom *g, const void *plan, const float *vox, int32_t npix, const int32_t *views,
-                      int32_t nviews, void *sino, int32_t sino_dtype, int32_t sino_layout, int32_t accumulate,
-                      const float *weights, double *sumsq, void *stream) {
+int mbx_forward_apply(const mbx_geom *g, const void *plan, int64_t plan_bytes, const float *vox, int32_t npix,
+                      const int32_t *views, int32_t nviews, void *sino, int32_t sino_dtype, int32_t sino_layout,
+                      int32_t accumulate, const float *weights, double *sumsq, void *stream) {
     DevGeom d;
     int rc = make_dev_geom(g, &d);
     if (rc != MBX_OK) return rc;
@@ -1835,6 +1856,15 @@ int mbx_forward_apply(const mbx_geom *g, const void *plan, const float *vox, int
     const int chunk = table_chunk(d, std::max(npix, 1), std::max(nviews, 1));
     if (npix > chunk) return fail(MBX_EUNSUPPORTED, "forward apply: the pixel set exceeds one table chunk");
     if (!plan || !sino || (npix > 0 && !vox)) return fail(MBX_EINVAL, "NULL device pointer (plan, sino or vox)");
+    // the plan's layout follows from its pixel and view counts, and its size
+    // grows with either: an apply of more pixels or views than the plan was
+    // built for, which would read past its end, is rejected here (counts that
+    // differ only within the 256-byte rounding of a tiny plan are not told apart)
+    const int64_t need = fwd_workspace_bytes(d, npix, nviews);
+    if (plan_bytes != need)
+        return fail(MBX_EINVAL, "forward apply: plan_bytes " + std::to_string(plan_bytes) + " is not the size " +
+                                    std::to_string(need) + " of a plan of " + std::to_string(npix) + " pixels and " +
+                                    std::to_string(nviews) + " views");
     hipStream_t st = (hipStream_t)stream;
     if (nviews == 0) {
         if (sumsq) return launch_sum_partials(nullptr, 0, sumsq, 0, st);  // writes 0

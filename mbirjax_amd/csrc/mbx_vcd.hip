@@ -401,8 +401,10 @@ static VcdLayout vcd_layout(const DevGeom &g, int npix, int ddt) {
     const size_t cyl = (size_t)npix * g.Nz * sizeof(float);
     L.table = off;
     off += align256((size_t)table_bytes(L.chunk, g.V));
-    L.bpart = off;
-    off += align256((size_t)back_part_bytes(g, L.chunk, g.V));
+    L.bpart = off;  // sized for every subset of up to npix pixels, so the whole
+                    // layout grows with npix: a workspace sized for the largest
+                    // subset of a set of partitions holds every smaller one
+    off += align256((size_t)back_part_bound(g, L.chunk, g.V));
     L.fpart = off;
     off += align256((size_t)fwd_aux_bytes(g, L.chunk, g.V));
     L.bp = off;
@@ -540,13 +542,16 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
     SideStream *side = nullptr;
     std::unique_lock<std::mutex> side_lock;
     // the caller's stream waits for the side stream's work on every return
-    // path once it is forked, so nothing of this call runs past its end
+    // path once it is forked, so nothing of this call runs past its end: through
+    // the join event once it is recorded, else (an error between the fork and
+    // the record) by draining the side stream before returning
     struct Join {
         SideStream *&side;
         hipStream_t st;
-        bool forked = false;
+        bool forked = false, recorded = false;
         ~Join() {
-            if (forked) (void)hipStreamWaitEvent(st, side->join, 0);
+            if (forked && recorded) (void)hipStreamWaitEvent(st, side->join, 0);
+            else if (forked) (void)hipStreamSynchronize(side->s);
         }
     } join{side, st};
     for (int c0 = 0; c0 < npix; c0 += L.chunk) {
@@ -557,11 +562,12 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
             side_lock = std::unique_lock<std::mutex>(side->mu);
             MBX_HIP_CHECK(hipEventRecord(side->fork, st));
             MBX_HIP_CHECK(hipStreamWaitEvent(side->s, side->fork, 0));
+            join.forked = true;  // from here on the side stream may hold work of this call
             rc = launch_fwd_tab(d, T, NC, delta, n, d.V, dsino, delta_dtype, MBX_LAYOUT_VCR, 0, weights, fpart,
                                 L.chunk, nullptr, side->s, 1, kFwdBin);
             if (rc != MBX_OK) return rc;
             MBX_HIP_CHECK(hipEventRecord(side->join, side->s));
-            join.forked = true;
+            join.recorded = true;
         }
         rc = launch_back_tab(d, T, n, e, e_dtype, weights, nullptr, d.V, 1, kahan, 1.0f, 0, bp + (size_t)c0 * d.Nz,
                              bpart, st, one_table ? &bsplit : nullptr, L.chunk);
@@ -607,7 +613,7 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
     }
     MBX_LAUNCH_CHECK();
     if (join.forked) {  // join: the bins and records are in fpart
-        join.forked = false;
+        join.forked = join.recorded = false;
         MBX_HIP_CHECK(hipStreamWaitEvent(st, side->join, 0));
         side_lock.unlock();
         binned = true;

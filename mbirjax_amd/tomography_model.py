@@ -203,21 +203,24 @@ class TomographyModel(ParameterHandler):
         V = self.get_params('sinogram_shape')[0]
         nv = V if views is None else int(views.numel())
         nb = int(_lib.lib().mbx_forward_workspace_bytes(self._geom, int(pix.numel()), nv))
-        plan = torch.empty(max(nb, 256), dtype=torch.uint8, device=pix.device)
+        buf = torch.empty(max(nb, 256), dtype=torch.uint8, device=pix.device)
         _lib.check(_lib.lib().mbx_forward_plan(self._geom, _lib.ptr(self._angles()), _lib.ptr(pix), int(pix.numel()),
-                                               _lib.ptr(views), nv, _lib.ptr(plan), nb, _lib.stream()))
-        return plan
+                                               _lib.ptr(views), nv, _lib.ptr(buf), nb, _lib.stream()))
+        return ForwardPlan(buf, nb, int(pix.numel()), nv)
 
     def _fwd_apply(self, plan, vox, views=None, layout=_lib.LAYOUT_VCR, out_dtype=_F32, out=None, accumulate=False):
         """Forward projection of cylinders vox (npix, Nz) with a plan from _fwd_plan."""
         V, R, C = self.get_params('sinogram_shape')
         nv = V if views is None else int(views.numel())
+        if int(vox.shape[0]) != plan.npix or nv != plan.nviews:
+            raise ValueError('forward plan of {} pixels and {} views applied to {} pixels and {} views'.format(
+                plan.npix, plan.nviews, int(vox.shape[0]), nv))
         if out is None:
             shape = (nv, C, R) if layout == _lib.LAYOUT_VCR else (nv, R, C)
             out = torch.empty(shape, dtype=_torch_dtype(out_dtype), device=vox.device)
-        _lib.check(_lib.lib().mbx_forward_apply(self._geom, _lib.ptr(plan), _lib.ptr(vox), int(vox.shape[0]),
-                                                _lib.ptr(views), nv, _lib.ptr(out), out_dtype, layout, int(accumulate),
-                                                None, None, _lib.stream()))
+        _lib.check(_lib.lib().mbx_forward_apply(self._geom, _lib.ptr(plan.buf), plan.nbytes, _lib.ptr(vox),
+                                                int(vox.shape[0]), _lib.ptr(views), nv, _lib.ptr(out), out_dtype,
+                                                layout, int(accumulate), None, None, _lib.stream()))
         return out
 
     def _fwd_rows(self, vox, pix, views=None):
@@ -643,6 +646,13 @@ class TomographyModel(ParameterHandler):
             model = cls(**required)
             model.set_params(**params)
         return model
+
+
+class ForwardPlan:
+    """A device buffer from mbx_forward_plan with the counts it was built for."""
+
+    def __init__(self, buf, nbytes, npix, nviews):
+        self.buf, self.nbytes, self.npix, self.nviews = buf, int(nbytes), int(npix), int(nviews)
 
 
 class VCDState:

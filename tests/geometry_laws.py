@@ -88,10 +88,12 @@ def impulse_responses(project, geom, voxels):
     return {(i, j, k): np.asarray(project(i * cols + j, k), dtype=np.float64) for (i, j, k) in voxels}
 
 
-def law_errors(geom, responses):
+def law_errors(geom, responses, rel_eps=1e-9):
     """Largest violation, relative to each law's tolerance, of the centroid,
     plateau and mass laws over the given impulse responses (a value <= 1
-    means the law holds).  Returns a dict law -> max ratio."""
+    means the law holds).  rel_eps widens every relative tolerance by the
+    arithmetic's rounding (1e-9 for FP64; a float32 projector needs ~1e-5).
+    Returns a dict law -> max ratio."""
     dv = geom.delta_voxel
     dc, dr = geom.delta_det_channel, geom.delta_det_row
     worst = dict(centroid=0.0, plateau=0.0, mass=0.0, border=0.0)
@@ -112,7 +114,7 @@ def law_errors(geom, responses):
             # footprint widths in cells, bounded below without the projector:
             # the voxel's shadow is at least dv * jac_mag * cos(gamma) / sqrt(2) wide
             w_lo = min(dv * jac_mag * cos_gamma / math.sqrt(2) / dc, dv * jac_mag * cos_gamma / dr)
-            tol_c = 1.0 / (8.0 * w_lo) + 1e-9
+            tol_c = 1.0 / (8.0 * w_lo) + rel_eps * max(R, C)
             if tot <= 0:
                 worst['centroid'] = max(worst['centroid'], np.inf)
                 continue
@@ -121,7 +123,7 @@ def law_errors(geom, responses):
             worst['centroid'] = max(worst['centroid'], abs(cm - m) / tol_c, abs(cn - n) / tol_c)
             a = u / geom.source_detector_dist
             b = v / geom.source_detector_dist
-            tol_2 = 0.5 * a * a * b * b + 1e-9
+            tol_2 = 0.5 * a * a * b * b + rel_eps
             mass_phys = dv ** 3 * jac_mag ** 2 / cos_gamma / (dc * dr)
             worst['mass'] = max(worst['mass'], abs(tot / mass_phys - 1) / tol_2)
             if w_lo >= 2.0:

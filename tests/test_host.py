@@ -81,8 +81,16 @@ def test_validation_before_any_hip_call():
     assert b"table chunk" in L.mbx_last_error()
     assert L.mbx_forward_plan(geom(), fake, fake, 4, None, 0, fake, L.mbx_forward_workspace_bytes(geom(), 4, 8) - 1,
                               None) == _lib.MBX_EWORKSPACE
-    assert L.mbx_forward_apply(geom(), fake, fake, 4, None, 0, fake, 0, 7, 0, None, None, None) == _lib.MBX_EINVAL
-    assert L.mbx_forward_apply(geom(), None, fake, 4, None, 0, fake, 0, 0, 0, None, None, None) == _lib.MBX_EINVAL
+    nb4 = L.mbx_forward_workspace_bytes(geom(), 4, 8)
+    assert L.mbx_forward_apply(geom(), fake, nb4, fake, 4, None, 0, fake, 0, 7, 0, None, None, None) == _lib.MBX_EINVAL
+    assert L.mbx_forward_apply(geom(), None, nb4, fake, 4, None, 0, fake, 0, 0, 0, None, None,
+                               None) == _lib.MBX_EINVAL
+    # an apply of more pixels than the plan holds is rejected by the plan size
+    nb40 = L.mbx_forward_workspace_bytes(geom(), 40, 8)
+    assert nb40 > nb4
+    assert L.mbx_forward_apply(geom(), fake, nb4, fake, 40, None, 0, fake, 0, 0, 0, None, None,
+                               None) == _lib.MBX_EINVAL
+    assert b"plan_bytes" in L.mbx_last_error()
     # transpose aliasing, dot and axpy arguments
     assert L.mbx_sino_transpose(fake, 0, 0, fake, 0, 1, 2, 3, 4, None) == _lib.MBX_EINVAL
     assert L.mbx_dot(fake, 0, None, 0, None, -1, fake, fake, 8192, None) == _lib.MBX_EINVAL
@@ -98,6 +106,24 @@ def test_validation_before_any_hip_call():
     assert L.mbx_vcd_subset_update(geom(), fake, fake, fake, 0, None, fake, fake, 4, 1.0, q, None, 1.0, 0, 0, 0,
                                    None, fake, 16, None) == _lib.MBX_EWORKSPACE
     assert L.mbx_vcd_workspace_bytes(geom(), 4, 0) > L.mbx_vcd_workspace_bytes(geom(), 4, 1)
+
+
+@pytest.mark.parametrize("views,slices", [(64, 512), (32, 512), (16, 1024)])
+def test_vcd_workspace_serves_every_smaller_subset(views, slices):
+    """The VCD workspace grows with the subset size, so one sized for the
+    largest subset of the default partitions (256^2 recon, 51040 ROR pixels)
+    holds every smaller subset: few-view, many-slice geometries get back
+    projection view-group partials only at the small subsets."""
+    L = _lib.lib()
+    g = geom(sinogram_shape=(views, 256, 384), recon_shape=(256, 256, slices), source_detector_dist=1536.0)
+    P = 51040
+    sizes = sorted({-(-P // K) for K in (1, 2, 4, 8, 16, 32, 64, 128, 256)}, reverse=True)
+    for dt in (0, 1):
+        ws = [L.mbx_vcd_workspace_bytes(g, n, dt) for n in sizes]
+        assert all(w > 0 for w in ws)
+        assert ws[0] == max(ws), (views, slices, dt, ws)
+        sweep = [L.mbx_vcd_workspace_bytes(g, n, dt) for n in range(1, 4000, 37)]
+        assert all(a <= b for a, b in zip(sweep, sweep[1:]))
 
 
 def test_check_raises_with_message():
