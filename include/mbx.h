@@ -177,6 +177,26 @@ int mbx_back_project(const mbx_geom *g, const float *angles, const void *sino, i
                      const int32_t *views, int32_t nviews, int32_t coeff_power, int32_t kahan, float scale,
                      int32_t accumulate, float *out, void *workspace, int64_t workspace_bytes, void *stream);
 
+/* mbx_back_project without a workspace (no counterpart in mbirjax: for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and result
+ * as mbx_back_project without kahan (plain summation) and without workspace:
+ * the geometry table and the view-group partials live in a file-scope device
+ * array of mbx_noalloc_scratch_bytes() bytes (64 MiB), which every GPU the
+ * library runs on holds until the process ends.  Large pixel sets are
+ * projected in table chunks that fit it (bitwise the sums of
+ * mbx_back_project whenever its own chunk fits); a call whose single-pixel
+ * table and partials do not fit returns MBX_EUNSUPPORTED.  The call allocates
+ * nothing (no hipMalloc, hipFree or allocating library routine) and contains
+ * no memset, so it can be captured into a HIP graph.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams or
+ * host threads) race and must be serialised by the caller. */
+int64_t mbx_noalloc_scratch_bytes(void);
+int mbx_back_project_noalloc(const mbx_geom *g, const float *angles, const void *sino, int32_t sino_dtype,
+                             int32_t num_sino_views, const float *weights, const int32_t *pix, int32_t npix,
+                             const int32_t *views, int32_t nviews, int32_t coeff_power, float scale,
+                             int32_t accumulate, float *out, void *stream);
+
 /* Convert a sinogram between layouts (and dtypes).  src and dst must not alias. */
 int mbx_sino_transpose(const void *src, int32_t src_dtype, int32_t src_layout, void *dst, int32_t dst_dtype,
                        int32_t dst_layout, int32_t num_views, int32_t num_rows, int32_t num_channels,

@@ -4,7 +4,7 @@ import subprocess
 import sys
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ["csrc/mbx_project.hip", "csrc/mbx_vcd.hip", "csrc/mbx_denoise.hip"]
+SOURCES = ["csrc/mbx_project.hip", "csrc/mbx_vcd.hip", "csrc/mbx_denoise.hip", "csrc/mbx_noalloc.hip"]
 OUTPUT = os.path.join(_HERE, "libmbx.so")
 
 HIPCC_FLAGS = [

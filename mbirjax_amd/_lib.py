@@ -34,6 +34,7 @@ EXPORTS = (
     "mbx_qggmrf_grad_hess",
     "mbx_dot_workspace_bytes", "mbx_dot", "mbx_sino_axpy", "mbx_vcd_workspace_bytes", "mbx_vcd_subset_update",
     "mbx_denoise_workspace_bytes", "mbx_denoise_subset_update", "mbx_median_filter3d",
+    "mbx_noalloc_scratch_bytes", "mbx_back_project_noalloc",
 )
 
 
@@ -103,6 +104,8 @@ def lib():
             "mbx_denoise_workspace_bytes": (i64, [i32, i32]),
             "mbx_denoise_subset_update": (i32, [vp, vp, i32, i32, i32, vp, i32, f64, Q, vp, vp, i64, vp]),
             "mbx_median_filter3d": (i32, [vp, i32, i32, i32, i32, vp, vp]),
+            "mbx_noalloc_scratch_bytes": (i64, []),
+            "mbx_back_project_noalloc": (i32, [G, vp, vp, i32, i32, vp, vp, i32, vp, i32, i32, f32, i32, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

@@ -126,6 +126,25 @@ def test_vcd_workspace_serves_every_smaller_subset(views, slices):
         assert all(a <= b for a, b in zip(sweep, sweep[1:]))
 
 
+def test_noalloc_back_projection_validation():
+    """mbx_back_project_noalloc validates like mbx_back_project and rejects a
+    call whose single-pixel table and partials exceed its scratch, before any
+    HIP call."""
+    L = _lib.lib()
+    fake = ctypes.c_void_p(16)
+    assert L.mbx_noalloc_scratch_bytes() == 64 << 20
+    assert L.mbx_back_project_noalloc(None, fake, fake, 0, 8, None, fake, 1, None, 0, 1, 1.0, 0, fake,
+                                      None) == _lib.MBX_EINVAL
+    assert L.mbx_back_project_noalloc(geom(), fake, fake, 0, 7, None, fake, 1, None, 0, 1, 1.0, 0, fake,
+                                      None) == _lib.MBX_EINVAL
+    assert L.mbx_back_project_noalloc(geom(), fake, fake, 0, 8, None, fake, 1, None, 0, 3, 1.0, 0, fake,
+                                      None) == _lib.MBX_EINVAL
+    huge = geom(sinogram_shape=(1_000_000, 4, 6))
+    assert L.mbx_back_project_noalloc(huge, fake, fake, 0, 1_000_000, None, fake, 1, None, 0, 1, 1.0, 0, fake,
+                                      None) == _lib.MBX_EUNSUPPORTED
+    assert b"MiB scratch" in L.mbx_last_error()
+
+
 def test_check_raises_with_message():
     L = _lib.lib()
     L.mbx_sino_transpose(None, 0, 0, None, 0, 0, 1, 1, 1, None)
