@@ -117,8 +117,7 @@ def run_precision(model, y, part_np, precision, steps, warmup):
     sub = part[0]
     nz = model.get_params('recon_shape')[2]
     out = torch.empty((nS, nz), dtype=torch.float32, device=dev)
-    kahan = state.kahan
-    t_back = timed(lambda: model._back(state.e, sub, kahan=kahan, out=out), 20, stream)
+    t_back = timed(lambda: model._back(state.e, sub, out=out), 20, stream)
     d_dtype = state.delta_dtype
     vox = torch.randn((nS, nz), device=dev)
     V, R, C = model.get_params('sinogram_shape')
@@ -128,8 +127,8 @@ def run_precision(model, y, part_np, precision, steps, warmup):
     plan = model._fwd_plan(sub)
     t_fwd_kernel = timed(lambda: model._fwd_apply(plan, vox, out_dtype=d_dtype, out=ds), 20, stream)
     del plan
-    b_e = 2 if state.e_dtype == _lib.DTYPE_BF16 else 4
-    b_d = 2 if d_dtype == _lib.DTYPE_BF16 else 4
+    b_e = 4 if state.e_dtype == _lib.DTYPE_F32 else 2
+    b_d = 4 if d_dtype == _lib.DTYPE_F32 else 2
     b_back, b_fwd = algorithmic_bytes(model, nS, b_e, b_d)
     res = dict(precision=precision, t_iter=t_iter, iter_per_s=1.0 / t_iter, wall_per_iter=t_wall / steps,
                t_back_ms=t_back * 1e3, t_fwd_ms=t_fwd * 1e3, t_fwd_kernel_ms=t_fwd_kernel * 1e3,
@@ -242,7 +241,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cone512", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="all", choices=["fp32", "mixed", "bf16", "all"])
+    ap.add_argument("--precision", default="all", choices=["fp32", "mixed", "fp16", "bf16", "all"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-full-run", action="store_true",
@@ -259,7 +258,7 @@ def main():
     model, y, part = build_problem(args.workload)
     log("setup %.1fs" % (time.time() - t0))
     results = {}
-    precisions = ["fp32", "mixed", "bf16"] if args.precision == "all" else [args.precision]
+    precisions = ["fp32", "mixed", "fp16", "bf16"] if args.precision == "all" else [args.precision]
     for p in precisions:
         results[p] = run_precision(model, y, part, p, args.steps, args.warmup)
         log(p, json.dumps(results[p]))

@@ -14,7 +14,7 @@
  *   - voxel cylinders: float32 (npix, num_recon_slices), row-major.
  *   - sinograms: views x channels x rows ("channel-major", MBX_LAYOUT_VCR) or
  *     the user-facing views x rows x channels (MBX_LAYOUT_VRC); float32 or
- *     bfloat16 (MBX_DTYPE_*).  The projectors read/write MBX_LAYOUT_VCR;
+ *     bfloat16 (MBX_DTYPE_*), and float16 where MBX_DTYPE_F16 says so.  The projectors read/write MBX_LAYOUT_VCR;
  *     mbx_sino_transpose converts.
  *   - angles: float32 (num_views,), radians.
  *
@@ -71,6 +71,10 @@ extern "C" {
 #define MBX_DTYPE_F32 0
 #define MBX_DTYPE_BF16 1
 #define MBX_DTYPE_F64 2 /* mbx_median_filter3d only */
+#define MBX_DTYPE_F16 3 /* IEEE binary16: a 2-byte error sinogram (read by the back projector, mbx_dot,
+                           mbx_sino_axpy, mbx_sino_transpose and the VCD update; never a forward output).
+                           The caller keeps |values| below 65504, e.g. by scaling the problem by a power
+                           of two, which leaves the VCD trajectory unchanged apart from the rounding */
 
 #define MBX_LAYOUT_VCR 0 /* (views, channels, rows): projector-native */
 #define MBX_LAYOUT_VRC 1 /* (views, rows, channels): mbirjax user layout */
@@ -169,18 +173,18 @@ int mbx_forward_apply(const mbx_geom *g, const void *plan, int64_t plan_bytes, c
  * nviews indices that select both the sinogram view and its angle (the
  * reference's view_indices semantics, tomography_model.py:661-663).
  * weights (nullable) multiplies the sinogram element-wise (same shape, f32).
- * coeff_power 1 or 2 (2 = Hessian diagonal).  kahan = 1 uses compensated
- * summation across views.  out (npix, Nz) f32 = (accumulate ? out : 0) +
+ * coeff_power 1 or 2 (2 = Hessian diagonal; float32 sinogram).  sino_dtype
+ * MBX_DTYPE_F32, MBX_DTYPE_BF16 or MBX_DTYPE_F16.  out (npix, Nz) f32 = (accumulate ? out : 0) +
  * scale * A^T(weights*sino).  workspace: mbx_back_workspace_bytes() bytes. */
 int mbx_back_project(const mbx_geom *g, const float *angles, const void *sino, int32_t sino_dtype,
                      int32_t num_sino_views, const float *weights, const int32_t *pix, int32_t npix,
-                     const int32_t *views, int32_t nviews, int32_t coeff_power, int32_t kahan, float scale,
-                     int32_t accumulate, float *out, void *workspace, int64_t workspace_bytes, void *stream);
+                     const int32_t *views, int32_t nviews, int32_t coeff_power, float scale, int32_t accumulate,
+                     float *out, void *workspace, int64_t workspace_bytes, void *stream);
 
 /* mbx_back_project without a workspace (no counterpart in mbirjax: for callers
  * that cannot allocate device memory -- code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  Same arguments and result
- * as mbx_back_project without kahan (plain summation) and without workspace:
+ * as mbx_back_project without workspace:
  * the geometry table and the view-group partials live in a file-scope device
  * array of mbx_noalloc_scratch_bytes() bytes (64 MiB), which every GPU the
  * library runs on holds until the process ends.  Large pixel sets are
@@ -222,10 +226,11 @@ int mbx_sino_axpy(void *e, int32_t e_dtype, const void *d, int32_t d_dtype, cons
 
 /* One VCD subset update: vcd_subset_updater (mbirjax/tomography_model.py:1343-1542).
  * State: flat_recon (rows*cols, Nz) f32 and the error sinogram e (V, C, R) in
- * e_dtype (MBX_LAYOUT_VCR), e = y - A x.  weights: NULL = constant 1, else
- * (V, C, R) f32.  fm_hessian: (rows*cols, Nz) f32 = diag(A^T W A).
- * prox_input: NULL = qGGMRF prior, else (rows*cols, Nz) f32 proximal-map input
- * with prior Hessian sigma_prox^2.  delta_dtype: storage of the delta sinogram.
+ * e_dtype (MBX_DTYPE_F32, _BF16 or _F16; MBX_LAYOUT_VCR), e = y - A x.
+ * weights: NULL = constant 1, else (V, C, R) f32.  fm_hessian: (rows*cols, Nz)
+ * f32 = diag(A^T W A).  prox_input: NULL = qGGMRF prior, else (rows*cols, Nz)
+ * f32 proximal-map input with prior Hessian sigma_prox^2.  delta_dtype:
+ * storage of the delta sinogram (MBX_DTYPE_F32 or _BF16).
  * stats (device double[4]) accumulates {sum |alpha*delta|, alpha, 1, 0} for
  * per-iteration statistics.  workspace: mbx_vcd_workspace_bytes() bytes.
  * Subsets of 2^18 (view, pixel) pairs or more bin the forward records on a
@@ -240,7 +245,7 @@ int64_t mbx_vcd_workspace_bytes(const mbx_geom *g, int32_t npix, int32_t delta_d
 int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_recon, void *e, int32_t e_dtype,
                           const float *weights, const float *fm_hessian, const int32_t *pix, int32_t npix,
                           double fm_constant, const mbx_qggmrf *qparams, const float *prox_input,
-                          double sigma_prox, int32_t positivity, int32_t delta_dtype, int32_t kahan,
+                          double sigma_prox, int32_t positivity, int32_t delta_dtype,
                           double *stats, void *workspace, int64_t workspace_bytes, void *stream);
 
 /* One subset update of the qGGMRF proximal-map denoiser (vcd_subset_denoiser,

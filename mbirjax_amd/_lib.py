@@ -24,6 +24,7 @@ GEOM_TRANSLATION = 2
 DTYPE_F32 = 0
 DTYPE_BF16 = 1
 DTYPE_F64 = 2
+DTYPE_F16 = 3
 LAYOUT_VCR = 0
 LAYOUT_VRC = 1
 
@@ -91,16 +92,15 @@ def lib():
             "mbx_forward_project": (i32, [G, vp, vp, vp, i32, vp, i32, vp, i32, i32, i32, vp, vp, vp, i64, vp]),
             "mbx_forward_plan": (i32, [G, vp, vp, i32, vp, i32, vp, i64, vp]),
             "mbx_forward_apply": (i32, [G, vp, i64, vp, i32, vp, i32, vp, i32, i32, i32, vp, vp, vp]),
-            "mbx_back_project": (i32, [G, vp, vp, i32, i32, vp, vp, i32, vp, i32, i32, i32, f32, i32, vp, vp, i64,
-                                       vp]),
+            "mbx_back_project": (i32, [G, vp, vp, i32, i32, vp, vp, i32, vp, i32, i32, f32, i32, vp, vp, i64, vp]),
             "mbx_sino_transpose": (i32, [vp, i32, i32, vp, i32, i32, i32, i32, i32, vp]),
             "mbx_qggmrf_grad_hess": (i32, [vp, i32, i32, i32, vp, i32, Q, vp, vp, vp]),
             "mbx_dot_workspace_bytes": (i64, []),
             "mbx_dot": (i32, [vp, i32, vp, i32, vp, i64, vp, vp, i64, vp]),
             "mbx_sino_axpy": (i32, [vp, i32, vp, i32, vp, i64, vp]),
             "mbx_vcd_workspace_bytes": (i64, [G, i32, i32]),
-            "mbx_vcd_subset_update": (i32, [G, vp, vp, vp, i32, vp, vp, vp, i32, f64, Q, vp, f64, i32, i32, i32, vp,
-                                            vp, i64, vp]),
+            "mbx_vcd_subset_update": (i32, [G, vp, vp, vp, i32, vp, vp, vp, i32, f64, Q, vp, f64, i32, i32, vp, vp,
+                                            i64, vp]),
             "mbx_denoise_workspace_bytes": (i64, [i32, i32]),
             "mbx_denoise_subset_update": (i32, [vp, vp, i32, i32, i32, vp, i32, f64, Q, vp, vp, i64, vp]),
             "mbx_median_filter3d": (i32, [vp, i32, i32, i32, i32, vp, vp]),

@@ -24,8 +24,21 @@ struct bf16 {
     uint16_t bits;
 };
 
+// ---------------------------------------------------------------- float16
+// Storage type for the 2-byte error sinogram of precision 'fp16': IEEE binary16
+// (11-bit significand, 8x finer than bfloat16), rounded to nearest-even on
+// store; the host keeps |e| within its range by scaling the problem by a
+// power of two.
+struct f16 {
+    uint16_t bits;
+};
+
+__device__ __forceinline__ float h2f(uint32_t b16) { return (float)__builtin_bit_cast(_Float16, (uint16_t)b16); }
+__device__ __forceinline__ uint32_t f2h(float x) { return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)x); }
+
 __device__ __forceinline__ float to_float(float x) { return x; }
 __device__ __forceinline__ float to_float(bf16 x) { return __uint_as_float(((uint32_t)x.bits) << 16); }
+__device__ __forceinline__ float to_float(f16 x) { return h2f(x.bits); }
 
 template <typename T>
 __device__ __forceinline__ T from_float(float x);
@@ -38,6 +51,12 @@ __device__ __forceinline__ bf16 from_float<bf16>(float x) {
     // round to nearest even (NaN stays a quiet NaN): one v_cvt_pk_bf16_f32 on gfx950
     bf16 r;
     r.bits = __builtin_bit_cast(uint16_t, (__bf16)x);
+    return r;
+}
+template <>
+__device__ __forceinline__ f16 from_float<f16>(float x) {
+    f16 r;  // v_cvt_f16_f32: round to nearest even
+    r.bits = (uint16_t)f2h(x);
     return r;
 }
 
@@ -56,7 +75,20 @@ __device__ __forceinline__ void load4(const bf16 *p, float v[4]) {
     v[2] = __uint_as_float(t.y << 16);
     v[3] = __uint_as_float(t.y & 0xffff0000u);
 }
+__device__ __forceinline__ void load4(const f16 *p, float v[4]) {
+    const uint2 t = *(const uint2 *)p;
+    v[0] = h2f(t.x & 0xffffu);
+    v[1] = h2f(t.x >> 16);
+    v[2] = h2f(t.y & 0xffffu);
+    v[3] = h2f(t.y >> 16);
+}
 __device__ __forceinline__ void store4(float *p, const float v[4]) { *(float4 *)p = make_float4(v[0], v[1], v[2], v[3]); }
+__device__ __forceinline__ void store4(f16 *p, const float v[4]) {
+    uint2 t;
+    t.x = f2h(v[0]) | (f2h(v[1]) << 16);
+    t.y = f2h(v[2]) | (f2h(v[3]) << 16);
+    *(uint2 *)p = t;
+}
 __device__ __forceinline__ void store4(bf16 *p, const float v[4]) {
     uint2 t;
     t.x = (uint32_t)from_float<bf16>(v[0]).bits | ((uint32_t)from_float<bf16>(v[1]).bits << 16);
@@ -80,6 +112,19 @@ __device__ __forceinline__ void load4_nt(const bf16 *p, float v[4]) {
     v[1] = __uint_as_float(t.x & 0xffff0000u);
     v[2] = __uint_as_float(t.y << 16);
     v[3] = __uint_as_float(t.y & 0xffff0000u);
+}
+__device__ __forceinline__ void load4_nt(const f16 *p, float v[4]) {
+    const mbx_u32x2 t = __builtin_nontemporal_load((const mbx_u32x2 *)p);
+    v[0] = h2f(t.x & 0xffffu);
+    v[1] = h2f(t.x >> 16);
+    v[2] = h2f(t.y & 0xffffu);
+    v[3] = h2f(t.y >> 16);
+}
+__device__ __forceinline__ void store4_nt(f16 *p, const float v[4]) {
+    mbx_u32x2 t;
+    t.x = f2h(v[0]) | (f2h(v[1]) << 16);
+    t.y = f2h(v[2]) | (f2h(v[3]) << 16);
+    __builtin_nontemporal_store(t, (mbx_u32x2 *)p);
 }
 __device__ __forceinline__ void store4_nt(float *p, const float v[4]) {
     mbx_f32x4 t;
@@ -124,6 +169,17 @@ __device__ __forceinline__ mbx_f32x4 buf_load4<bf16>(i32x4_t rsrc, int voffset, 
     return r;
 }
 
+template <>
+__device__ __forceinline__ mbx_f32x4 buf_load4<f16>(i32x4_t rsrc, int voffset, int soffset) {
+    const mbx_u32x2 u = raw_load_u32x2(rsrc, voffset, soffset, 0);
+    mbx_f32x4 r;
+    r.x = h2f(u.x & 0xffffu);
+    r.y = h2f(u.x >> 16);
+    r.z = h2f(u.y & 0xffffu);
+    r.w = h2f(u.y >> 16);
+    return r;
+}
+
 __device__ __forceinline__ i32x4_t make_rsrc(const void *p) {
     const uint64_t a = (uint64_t)p;
     i32x4_t r;
@@ -144,6 +200,10 @@ template <>
 __device__ __forceinline__ float buf_load<bf16>(i32x4_t rsrc, int voffset, int soffset) {
     const unsigned short b = (unsigned short)raw_load_i16(rsrc, voffset, soffset, 0);
     return __uint_as_float(((uint32_t)b) << 16);
+}
+template <>
+__device__ __forceinline__ float buf_load<f16>(i32x4_t rsrc, int voffset, int soffset) {
+    return h2f((unsigned short)raw_load_i16(rsrc, voffset, soffset, 0));
 }
 
 // ---------------------------------------------------------------- wave sync

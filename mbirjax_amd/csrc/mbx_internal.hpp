@@ -34,8 +34,11 @@ int fail(int code, const std::string &msg);
 // Validates g and fills the float32 device copy.  Returns MBX_OK or an error.
 int make_dev_geom(const mbx_geom *g, DevGeom *out);
 
-inline int dtype_size(int dt) { return dt == MBX_DTYPE_BF16 ? 2 : 4; }
-inline bool dtype_ok(int dt) { return dt == MBX_DTYPE_F32 || dt == MBX_DTYPE_BF16; }
+inline int dtype_size(int dt) { return dt == MBX_DTYPE_BF16 || dt == MBX_DTYPE_F16 ? 2 : 4; }
+// sinogram storage the back projector, transposes, dots and the error-sinogram update read
+inline bool dtype_ok(int dt) { return dt == MBX_DTYPE_F32 || dt == MBX_DTYPE_BF16 || dt == MBX_DTYPE_F16; }
+// storage the forward projector writes (and the delta sinogram of the VCD update)
+inline bool dtype_fwd_ok(int dt) { return dt == MBX_DTYPE_F32 || dt == MBX_DTYPE_BF16; }
 
 // Per-(view, pixel) geometry table shared by both projectors.
 int table_chunk(const DevGeom &g, int npix, int nviews);  // pixels per table
@@ -49,14 +52,13 @@ inline int *table_nc(float *T, int chunk, int nviews) {
 
 // Back projection planning: number of view splits (partials) and slice chunking.
 struct BackPlan {
-    int ppw;     // pixels per wavefront (2; 1 with Kahan compensation)
     int nsplit;  // view groups reduced at the end (1 = write directly)
     int kpl;     // slices per lane (chunk = 64*kpl slices)
     int nchunk;  // slice chunks
     int cap;     // LDS floats per wave for the detector column
     int unit;    // every W = mag*dv/dr <= 2 with radius-1 row taps: the unit-sum weights (coeff_power 1)
 };
-BackPlan plan_back(const DevGeom &g, int npix, int nviews, int ppw);
+BackPlan plan_back(const DevGeom &g, int npix, int nviews);
 int64_t back_part_bytes(const DevGeom &g, int chunk, int nviews);  // view-group partials
 // The most partials any pixel count up to n needs (the split grows as the
 // count falls, so a smaller subset can need more than a larger one).
@@ -86,7 +88,7 @@ int64_t fwd_aux_bytes(const DevGeom &g, int chunk, int nviews);
 // their workspace (chunking large pixel sets); the _tab variants take a
 // table that is already built.
 int launch_back(const DevGeom &g, const float *angles, const void *sino, int sino_dtype, const float *weights,
-                const int32_t *pix, int npix, const int32_t *views, int nviews, int coeff_power, int kahan,
+                const int32_t *pix, int npix, const int32_t *views, int nviews, int coeff_power,
                 float scale, int accumulate, float *out, void *ws, hipStream_t st);
 int launch_fwd(const DevGeom &g, const float *angles, const float *vox, const int32_t *pix, int npix,
                const int32_t *views, int nviews, void *sino, int sino_dtype, int layout, int accumulate,
@@ -99,7 +101,7 @@ int launch_fwd(const DevGeom &g, const float *angles, const float *vox, const in
 // size that back_part_bytes() sized part for (the split grows as the pixel
 // count falls, so a smaller last chunk planned on its own could need more).
 int launch_back_tab(const DevGeom &g, const float *T, int npix, const void *sino, int sino_dtype,
-                    const float *weights, const int32_t *views, int nviews, int coeff_power, int kahan, float scale,
+                    const float *weights, const int32_t *views, int nviews, int coeff_power, float scale,
                     int accumulate, float *out, float *part, hipStream_t st, int *defer = nullptr,
                     int plan_npix = 0);
 // The cylinder of table slot s is vox + s * pstride * Nz.  steps: kFwdBin

@@ -130,7 +130,7 @@ int mbx_back_project_noalloc(const mbx_geom *g, const float *angles, const void 
         rc = launch_table(d, angles, pix + c0, n, views, nviews, T, NC, st);
         if (rc != MBX_OK) return rc;
         // every chunk planned as a full one, as mbx_back_project does
-        rc = launch_back_tab(d, T, n, sino, sino_dtype, weights, views, nviews, coeff_power, 0, scale, accumulate,
+        rc = launch_back_tab(d, T, n, sino, sino_dtype, weights, views, nviews, coeff_power, scale, accumulate,
                              out + (size_t)c0 * d.Nz, part, st, nullptr, chunk);
         if (rc != MBX_OK) return rc;
     }

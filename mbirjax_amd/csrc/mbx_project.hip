@@ -133,7 +133,6 @@ int make_dev_geom(const mbx_geom *g, DevGeom *d) {
 constexpr int kRec = 16;
 constexpr int kFill = 4;  // 64-element groups loaded per round (forward staging)
 constexpr int kBackFill = 3;  // 64-row chunks loaded per round (back projection)
-constexpr int kKahanViews = 16;  // views summed plainly between Kahan steps
 constexpr int64_t kTableBudget = 512ll << 20;
 
 int table_chunk(const DevGeom &g, int npix, int nviews) {
@@ -291,29 +290,25 @@ int launch_table(const DevGeom &g, const float *angles, const int32_t *pix, int 
 // Back projection
 // ======================================================================
 
-BackPlan plan_back(const DevGeom &g, int npix, int nviews, int ppw) {
+BackPlan plan_back(const DevGeom &g, int npix, int nviews) {
     BackPlan p;
     const int per_lane = (g.Nz + kWave - 1) / kWave;
     p.kpl = per_lane <= 2 ? 2 : per_lane <= 4 ? 4 : per_lane <= 8 ? 8 : 16;
     p.nchunk = (g.Nz + kWave * p.kpl - 1) / (kWave * p.kpl);
-    p.ppw = ppw;
-    const int64_t units = ((int64_t)npix + ppw - 1) / ppw * p.nchunk;  // waves per view group
+    const int64_t units = ((int64_t)npix + 1) / 2 * p.nchunk;  // waves per view group (two pixels each)
     // Small subsets: split the views over the XCDs so that each XCD's L2
     // streams only its share of the sinogram, keeping every wave of the pass
-    // resident at once (7 waves per SIMD); large passes are VALU-bound.  With
-    // two pixels per wave the split (a multiple of the 8 XCDs) grows while the
-    // pass has fewer than 4096 waves, every group keeps 4 views or more and
-    // the partials stay within 64 MiB.
-    const int64_t resident = 7LL * 4 * 256;
+    // resident at once (7 waves per SIMD); large passes are VALU-bound.  The
+    // split (a multiple of the 8 XCDs) grows while the pass has fewer than
+    // 4096 waves, every group keeps 4 views or more and the partials stay
+    // within 64 MiB.
     if (nviews < 8 || units > 16384) {
         p.nsplit = 1;
-    } else if (ppw == 2) {
+    } else {
         p.nsplit = 8;
         while (units * p.nsplit < 4096 && p.nsplit < 64 && nviews >= 8 * p.nsplit &&
                2 * p.nsplit * (int64_t)npix * g.Nz * (int64_t)sizeof(float) <= (64ll << 20))
             p.nsplit *= 2;
-    } else {
-        p.nsplit = units * 8 <= resident ? 8 : units * 4 <= resident ? 4 : 2;
     }
     // LDS detector-column capacity per pixel: the rows one slice chunk can
     // reach, from the largest pixel magnification in the recon disc (plus
@@ -337,13 +332,10 @@ BackPlan plan_back(const DevGeom &g, int npix, int nviews, int ppw) {
     return p;
 }
 
-// View-group partials of either pixels-per-wave plan.
+// View-group partials of the plan of `chunk` pixels.
 int64_t back_part_bytes(const DevGeom &g, int chunk, int nviews) {
-    int64_t part = 0;
-    for (int ppw = 1; ppw <= 2; ++ppw) {
-        const BackPlan p = plan_back(g, chunk, nviews, ppw);
-        if (p.nsplit > 1) part = std::max<int64_t>(part, (int64_t)p.nsplit * chunk * g.Nz * (int64_t)sizeof(float));
-    }
+    const BackPlan p = plan_back(g, chunk, nviews);
+    const int64_t part = p.nsplit > 1 ? (int64_t)p.nsplit * chunk * g.Nz * (int64_t)sizeof(float) : 0;
     return (part + 255) & ~255ll;
 }
 
@@ -353,17 +345,15 @@ int64_t back_part_bytes(const DevGeom &g, int chunk, int nviews) {
 // maximum sits at the run's last count: found by bisection for each split.
 int64_t back_part_bound(const DevGeom &g, int n, int nviews) {
     int64_t best = 0;
-    for (int ppw = 1; ppw <= 2; ++ppw) {
-        for (int s = 2; s <= 64; s *= 2) {
-            if (n < 1 || plan_back(g, 1, nviews, ppw).nsplit < s) break;
-            int lo = 1, hi = n;  // largest count whose split is >= s
-            while (lo < hi) {
-                const int mid = lo + (hi - lo + 1) / 2;
-                if (plan_back(g, mid, nviews, ppw).nsplit >= s) lo = mid; else hi = mid - 1;
-            }
-            const BackPlan p = plan_back(g, lo, nviews, ppw);
-            best = std::max<int64_t>(best, (int64_t)p.nsplit * lo * g.Nz * (int64_t)sizeof(float));
+    for (int s = 2; s <= 64; s *= 2) {
+        if (n < 1 || plan_back(g, 1, nviews).nsplit < s) break;
+        int lo = 1, hi = n;  // largest count whose split is >= s
+        while (lo < hi) {
+            const int mid = lo + (hi - lo + 1) / 2;
+            if (plan_back(g, mid, nviews).nsplit >= s) lo = mid; else hi = mid - 1;
         }
+        const BackPlan p = plan_back(g, lo, nviews);
+        best = std::max<int64_t>(best, (int64_t)p.nsplit * lo * g.Nz * (int64_t)sizeof(float));
     }
     return (best + 255) & ~255ll;
 }
@@ -614,12 +604,12 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *_
 
 // TAIL: the launch covers only the last, partial slice chunk (lanes past the
 // last slice repeat it); otherwise every chunk of the launch is full.
-template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN, bool WGT, bool TAIL>
+template <int GEOM, int KPL, int RAD, typename ET, int POW, bool WGT, bool TAIL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void back_kernel(
     DevGeom g, const float *__restrict__ T, const ET *__restrict__ sino, const float *__restrict__ w, int npix,
     const int32_t *__restrict__ views, int nviews, int nsplit, int chunk0, int nchunk, int cap, int flags, float scale,
     int accumulate, float *__restrict__ out, float *__restrict__ part) {
-    constexpr int PPW = KAHAN ? 1 : 2;  // pixels per wave
+    constexpr int PPW = 2;  // pixels per wave
     extern __shared__ float lds[];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -629,21 +619,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
     const int q = __builtin_amdgcn_readfirstlane((grp / nchunk) * 4 + wave);
     const int sA = PPW * q;
     if (sA >= npix) return;  // whole wavefront leaves; no block barriers below
-    const bool hasB = PPW == 2 && sA + 1 < npix;
+    const bool hasB = sA + 1 < npix;
     const int sB = hasB ? sA + 1 : sA;
     const int k0 = chunk * kWave * KPL;
     const int kend = min(g.Nz, k0 + kWave * KPL);
     const int vb = (int)(((int64_t)nviews * split) / nsplit);
     const int ve = (int)(((int64_t)nviews * (split + 1)) / nsplit);
     float *colA = lds + wave * PPW * cap;
-    float *colB = colA + cap;  // PPW == 2 only
+    float *colB = colA + cap;
 
-    // KAHAN: plain float32 sums over blocks of kKahanViews views, added to the
-    // total with Kahan compensation (compensated summation across views at a
-    // fraction of the per-view cost).
-    float accA[KPL], accB[KPL], tot[KPL], cmp[KPL];
+    float accA[KPL], accB[KPL];
 #pragma unroll
-    for (int i = 0; i < KPL; ++i) accA[i] = accB[i] = tot[i] = cmp[i] = 0.0f;
+    for (int i = 0; i < KPL; ++i) accA[i] = accB[i] = 0.0f;
     for (int vi = vb; vi < ve; ++vi) {
         const int vd = views ? views[vi] : vi;
         const float *recA = T + ((size_t)vi * npix + sA) * kRec;
@@ -653,20 +640,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
         if (hasB)
             back_pixel_view<GEOM, KPL, RAD, ET, POW, WGT, TAIL>(g, recB, sino, w, vd, k0, kend, cap, flags, colB, lane,
                                                                 accB);
-        if (KAHAN && ((vi - vb) % kKahanViews == kKahanViews - 1 || vi == ve - 1)) {  // wave-uniform
-#pragma unroll
-            for (int i = 0; i < KPL; ++i) {
-                const float y = accA[i] - cmp[i];
-                const float t = tot[i] + y;
-                cmp[i] = (t - tot[i]) - y;
-                tot[i] = t;
-                accA[i] = 0.0f;
-            }
-        }
-    }
-    if (KAHAN) {
-#pragma unroll
-        for (int i = 0; i < KPL; ++i) accA[i] = tot[i];
     }
 #pragma unroll
     for (int i = 0; i < KPL; ++i) {
@@ -700,121 +673,114 @@ __global__ __launch_bounds__(256) void back_reduce_kernel(const float *__restric
     }
 }
 
-template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN, bool WGT, bool TAIL>
+template <int GEOM, int KPL, int RAD, typename ET, int POW, bool WGT, bool TAIL>
 static void launch_back_range(const DevGeom &g, const BackPlan &pl, const float *T, const ET *sino, const float *w,
                               int npix, const int32_t *views, int nviews, int chunk0, int nch, float scale,
                               int accumulate, float *out, float *part, hipStream_t st) {
-    constexpr int PPW = KAHAN ? 1 : 2;  // pixels per wave, as in back_kernel
+    constexpr int PPW = 2;  // pixels per wave, as in back_kernel
     const int64_t groups = ((int64_t)npix + 4 * PPW - 1) / (4 * PPW);  // 4 waves per block
     const int64_t blocks = groups * nch * pl.nsplit;
     const size_t lds = GEOM == MBX_GEOM_CONE ? 4 * PPW * (size_t)pl.cap * sizeof(float) : 0;
     const bool vec4 = g.R % 4 == 0 && ((uintptr_t)sino & (4 * sizeof(ET) - 1)) == 0;
     const int flags = (pl.unit ? kBackUnit : 0) | (vec4 ? kBackVec4 : 0);
-    hipLaunchKernelGGL((back_kernel<GEOM, KPL, RAD, ET, POW, KAHAN, WGT, TAIL>), dim3((unsigned)blocks), dim3(256), lds,
+    hipLaunchKernelGGL((back_kernel<GEOM, KPL, RAD, ET, POW, WGT, TAIL>), dim3((unsigned)blocks), dim3(256), lds,
                        st, g, T, sino, w, npix, views, nviews, pl.nsplit, chunk0, nch, pl.cap, flags, scale,
                        accumulate, out, part);
 }
 
 // Full slice chunks in one launch, a partial last chunk in a second one.
-template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN, bool WGT>
+template <int GEOM, int KPL, int RAD, typename ET, int POW, bool WGT>
 static void launch_back_w(const DevGeom &g, const BackPlan &pl, const float *T, const ET *sino, const float *w,
                           int npix, const int32_t *views, int nviews, float scale, int accumulate, float *out,
                           float *part, hipStream_t st) {
     const int full = g.Nz / (kWave * KPL);
     if (full > 0)
-        launch_back_range<GEOM, KPL, RAD, ET, POW, KAHAN, WGT, false>(g, pl, T, sino, w, npix, views, nviews, 0, full,
+        launch_back_range<GEOM, KPL, RAD, ET, POW, WGT, false>(g, pl, T, sino, w, npix, views, nviews, 0, full,
                                                                       scale, accumulate, out, part, st);
     if (full < pl.nchunk)
-        launch_back_range<GEOM, KPL, RAD, ET, POW, KAHAN, WGT, true>(g, pl, T, sino, w, npix, views, nviews, full, 1,
+        launch_back_range<GEOM, KPL, RAD, ET, POW, WGT, true>(g, pl, T, sino, w, npix, views, nviews, full, 1,
                                                                      scale, accumulate, out, part, st);
 }
 
-template <int GEOM, int KPL, int RAD, typename ET, int POW, bool KAHAN>
+template <int GEOM, int KPL, int RAD, typename ET, int POW>
 static void launch_back_t(const DevGeom &g, const BackPlan &pl, const float *T, const ET *sino, const float *w,
                           int npix, const int32_t *views, int nviews, float scale, int accumulate, float *out,
                           float *part, hipStream_t st) {
     if (w)
-        launch_back_w<GEOM, KPL, RAD, ET, POW, KAHAN, true>(g, pl, T, sino, w, npix, views, nviews, scale, accumulate,
+        launch_back_w<GEOM, KPL, RAD, ET, POW, true>(g, pl, T, sino, w, npix, views, nviews, scale, accumulate,
                                                             out, part, st);
     else
-        launch_back_w<GEOM, KPL, RAD, ET, POW, KAHAN, false>(g, pl, T, sino, w, npix, views, nviews, scale, accumulate,
+        launch_back_w<GEOM, KPL, RAD, ET, POW, false>(g, pl, T, sino, w, npix, views, nviews, scale, accumulate,
                                                              out, part, st);
 }
 
 template <int GEOM, int KPL, int RAD>
 static int dispatch_back_3(const DevGeom &g, const BackPlan &pl, const float *T, const void *sino, int dt,
-                           const float *w, int npix, const int32_t *views, int nviews, int pw, int kahan, float scale,
+                           const float *w, int npix, const int32_t *views, int nviews, int pw, float scale,
                            int accumulate, float *out, float *part, hipStream_t st) {
     if (pw == 2) {
-        if (dt != MBX_DTYPE_F32 || kahan)
-            return fail(MBX_EUNSUPPORTED, "coeff_power=2 supports float32 input without Kahan summation");
-        launch_back_t<GEOM, KPL, RAD, float, 2, false>(g, pl, T, (const float *)sino, w, npix, views, nviews, scale,
-                                                       accumulate, out, part, st);
+        if (dt != MBX_DTYPE_F32) return fail(MBX_EUNSUPPORTED, "coeff_power=2 supports float32 input only");
+        launch_back_t<GEOM, KPL, RAD, float, 2>(g, pl, T, (const float *)sino, w, npix, views, nviews, scale,
+                                                accumulate, out, part, st);
     } else if (dt == MBX_DTYPE_F32) {
-        if (kahan)
-            launch_back_t<GEOM, KPL, RAD, float, 1, true>(g, pl, T, (const float *)sino, w, npix, views, nviews, scale,
-                                                          accumulate, out, part, st);
-        else
-            launch_back_t<GEOM, KPL, RAD, float, 1, false>(g, pl, T, (const float *)sino, w, npix, views, nviews,
-                                                           scale, accumulate, out, part, st);
+        launch_back_t<GEOM, KPL, RAD, float, 1>(g, pl, T, (const float *)sino, w, npix, views, nviews, scale,
+                                                accumulate, out, part, st);
+    } else if (dt == MBX_DTYPE_BF16) {
+        launch_back_t<GEOM, KPL, RAD, bf16, 1>(g, pl, T, (const bf16 *)sino, w, npix, views, nviews, scale,
+                                               accumulate, out, part, st);
     } else {
-        if (kahan)
-            launch_back_t<GEOM, KPL, RAD, bf16, 1, true>(g, pl, T, (const bf16 *)sino, w, npix, views, nviews, scale,
-                                                         accumulate, out, part, st);
-        else
-            launch_back_t<GEOM, KPL, RAD, bf16, 1, false>(g, pl, T, (const bf16 *)sino, w, npix, views, nviews, scale,
-                                                          accumulate, out, part, st);
+        launch_back_t<GEOM, KPL, RAD, f16, 1>(g, pl, T, (const f16 *)sino, w, npix, views, nviews, scale, accumulate,
+                                              out, part, st);
     }
     return MBX_OK;
 }
 
 template <int GEOM, int KPL>
 static int dispatch_back_2(const DevGeom &g, const BackPlan &pl, const float *T, const void *sino, int dt,
-                           const float *w, int npix, const int32_t *views, int nviews, int pw, int kahan, float scale,
+                           const float *w, int npix, const int32_t *views, int nviews, int pw, float scale,
                            int accumulate, float *out, float *part, hipStream_t st) {
     if (g.rh == 1)
-        return dispatch_back_3<GEOM, KPL, 1>(g, pl, T, sino, dt, w, npix, views, nviews, pw, kahan, scale, accumulate,
-                                             out, part, st);
-    return dispatch_back_3<GEOM, KPL, 2>(g, pl, T, sino, dt, w, npix, views, nviews, pw, kahan, scale, accumulate, out,
-                                         part, st);
+        return dispatch_back_3<GEOM, KPL, 1>(g, pl, T, sino, dt, w, npix, views, nviews, pw, scale, accumulate, out,
+                                             part, st);
+    return dispatch_back_3<GEOM, KPL, 2>(g, pl, T, sino, dt, w, npix, views, nviews, pw, scale, accumulate, out, part,
+                                         st);
 }
 
 template <int GEOM>
 static int dispatch_back_1(const DevGeom &g, const BackPlan &pl, const float *T, const void *sino, int dt,
-                           const float *w, int npix, const int32_t *views, int nviews, int pw, int kahan, float scale,
+                           const float *w, int npix, const int32_t *views, int nviews, int pw, float scale,
                            int accumulate, float *out, float *part, hipStream_t st) {
     switch (pl.kpl) {
         case 4:
-            return dispatch_back_2<GEOM, 4>(g, pl, T, sino, dt, w, npix, views, nviews, pw, kahan, scale, accumulate,
-                                            out, part, st);
+            return dispatch_back_2<GEOM, 4>(g, pl, T, sino, dt, w, npix, views, nviews, pw, scale, accumulate, out,
+                                            part, st);
         case 2:
-            return dispatch_back_2<GEOM, 2>(g, pl, T, sino, dt, w, npix, views, nviews, pw, kahan, scale, accumulate,
-                                            out, part, st);
+            return dispatch_back_2<GEOM, 2>(g, pl, T, sino, dt, w, npix, views, nviews, pw, scale, accumulate, out,
+                                            part, st);
         case 8:
-            return dispatch_back_2<GEOM, 8>(g, pl, T, sino, dt, w, npix, views, nviews, pw, kahan, scale, accumulate,
-                                            out, part, st);
+            return dispatch_back_2<GEOM, 8>(g, pl, T, sino, dt, w, npix, views, nviews, pw, scale, accumulate, out,
+                                            part, st);
         default:
-            return dispatch_back_2<GEOM, 16>(g, pl, T, sino, dt, w, npix, views, nviews, pw, kahan, scale, accumulate,
-                                             out, part, st);
+            return dispatch_back_2<GEOM, 16>(g, pl, T, sino, dt, w, npix, views, nviews, pw, scale, accumulate, out,
+                                             part, st);
     }
 }
 
 // Back projection of pixels whose table T (nviews x npix records) is built.
 int launch_back_tab(const DevGeom &g, const float *T, int npix, const void *sino, int sino_dtype,
-                    const float *weights, const int32_t *views, int nviews, int coeff_power, int kahan, float scale,
+                    const float *weights, const int32_t *views, int nviews, int coeff_power, float scale,
                     int accumulate, float *out, float *part, hipStream_t st, int *defer, int plan_npix) {
     if (defer) *defer = 0;
     if (npix == 0) return MBX_OK;
-    // Kahan keeps a compensation per slice: one pixel per wave keeps it in registers
-    const BackPlan pl = plan_back(g, std::max(npix, plan_npix), nviews, kahan ? 1 : 2);
+    const BackPlan pl = plan_back(g, std::max(npix, plan_npix), nviews);
     if (pl.nsplit > 1 && !part) return fail(MBX_EWORKSPACE, "back projection needs a partials workspace");
     int rc;
     if (is_fan(g))
         rc = dispatch_back_1<MBX_GEOM_CONE>(g, pl, T, sino, sino_dtype, weights, npix, views, nviews, coeff_power,
-                                            kahan, scale, accumulate, out, part, st);
+                                            scale, accumulate, out, part, st);
     else
         rc = dispatch_back_1<MBX_GEOM_PARALLEL>(g, pl, T, sino, sino_dtype, weights, npix, views, nviews, coeff_power,
-                                                kahan, scale, accumulate, out, part, st);
+                                                scale, accumulate, out, part, st);
     if (rc != MBX_OK) return rc;
     MBX_LAUNCH_CHECK();
     if (pl.nsplit > 1 && defer) {
@@ -832,7 +798,7 @@ int launch_back_tab(const DevGeom &g, const float *T, int npix, const void *sino
 }
 
 int launch_back(const DevGeom &g, const float *angles, const void *sino, int sino_dtype, const float *weights,
-                const int32_t *pix, int npix, const int32_t *views, int nviews, int coeff_power, int kahan, float scale,
+                const int32_t *pix, int npix, const int32_t *views, int nviews, int coeff_power, float scale,
                 int accumulate, float *out, void *ws, hipStream_t st) {
     if (npix == 0 || nviews == 0) return MBX_OK;
     if (!ws) return fail(MBX_EWORKSPACE, "back projection needs a workspace");
@@ -846,7 +812,7 @@ int launch_back(const DevGeom &g, const float *angles, const void *sino, int sin
         int rc = launch_table(g, angles, pix + c0, n, views, nviews, T, NC, st);
         if (rc != MBX_OK) return rc;
         // every chunk planned as a full one: the partials were sized for it
-        rc = launch_back_tab(g, T, n, sino, sino_dtype, weights, views, nviews, coeff_power, kahan, scale, accumulate,
+        rc = launch_back_tab(g, T, n, sino, sino_dtype, weights, views, nviews, coeff_power, scale, accumulate,
                              out + (size_t)c0 * g.Nz, part, st, nullptr, chunk);
         if (rc != MBX_OK) return rc;
     }
@@ -1800,7 +1766,7 @@ int mbx_forward_project(const mbx_geom *g, const float *angles, const float *vox
     if (npix < 0) return fail(MBX_EINVAL, "npix must be >= 0");
     if (!views) nviews = d.V;
     if (nviews < 0) return fail(MBX_EINVAL, "nviews must be >= 0");
-    if (!dtype_ok(sino_dtype)) return fail(MBX_EINVAL, "unknown sinogram dtype");
+    if (!dtype_fwd_ok(sino_dtype)) return fail(MBX_EINVAL, "unknown sinogram dtype");
     if (sino_layout != MBX_LAYOUT_VCR && sino_layout != MBX_LAYOUT_VRC)
         return fail(MBX_EINVAL, "unknown sinogram layout");
     if (!angles || !sino || (npix > 0 && (!vox || !pix)))
@@ -1850,7 +1816,7 @@ int mbx_forward_apply(const mbx_geom *g, const void *plan, int64_t plan_bytes, c
     if (npix < 0) return fail(MBX_EINVAL, "npix must be >= 0");
     if (!views) nviews = d.V;
     if (nviews < 0) return fail(MBX_EINVAL, "nviews must be >= 0");
-    if (!dtype_ok(sino_dtype)) return fail(MBX_EINVAL, "unknown sinogram dtype");
+    if (!dtype_fwd_ok(sino_dtype)) return fail(MBX_EINVAL, "unknown sinogram dtype");
     if (sino_layout != MBX_LAYOUT_VCR && sino_layout != MBX_LAYOUT_VRC)
         return fail(MBX_EINVAL, "unknown sinogram layout");
     const int chunk = table_chunk(d, std::max(npix, 1), std::max(nviews, 1));
@@ -1879,8 +1845,8 @@ int mbx_forward_apply(const mbx_geom *g, const void *plan, int64_t plan_bytes, c
 
 int mbx_back_project(const mbx_geom *g, const float *angles, const void *sino, int32_t sino_dtype,
                      int32_t num_sino_views, const float *weights, const int32_t *pix, int32_t npix,
-                     const int32_t *views, int32_t nviews, int32_t coeff_power, int32_t kahan, float scale,
-                     int32_t accumulate, float *out, void *workspace, int64_t workspace_bytes, void *stream) {
+                     const int32_t *views, int32_t nviews, int32_t coeff_power, float scale, int32_t accumulate,
+                     float *out, void *workspace, int64_t workspace_bytes, void *stream) {
     DevGeom d;
     int rc = make_dev_geom(g, &d);
     if (rc != MBX_OK) return rc;
@@ -1904,8 +1870,8 @@ int mbx_back_project(const mbx_geom *g, const float *angles, const void *sino, i
             MBX_HIP_CHECK(hipMemsetAsync(out, 0, (size_t)npix * d.Nz * sizeof(float), (hipStream_t)stream));
         return MBX_OK;
     }
-    return launch_back(d, angles, sino, sino_dtype, weights, pix, npix, views, nviews, coeff_power, kahan, scale,
-                       accumulate, out, workspace, (hipStream_t)stream);
+    return launch_back(d, angles, sino, sino_dtype, weights, pix, npix, views, nviews, coeff_power, scale, accumulate,
+                       out, workspace, (hipStream_t)stream);
 }
 
 int mbx_sino_transpose(const void *src, int32_t src_dtype, int32_t src_layout, void *dst, int32_t dst_dtype,
@@ -1923,7 +1889,16 @@ int mbx_sino_transpose(const void *src, int32_t src_dtype, int32_t src_layout, v
     const int B = src_layout == MBX_LAYOUT_VRC ? num_channels : num_rows;
     const bool same = src_layout == dst_layout;
     hipStream_t st = (hipStream_t)stream;
-    if (src_dtype == MBX_DTYPE_F32 && dst_dtype == MBX_DTYPE_F32)
+    const bool f16_src = src_dtype == MBX_DTYPE_F16, f16_dst = dst_dtype == MBX_DTYPE_F16;
+    if ((f16_src && dst_dtype == MBX_DTYPE_BF16) || (f16_dst && src_dtype == MBX_DTYPE_BF16))
+        return fail(MBX_EUNSUPPORTED, "no direct float16 <-> bfloat16 conversion; go through float32");
+    if (f16_src && f16_dst)
+        launch_transpose_t<f16, f16>(src, dst, num_views, A, B, same, st);
+    else if (f16_src)
+        launch_transpose_t<f16, float>(src, dst, num_views, A, B, same, st);
+    else if (f16_dst)
+        launch_transpose_t<float, f16>(src, dst, num_views, A, B, same, st);
+    else if (src_dtype == MBX_DTYPE_F32 && dst_dtype == MBX_DTYPE_F32)
         launch_transpose_t<float, float>(src, dst, num_views, A, B, same, st);
     else if (src_dtype == MBX_DTYPE_F32)
         launch_transpose_t<float, bf16>(src, dst, num_views, A, B, same, st);

@@ -204,6 +204,17 @@ __global__ __launch_bounds__(256) void dot_kernel(const TA *__restrict__ a, cons
     if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
+template <typename TA>
+static void launch_dot_a(const TA *a, const void *b, int bdt, const float *w, int64_t n, int nb, double *part,
+                         hipStream_t st) {
+    if (bdt == MBX_DTYPE_F32)
+        hipLaunchKernelGGL((dot_kernel<TA, float>), dim3(nb), dim3(256), 0, st, a, (const float *)b, w, n, part);
+    else if (bdt == MBX_DTYPE_BF16)
+        hipLaunchKernelGGL((dot_kernel<TA, bf16>), dim3(nb), dim3(256), 0, st, a, (const bf16 *)b, w, n, part);
+    else
+        hipLaunchKernelGGL((dot_kernel<TA, f16>), dim3(nb), dim3(256), 0, st, a, (const f16 *)b, w, n, part);
+}
+
 // e = e - float(alpha) * d: thread t updates elements 4t .. 4t+3 with streaming
 // (nontemporal) accesses, the grid covering n/4 threads; the remaining (or, for
 // unaligned arrays, all) elements go element-wise over the grid.
@@ -234,21 +245,26 @@ __global__ __launch_bounds__(256) void axpy_kernel(TE *__restrict__ e, const TD 
     axpy_body(e, d, (float)alpha[0], n);
 }
 
-static int launch_axpy(void *e, int edt, const void *d, int ddt, const double *alpha, int64_t n, hipStream_t st) {
+template <typename TE>
+static void launch_axpy_e(TE *e, const void *d, int ddt, const double *alpha, int64_t n, hipStream_t st) {
     const int64_t blocks = axpy_blocks(n);
-    if (n == 0) return MBX_OK;
-    if (edt == MBX_DTYPE_F32 && ddt == MBX_DTYPE_F32)
-        hipLaunchKernelGGL((axpy_kernel<float, float>), dim3((unsigned)blocks), dim3(256), 0, st, (float *)e,
-                           (const float *)d, alpha, n);
-    else if (edt == MBX_DTYPE_F32)
-        hipLaunchKernelGGL((axpy_kernel<float, bf16>), dim3((unsigned)blocks), dim3(256), 0, st, (float *)e,
-                           (const bf16 *)d, alpha, n);
-    else if (ddt == MBX_DTYPE_F32)
-        hipLaunchKernelGGL((axpy_kernel<bf16, float>), dim3((unsigned)blocks), dim3(256), 0, st, (bf16 *)e,
-                           (const float *)d, alpha, n);
+    if (ddt == MBX_DTYPE_F32)
+        hipLaunchKernelGGL((axpy_kernel<TE, float>), dim3((unsigned)blocks), dim3(256), 0, st, e, (const float *)d,
+                           alpha, n);
     else
-        hipLaunchKernelGGL((axpy_kernel<bf16, bf16>), dim3((unsigned)blocks), dim3(256), 0, st, (bf16 *)e,
-                           (const bf16 *)d, alpha, n);
+        hipLaunchKernelGGL((axpy_kernel<TE, bf16>), dim3((unsigned)blocks), dim3(256), 0, st, e, (const bf16 *)d,
+                           alpha, n);
+}
+
+// e (edt: f32, bf16 or f16) -= float(alpha[0]) * d (ddt: f32 or bf16)
+static int launch_axpy(void *e, int edt, const void *d, int ddt, const double *alpha, int64_t n, hipStream_t st) {
+    if (n == 0) return MBX_OK;
+    if (edt == MBX_DTYPE_F32)
+        launch_axpy_e((float *)e, d, ddt, alpha, n, st);
+    else if (edt == MBX_DTYPE_BF16)
+        launch_axpy_e((bf16 *)e, d, ddt, alpha, n, st);
+    else
+        launch_axpy_e((f16 *)e, d, ddt, alpha, n, st);
     MBX_LAUNCH_CHECK();
     return MBX_OK;
 }
@@ -379,12 +395,6 @@ __global__ __launch_bounds__(256) void update_recon_kernel(float *__restrict__ x
     if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
-// e -= float(alpha) * d, reading alpha from scal[1]
-template <typename TE, typename TD>
-__global__ __launch_bounds__(256) void axpy_scal_kernel(TE *__restrict__ e, const TD *__restrict__ d,
-                                                        const double *__restrict__ scal, int64_t n) {
-    axpy_body(e, d, (float)scal[1], n);
-}
 
 struct VcdLayout {
     int chunk;  // pixels per geometry table
@@ -463,18 +473,12 @@ int mbx_dot(const void *a, int32_t a_dtype, const void *b, int32_t b_dtype, cons
     const int nb = red_blocks(n);
     double *part = (double *)workspace;
     if (!b) b_dtype = a_dtype;
-    if (a_dtype == MBX_DTYPE_F32 && b_dtype == MBX_DTYPE_F32)
-        hipLaunchKernelGGL((dot_kernel<float, float>), dim3(nb), dim3(256), 0, st, (const float *)a,
-                           (const float *)b, w, n, part);
-    else if (a_dtype == MBX_DTYPE_F32)
-        hipLaunchKernelGGL((dot_kernel<float, bf16>), dim3(nb), dim3(256), 0, st, (const float *)a, (const bf16 *)b,
-                           w, n, part);
-    else if (b_dtype == MBX_DTYPE_F32)
-        hipLaunchKernelGGL((dot_kernel<bf16, float>), dim3(nb), dim3(256), 0, st, (const bf16 *)a, (const float *)b,
-                           w, n, part);
+    if (a_dtype == MBX_DTYPE_F32)
+        launch_dot_a((const float *)a, b, b_dtype, w, n, nb, part, st);
+    else if (a_dtype == MBX_DTYPE_BF16)
+        launch_dot_a((const bf16 *)a, b, b_dtype, w, n, nb, part, st);
     else
-        hipLaunchKernelGGL((dot_kernel<bf16, bf16>), dim3(nb), dim3(256), 0, st, (const bf16 *)a, (const bf16 *)b, w,
-                           n, part);
+        launch_dot_a((const f16 *)a, b, b_dtype, w, n, nb, part, st);
     MBX_LAUNCH_CHECK();
     return launch_sum_partials(part, nb, out, 0, st);
 }
@@ -482,27 +486,27 @@ int mbx_dot(const void *a, int32_t a_dtype, const void *b, int32_t b_dtype, cons
 int mbx_sino_axpy(void *e, int32_t e_dtype, const void *d, int32_t d_dtype, const double *alpha, int64_t n,
                   void *stream) {
     if (n < 0) return fail(MBX_EINVAL, "n must be >= 0");
-    if (!dtype_ok(e_dtype) || !dtype_ok(d_dtype)) return fail(MBX_EINVAL, "unknown dtype");
+    if (!dtype_ok(e_dtype) || !dtype_fwd_ok(d_dtype)) return fail(MBX_EINVAL, "unknown dtype");
     if (n > 0 && (!e || !d || !alpha)) return fail(MBX_EINVAL, "NULL device pointer");
     return launch_axpy(e, e_dtype, d, d_dtype, alpha, n, (hipStream_t)stream);
 }
 
 int64_t mbx_vcd_workspace_bytes(const mbx_geom *g, int32_t npix, int32_t delta_dtype) {
     DevGeom d;
-    if (make_dev_geom(g, &d) != MBX_OK || npix < 0 || !dtype_ok(delta_dtype)) return -1;
+    if (make_dev_geom(g, &d) != MBX_OK || npix < 0 || !dtype_fwd_ok(delta_dtype)) return -1;
     return (int64_t)vcd_layout(d, npix, delta_dtype).total;
 }
 
 int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_recon, void *e, int32_t e_dtype,
                           const float *weights, const float *fm_hessian, const int32_t *pix, int32_t npix,
                           double fm_constant, const mbx_qggmrf *qparams, const float *prox_input, double sigma_prox,
-                          int32_t positivity, int32_t delta_dtype, int32_t kahan, double *stats, void *workspace,
+                          int32_t positivity, int32_t delta_dtype, double *stats, void *workspace,
                           int64_t workspace_bytes, void *stream) {
     DevGeom d;
     int rc = make_dev_geom(g, &d);
     if (rc != MBX_OK) return rc;
     if (npix <= 0) return fail(MBX_EINVAL, "npix must be positive");
-    if (!dtype_ok(e_dtype) || !dtype_ok(delta_dtype)) return fail(MBX_EINVAL, "unknown dtype");
+    if (!dtype_ok(e_dtype) || !dtype_fwd_ok(delta_dtype)) return fail(MBX_EINVAL, "unknown dtype");
     if (!angles || !flat_recon || !e || !fm_hessian || !pix) return fail(MBX_EINVAL, "NULL device pointer");
     if (!prox_input) {
         rc = check_qparams(qparams);
@@ -569,7 +573,7 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
             MBX_HIP_CHECK(hipEventRecord(side->join, side->s));
             join.recorded = true;
         }
-        rc = launch_back_tab(d, T, n, e, e_dtype, weights, nullptr, d.V, 1, kahan, 1.0f, 0, bp + (size_t)c0 * d.Nz,
+        rc = launch_back_tab(d, T, n, e, e_dtype, weights, nullptr, d.V, 1, 1.0f, 0, bp + (size_t)c0 * d.Nz,
                              bpart, st, one_table ? &bsplit : nullptr, L.chunk);
         if (rc != MBX_OK) return rc;
     }
@@ -640,22 +644,8 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
         rc = launch_sum_partials(l1part, nb, stats + 0, 1, st);
         if (rc != MBX_OK) return rc;
     }
-    // (l) error sinogram update
-    const int64_t blocks = axpy_blocks(nsino);
-    if (e_dtype == MBX_DTYPE_F32 && delta_dtype == MBX_DTYPE_F32)
-        hipLaunchKernelGGL((axpy_scal_kernel<float, float>), dim3((unsigned)blocks), dim3(256), 0, st, (float *)e,
-                           (const float *)dsino, scal, nsino);
-    else if (e_dtype == MBX_DTYPE_F32)
-        hipLaunchKernelGGL((axpy_scal_kernel<float, bf16>), dim3((unsigned)blocks), dim3(256), 0, st, (float *)e,
-                           (const bf16 *)dsino, scal, nsino);
-    else if (delta_dtype == MBX_DTYPE_F32)
-        hipLaunchKernelGGL((axpy_scal_kernel<bf16, float>), dim3((unsigned)blocks), dim3(256), 0, st, (bf16 *)e,
-                           (const float *)dsino, scal, nsino);
-    else
-        hipLaunchKernelGGL((axpy_scal_kernel<bf16, bf16>), dim3((unsigned)blocks), dim3(256), 0, st, (bf16 *)e,
-                           (const bf16 *)dsino, scal, nsino);
-    MBX_LAUNCH_CHECK();
-    return MBX_OK;
+    // (l) error sinogram update, alpha read from scal[1]
+    return launch_axpy(e, e_dtype, dsino, delta_dtype, scal + 1, nsino, st);
 }
 
 }  // extern "C"

@@ -34,10 +34,18 @@ ReconParams = namedtuple('ReconParams', recon_param_names)
 
 _F32 = _lib.DTYPE_F32
 _BF16 = _lib.DTYPE_BF16
+_F16 = _lib.DTYPE_F16
+
+# precision mode -> (error-sinogram storage, delta-sinogram storage)
+PRECISIONS = {'fp32': (_F32, _F32), 'mixed': (_F32, _BF16), 'fp16': (_F16, _BF16), 'bf16': (_BF16, _BF16)}
 
 
 def _torch_dtype(dt):
-    return torch.float32 if dt == _F32 else torch.bfloat16
+    return {_F32: torch.float32, _BF16: torch.bfloat16, _F16: torch.float16}[dt]
+
+
+def _dtype_code(t):
+    return {torch.float32: _F32, torch.bfloat16: _BF16, torch.float16: _F16}[t.dtype]
 
 
 def _as_host_f32(a):
@@ -163,7 +171,7 @@ class TomographyModel(ParameterHandler):
         R, C = self.get_params('sinogram_shape')[1:]
         shape = (V, C, R) if dst_layout == _lib.LAYOUT_VCR else (V, R, C)
         dst = torch.empty(shape, dtype=_torch_dtype(dst_dtype), device=src.device)
-        src_dt = _F32 if src.dtype == torch.float32 else _BF16
+        src_dt = _dtype_code(src)
         _lib.check(_lib.lib().mbx_sino_transpose(_lib.ptr(src), src_dt, src_layout, _lib.ptr(dst), dst_dtype,
                                                  dst_layout, V, R, C, _lib.stream()))
         return dst
@@ -229,9 +237,8 @@ class TomographyModel(ParameterHandler):
         window beats the row-major register tile by more than the transpose costs)."""
         return self.from_channel_major(self._fwd(vox, pix, views, layout=_lib.LAYOUT_VCR))
 
-    def _back(self, sino_vcr, pix, views=None, coeff_power=1, weights=None, kahan=False, scale=1.0, out=None,
-              accumulate=False):
-        """Back project a channel-major sinogram (f32 or bf16) to cylinders (npix, Nz)."""
+    def _back(self, sino_vcr, pix, views=None, coeff_power=1, weights=None, scale=1.0, out=None, accumulate=False):
+        """Back project a channel-major sinogram (f32, bf16 or f16) to cylinders (npix, Nz)."""
         nz = self.get_params('recon_shape')[2]
         npix = int(pix.numel())
         nv = sino_vcr.shape[0] if views is None else int(views.numel())
@@ -239,19 +246,18 @@ class TomographyModel(ParameterHandler):
             out = torch.empty((npix, nz), dtype=torch.float32, device=sino_vcr.device)
         wsb = int(_lib.lib().mbx_back_workspace_bytes(self._geom, npix, nv))
         ws = torch.empty(wsb, dtype=torch.uint8, device=sino_vcr.device) if wsb > 0 else None
-        dt = _F32 if sino_vcr.dtype == torch.float32 else _BF16
         _lib.check(_lib.lib().mbx_back_project(
-            self._geom, _lib.ptr(self._angles()), _lib.ptr(sino_vcr), dt, int(sino_vcr.shape[0]), _lib.ptr(weights),
-            _lib.ptr(pix), npix, _lib.ptr(views), nv, int(coeff_power), int(kahan), float(scale), int(accumulate),
-            _lib.ptr(out), _lib.ptr(ws), wsb, _lib.stream()))
+            self._geom, _lib.ptr(self._angles()), _lib.ptr(sino_vcr), _dtype_code(sino_vcr), int(sino_vcr.shape[0]),
+            _lib.ptr(weights), _lib.ptr(pix), npix, _lib.ptr(views), nv, int(coeff_power), float(scale),
+            int(accumulate), _lib.ptr(out), _lib.ptr(ws), wsb, _lib.stream()))
         return out
 
     def _dot(self, a, b=None, w=None):
         """FP64 sum(a*b*w) on the device (deterministic); returns a Python float."""
         out = torch.zeros(1, dtype=torch.float64, device=a.device)
         ws = torch.empty(int(_lib.lib().mbx_dot_workspace_bytes()), dtype=torch.uint8, device=a.device)
-        adt = _F32 if a.dtype == torch.float32 else _BF16
-        bdt = adt if b is None else (_F32 if b.dtype == torch.float32 else _BF16)
+        adt = _dtype_code(a)
+        bdt = adt if b is None else _dtype_code(b)
         _lib.check(_lib.lib().mbx_dot(_lib.ptr(a), adt, _lib.ptr(b), bdt, _lib.ptr(w), int(a.numel()),
                                       _lib.ptr(out), _lib.ptr(ws), int(ws.numel()), _lib.stream()))
         return float(out.item())
@@ -405,9 +411,13 @@ class TomographyModel(ParameterHandler):
 
         precision: 'fp32' (plain); 'mixed' (error sinogram accumulated in float32,
         delta sinogram A*delta stored in bfloat16, FP64 step-size and cost
-        reductions); or 'bf16' (as 'mixed' with the error sinogram also stored in
-        bfloat16: least memory traffic, but round-to-nearest drops updates below
-        half a bfloat16 ulp).
+        reductions); 'fp16' (as 'mixed' with the error sinogram stored in IEEE
+        float16, the problem scaled by a power of two so that it stays in range:
+        half the residual traffic of 'mixed' within NRMSE 1e-3 of FP64); or
+        'bf16' (experimental: the error sinogram in bfloat16, whose 8-bit
+        significand drops late updates below half an ulp of the residual --
+        NRMSE ~3.4e-3 of FP64 on the 512^3 benchmark's sub-problem, above the
+        1e-3 bar the other modes meet).
         """
         if num_iterations is not None:
             warnings.warn('num_iterations has been deprecated and will be removed in a future release.\n'
@@ -464,8 +474,12 @@ class TomographyModel(ParameterHandler):
         the subset-update workspace (mbirjax/tomography_model.py:1110-1192).
         With channel_major=True, sinogram (and weights) are already (views, C, R)
         float32 device tensors."""
-        if precision not in ('fp32', 'mixed', 'bf16'):
-            raise ValueError("precision must be 'fp32', 'mixed' or 'bf16'")
+        if precision not in PRECISIONS:
+            raise ValueError("precision must be one of {}".format(sorted(PRECISIONS)))
+        if precision == 'fp16' and prox_input is not None:
+            # the proximal prior's curvature is sigma_prox**2 (tomography_model.py:1390 of the reference),
+            # which does not scale with the problem, so the scaled float16 run would be another update
+            raise ValueError("precision 'fp16' is not available for the proximal map")
         dev = self._device()
         recon_shape = self.get_params('recon_shape')
         rows, cols, nz = recon_shape
@@ -494,7 +508,7 @@ class TomographyModel(ParameterHandler):
         self._log('info', 'Initializing error sinogram')
         full = vcd_utils.gen_full_indices(recon_shape, use_ror_mask=self.use_ror_mask)
         full_d = torch.as_tensor(full, device=dev)
-        e_dtype = _BF16 if precision == 'bf16' else _F32
+        e_dtype, delta_dtype = PRECISIONS[precision]
         e32 = y.clone()
         alpha0 = 1.0
         if not zero_init:  # (A 0 = 0: alpha0 = 1 and e = y)
@@ -505,14 +519,29 @@ class TomographyModel(ParameterHandler):
             _lib.check(_lib.lib().mbx_sino_axpy(_lib.ptr(e32), _F32, _lib.ptr(Ax), _F32, _lib.ptr(alpha_dev),
                                                 int(e32.numel()), _lib.stream()))
             del Ax
-        if e_dtype == _BF16:
-            e = torch.empty(e32.shape, dtype=torch.bfloat16, device=dev)
-            _lib.check(_lib.lib().mbx_sino_transpose(_lib.ptr(e32), _F32, _lib.LAYOUT_VCR, _lib.ptr(e), _BF16,
+        # float16 storage: solve the problem scaled by a power of two s (data,
+        # recon, sigma_y, sigma_x and sigma_prox all divided by s) that puts the
+        # largest data or residual magnitude in [2^11, 2^12), far below the
+        # float16 overflow at 65504 and far above its subnormals.  Division by a
+        # power of two is exact, and the VCD update is equivariant under the
+        # scaling (delta scales by 1/s, alpha and the costs are unchanged), so
+        # only the storage rounding differs from the unscaled run.
+        scale = 1.0
+        if e_dtype == _F16:
+            peak = max(float(torch.max(torch.abs(y)).item()), float(torch.max(torch.abs(e32)).item()))
+            if peak > 0 and math.isfinite(peak):
+                scale = 2.0 ** (math.floor(math.log2(peak)) - 11)
+                e32.mul_(1.0 / scale)
+        if e_dtype != _F32:
+            e = torch.empty(e32.shape, dtype=_torch_dtype(e_dtype), device=dev)
+            _lib.check(_lib.lib().mbx_sino_transpose(_lib.ptr(e32), _F32, _lib.LAYOUT_VCR, _lib.ptr(e), e_dtype,
                                                      _lib.LAYOUT_VCR, V, R, C, _lib.stream()))
             del e32
         else:
             e = e32
         flat_recon = (np.float32(alpha0) * init_recon).reshape(-1, nz).contiguous()
+        if scale != 1.0:
+            flat_recon.mul_(1.0 / scale)
 
         if prox_input is not None:
             prox_input = self._tensor(prox_input)
@@ -522,6 +551,8 @@ class TomographyModel(ParameterHandler):
                 self._log('error', msg)
                 raise ValueError(msg)
             prox_input = prox_input.reshape(-1, nz).contiguous()
+            if scale != 1.0:
+                prox_input = prox_input * (1.0 / scale)
 
         self._log('info', 'Computing Hessian diagonal')
         w_h = w if w is not None else torch.ones_like(y)
@@ -529,7 +560,8 @@ class TomographyModel(ParameterHandler):
         del w_h, y
         if max_subset_size is None:
             max_subset_size = rows * cols
-        return VCDState(self, e, e_dtype, w, flat_recon, fm_hessian, prox_input, precision, max_subset_size)
+        return VCDState(self, e, e_dtype, delta_dtype, w, flat_recon, fm_hessian, prox_input, precision,
+                        max_subset_size, scale)
 
     def _log(self, level, msg):
         if self.logger is not None:
@@ -576,8 +608,8 @@ class TomographyModel(ParameterHandler):
             if nmae_update[i] < stop_threshold_change_pct / 100:
                 self._log('warning', 'Change threshold stopping condition reached')
                 break
-        self.last_error_sinogram = state.e
-        return self.reshape_recon(state.flat_recon), (fm_rmse[:num_iters], pm_loss[:num_iters],
+        self.last_error_sinogram = state.error_sinogram()
+        return self.reshape_recon(state.recon()), (fm_rmse[:num_iters], pm_loss[:num_iters],
                                                       nmae_update[:num_iters], alpha_values[:num_iters])
 
     @staticmethod
@@ -657,35 +689,44 @@ class ForwardPlan:
 
 class VCDState:
     """Device state of one VCD reconstruction: the recon (rows*cols, Nz), the
-    error sinogram e = y - A x in channel-major layout (float32, or bfloat16
-    for precision='bf16'), weights, Hessian diagonal and workspace.  iterate()
-    runs one partition through mbx_vcd_subset_update with no host syncs."""
+    error sinogram e = y - A x in channel-major layout (float32, float16 or
+    bfloat16 by precision), weights, Hessian diagonal and workspace.  With
+    float16 storage the state holds the problem scaled by 1/scale (a power of
+    two): recon() and error_sinogram() undo it.  iterate() runs one partition
+    through mbx_vcd_subset_update with no host syncs."""
 
-    def __init__(self, model, e, e_dtype, w, flat_recon, fm_hessian, prox_input, precision, max_subset_size):
+    def __init__(self, model, e, e_dtype, delta_dtype, w, flat_recon, fm_hessian, prox_input, precision,
+                 max_subset_size, scale=1.0):
         self.model = model
         self.e = e
         self.e_dtype = e_dtype
+        self.delta_dtype = delta_dtype
         self.w = w
         self.flat_recon = flat_recon
         self.fm_hessian = fm_hessian
         self.prox_input = prox_input
         self.precision = precision
-        sigma_y = model.get_params('sigma_y')
-        self.fm_constant = 1.0 / (float(sigma_y) ** 2.0)
-        self.qp = model._qggmrf_params()
+        self.scale = float(scale)
+        sigma_y = float(model.get_params('sigma_y')) / self.scale
+        self.fm_constant = 1.0 / (sigma_y ** 2.0)
+        b, sigma_x, p, q, T = model._qggmrf_params()
+        self.qp = (b, sigma_x / self.scale, p, q, T)
         self.qstruct = _lib.qggmrf_struct(*self.qp)
-        self.sigma_prox = float(model.get_params('sigma_prox'))
+        self.sigma_prox = float(model.get_params('sigma_prox')) / self.scale
         self.positivity = bool(model.get_params('positivity_flag'))
-        # Kahan-compensated back projection is available (mbx_vcd_subset_update's
-        # kahan flag) but changes neither mode's NRMSE against the FP64 oracle
-        # (1.228e-4 / 1.227e-4 mixed, 3.426e-3 / 3.427e-3 bf16 on the 512^3
-        # benchmark's sub-problem): the bfloat16 storage dominates.
-        self.kahan = False
-        self.delta_dtype = _F32 if precision == 'fp32' else _BF16
         self.wsb = int(_lib.lib().mbx_vcd_workspace_bytes(model._geom, int(max_subset_size), self.delta_dtype))
         self.ws = torch.empty(self.wsb, dtype=torch.uint8, device=e.device)
         self.stats = torch.zeros(4, dtype=torch.float64, device=e.device)
         self.avg_w = 1.0 if w is None else float(torch.mean(w, dtype=torch.float64).item())
+
+    def recon(self):
+        """The flat recon (rows*cols, Nz) in the caller's units."""
+        return self.flat_recon if self.scale == 1.0 else self.flat_recon * self.scale
+
+    def error_sinogram(self):
+        """The error sinogram (views, C, R) as float32 in the caller's units."""
+        e = self.e.float()
+        return e if self.scale == 1.0 else e * self.scale
 
     def update_subset(self, subset):
         """One subset update (vcd_subset_updater); subset is an int32 device tensor."""
@@ -694,7 +735,7 @@ class VCDState:
             m._geom, _lib.ptr(m._angles()), _lib.ptr(self.flat_recon), _lib.ptr(self.e), self.e_dtype,
             _lib.ptr(self.w), _lib.ptr(self.fm_hessian), _lib.ptr(subset), int(subset.numel()), self.fm_constant,
             self.qstruct, _lib.ptr(self.prox_input), self.sigma_prox, int(self.positivity), self.delta_dtype,
-            int(self.kahan), _lib.ptr(self.stats), _lib.ptr(self.ws), self.wsb, _lib.stream()))
+            _lib.ptr(self.stats), _lib.ptr(self.ws), self.wsb, _lib.stream()))
 
     def iterate(self, part, order):
         """All subsets of a partition (K, nS) in the given order (vcd_partition_iterator);
@@ -710,4 +751,4 @@ class VCDState:
         return math.sqrt(self.fm_constant * self.model._dot(self.e, None, self.w) / (n * self.avg_w))
 
     def error_rmse(self):
-        return math.sqrt(self.model._dot(self.e) / self.e.numel())
+        return self.scale * math.sqrt(self.model._dot(self.e) / self.e.numel())

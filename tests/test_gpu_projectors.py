@@ -196,20 +196,24 @@ def test_hessian_matches_impulse(gpu):
 
 
 @pytest.mark.parametrize("case", ["cone_m2", "cone_multichunk"])  # (1300 rows: whole 256-row column groups)
-def test_bf16_and_kahan_back_projection(gpu, case):
-    print("case", case, flush=True)
+@pytest.mark.parametrize("dtype", ["bf16", "fp16"])
+def test_two_byte_back_projection(gpu, case, dtype):
+    """Back projection of a bfloat16 or float16 channel-major sinogram (the
+    2-byte error sinograms of precision 'bf16' / 'fp16') against the oracle on
+    the same rounded values."""
+    print("case", case, dtype, flush=True)
     m, g = CASES[case]()
     rows, cols, nz = m.get_params('recon_shape')
     pix = torch.arange(rows * cols, dtype=torch.int32, device='cuda')
     rng = np.random.default_rng(14)
     y = rng.standard_normal(g.sinogram_shape).astype(np.float32)
-    y16 = torch.as_tensor(y, device='cuda').to(torch.bfloat16)
-    y_rounded = y16.float().cpu().numpy()
+    tdt, code = {"bf16": (torch.bfloat16, mj._lib.DTYPE_BF16), "fp16": (torch.float16, mj._lib.DTYPE_F16)}[dtype]
+    y2 = torch.as_tensor(y, device='cuda').to(tdt)
+    y_rounded = y2.float().cpu().numpy()
     ref = op.back(g, y_rounded, pix.cpu().numpy())
-    vcr16 = m.to_channel_major(y16.float(), dtype=mj._lib.DTYPE_BF16)
-    for kahan in (False, True):
-        b = m._back(vcr16, pix, kahan=kahan)
-        assert rel(b, ref) <= TOL
+    vcr2 = m.to_channel_major(y2.float(), dtype=code)
+    assert vcr2.dtype == tdt and torch.equal(m.from_channel_major(vcr2), y2.float())
+    assert rel(m._back(vcr2, pix), ref) <= TOL
 
 
 def test_transpose_round_trip_and_determinism(gpu):

@@ -181,16 +181,10 @@ def test_prox_map_matches_oracle(gpu, kind):
     assert nrmse <= 1e-3
 
 
-def test_c4_subproblem_mixed_and_fp32_vs_oracle(gpu):
-    """The 512^3 benchmark's precision check on its sub-problem: 64 views over
-    2 pi, 128 x 192 detector, same magnification (sdd 768, sid 384), 128^3
-    recon; 10 iterations in each precision mode against the FP64 oracle under
-    the same partitions and subset order.  'fp32' and 'mixed' (float32 error
-    sinogram, bfloat16 delta sinogram) meet NRMSE <=
-    1e-3.  'bf16' also stores the error sinogram in bfloat16; late updates
-    smaller than half a bfloat16 ulp of the residual are rounded away, which
-    leaves it at ~3.4e-3 of the oracle (its error against the phantom is the
-    same as fp32's), so it is held to 5e-3."""
+def c4_subproblem():
+    """The 512^3 benchmark's precision sub-problem: 64 views over 2 pi,
+    128 x 192 detector, same magnification (sdd 768, sid 384), 128^3 recon,
+    seeded ellipsoid phantom + 1 % noise."""
     V, R, C = 64, 128, 192
     ang = np.linspace(0, 2 * np.pi, V, endpoint=False)
     m = mj.ConeBeamModel((V, R, C), ang, source_detector_dist=768.0, source_iso_dist=384.0)
@@ -201,10 +195,22 @@ def test_c4_subproblem_mixed_and_fp32_vs_oracle(gpu):
     ror = np.flatnonzero(opart.get_2d_ror_mask(g.recon_shape))
     sino = op.forward(g, phantom.reshape(-1, 128)[ror], ror)
     sino = (sino + 0.01 * np.max(sino) * np.random.default_rng(1).standard_normal(sino.shape)).astype(np.float32)
+    return m, g, phantom, sino
+
+
+def test_c4_subproblem_precisions_vs_oracle(gpu):
+    """10 iterations in each precision mode against the FP64 oracle under the
+    same partitions and subset order.  'fp32', 'mixed' (float32 error
+    sinogram, bfloat16 delta sinogram) and 'fp16' (float16 error sinogram)
+    meet NRMSE <= 1e-3.  'bf16' (bfloat16 error sinogram) is experimental:
+    late updates smaller than half a bfloat16 ulp of the residual are rounded
+    away, which leaves it at ~3.4e-3 of the oracle (its error against the
+    phantom equals fp32's); its line guards that level, not the bar."""
+    m, g, phantom, sino = c4_subproblem()
     om = ovcd.OracleModel(g, sharpness=0.0)
     np.random.seed(11)
     ref, _ = om.recon(sino, max_iterations=10, stop_threshold_change_pct=0)
-    for precision, bar in (("fp32", 1e-3), ("mixed", 1e-3), ("bf16", 5e-3)):
+    for precision, bar in (("fp32", 1e-3), ("mixed", 1e-3), ("fp16", 1e-3), ("bf16", 5e-3)):
         np.random.seed(11)
         recon, _ = m.recon(sino, max_iterations=10, stop_threshold_change_pct=0, print_logs=False,
                            logfile_path=None, precision=precision)
@@ -213,6 +219,24 @@ def test_c4_subproblem_mixed_and_fp32_vs_oracle(gpu):
         print("C4 sub-problem %s NRMSE vs FP64 oracle %.3e (vs phantom %.3f)" % (
             precision, nrmse, np.linalg.norm(r - phantom) / np.linalg.norm(phantom)), flush=True)
         assert nrmse <= bar
+
+
+def test_fp16_scaling_is_exact(gpu):
+    """The float16 mode solves the problem scaled by a power of two: data
+    2^16 times larger (peak ~1.3e6, beyond float16's 65504) reconstructs to
+    2^16 times the same recon."""
+    m, g, phantom, sino = c4_subproblem()
+    out = []
+    for f in (1.0, 65536.0):
+        np.random.seed(11)
+        recon, d = m.recon(sino * np.float32(f), max_iterations=3, stop_threshold_change_pct=0, print_logs=False,
+                           logfile_path=None, precision='fp16')
+        out.append((recon, d['recon_params']['fm_rmse']))
+    assert float(np.max(sino)) * 65536.0 > 65504.0
+    diff = float(torch.max(torch.abs(out[1][0] - 65536.0 * out[0][0])) / torch.max(torch.abs(65536.0 * out[0][0])))
+    print("fp16 scaled recon: max relative difference %.3e" % diff, flush=True)
+    assert torch.isfinite(out[1][0]).all() and diff <= 1e-6
+    assert np.allclose(out[0][1], out[1][1], rtol=1e-6)
 
 
 def test_c1_parallel_64_recon_vs_oracle(gpu):

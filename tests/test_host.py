@@ -54,23 +54,23 @@ def test_validation_before_any_hip_call():
                                  None) == _lib.MBX_EINVAL
     assert L.mbx_forward_project(geom(psf_radius=3), fake, fake, fake, 1, None, 0, fake, 0, 0, 0, None, None, None,
                                  0, None) == _lib.MBX_EUNSUPPORTED
-    assert L.mbx_back_project(geom(sinogram_shape=(0, 4, 6)), fake, fake, 0, 8, None, fake, 1, None, 0, 1, 0, 1.0,
-                              0, fake, None, 0, None) == _lib.MBX_EINVAL
+    assert L.mbx_back_project(geom(sinogram_shape=(0, 4, 6)), fake, fake, 0, 8, None, fake, 1, None, 0, 1, 1.0, 0,
+                              fake, None, 0, None) == _lib.MBX_EINVAL
     # parallel beam needs slices == rows
-    assert L.mbx_back_project(geom(kind=0, recon_shape=(6, 6, 5)), fake, fake, 0, 8, None, fake, 1, None, 0, 1, 0,
+    assert L.mbx_back_project(geom(kind=0, recon_shape=(6, 6, 5)), fake, fake, 0, 8, None, fake, 1, None, 0, 1,
                               1.0, 0, fake, None, 0, None) == _lib.MBX_EINVAL
     # sinogram view count without view indices
-    assert L.mbx_back_project(geom(), fake, fake, 0, 7, None, fake, 1, None, 0, 1, 0, 1.0, 0, fake, None, 0,
+    assert L.mbx_back_project(geom(), fake, fake, 0, 7, None, fake, 1, None, 0, 1, 1.0, 0, fake, None, 0,
                               None) == _lib.MBX_EINVAL
     # bad coeff power and dtype
-    assert L.mbx_back_project(geom(), fake, fake, 0, 8, None, fake, 1, None, 0, 3, 0, 1.0, 0, fake, None, 0,
+    assert L.mbx_back_project(geom(), fake, fake, 0, 8, None, fake, 1, None, 0, 3, 1.0, 0, fake, None, 0,
                               None) == _lib.MBX_EINVAL
-    assert L.mbx_back_project(geom(), fake, fake, 5, 8, None, fake, 1, None, 0, 1, 0, 1.0, 0, fake, None, 0,
+    assert L.mbx_back_project(geom(), fake, fake, 5, 8, None, fake, 1, None, 0, 1, 1.0, 0, fake, None, 0,
                               None) == _lib.MBX_EINVAL
     # workspace too small
     need = L.mbx_back_workspace_bytes(geom(), 1, 8)
     assert need > 0
-    assert L.mbx_back_project(geom(), fake, fake, 0, 8, None, fake, 1, None, 0, 1, 0, 1.0, 0, fake, fake, need - 1,
+    assert L.mbx_back_project(geom(), fake, fake, 0, 8, None, fake, 1, None, 0, 1, 1.0, 0, fake, fake, need - 1,
                               None) == _lib.MBX_EWORKSPACE
     assert L.mbx_forward_project(geom(), fake, fake, fake, 1, None, 0, fake, 0, 0, 0, None, None, fake,
                                  L.mbx_forward_workspace_bytes(geom(), 1, 8) - 1, None) == _lib.MBX_EWORKSPACE
@@ -101,11 +101,18 @@ def test_validation_before_any_hip_call():
     assert L.mbx_qggmrf_grad_hess(fake, 4, 4, 4, fake, 2, q, fake, fake, None) == _lib.MBX_EINVAL
     # VCD update
     q = _lib.qggmrf_struct([1 / 6] * 6, 1.0, 2.0, 1.2, 1.0)
-    assert L.mbx_vcd_subset_update(geom(), fake, fake, fake, 0, None, fake, fake, 0, 1.0, q, None, 1.0, 0, 0, 0,
+    assert L.mbx_vcd_subset_update(geom(), fake, fake, fake, 0, None, fake, fake, 0, 1.0, q, None, 1.0, 0, 0,
                                    None, fake, 1 << 30, None) == _lib.MBX_EINVAL
-    assert L.mbx_vcd_subset_update(geom(), fake, fake, fake, 0, None, fake, fake, 4, 1.0, q, None, 1.0, 0, 0, 0,
+    assert L.mbx_vcd_subset_update(geom(), fake, fake, fake, 0, None, fake, fake, 4, 1.0, q, None, 1.0, 0, 0,
                                    None, fake, 16, None) == _lib.MBX_EWORKSPACE
     assert L.mbx_vcd_workspace_bytes(geom(), 4, 0) > L.mbx_vcd_workspace_bytes(geom(), 4, 1)
+    # float16 is an error-sinogram storage type, never a forward output or delta sinogram
+    assert L.mbx_forward_project(geom(), fake, fake, fake, 1, None, 0, fake, 3, 0, 0, None, None, fake, 1 << 30,
+                                 None) == _lib.MBX_EINVAL
+    assert L.mbx_vcd_workspace_bytes(geom(), 4, 3) == -1
+    assert L.mbx_vcd_subset_update(geom(), fake, fake, fake, 3, None, fake, fake, 4, 1.0, q, None, 1.0, 0, 3,
+                                   None, fake, 1 << 30, None) == _lib.MBX_EINVAL
+    assert L.mbx_sino_transpose(fake, 3, 0, ctypes.c_void_p(32), 1, 0, 2, 3, 4, None) == _lib.MBX_EUNSUPPORTED
 
 
 @pytest.mark.parametrize("views,slices", [(64, 512), (32, 512), (16, 1024)])
