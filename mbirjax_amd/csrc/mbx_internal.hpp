@@ -40,7 +40,16 @@ inline bool dtype_ok(int dt) { return dt == MBX_DTYPE_F32 || dt == MBX_DTYPE_BF1
 // storage the forward projector writes (and the delta sinogram of the VCD update)
 inline bool dtype_fwd_ok(int dt) { return dt == MBX_DTYPE_F32 || dt == MBX_DTYPE_BF16; }
 
-// Per-(view, pixel) geometry table shared by both projectors.
+// Per-(view, pixel) geometry table shared by both projectors: one record of
+// kRec floats per (view slot, pixel) (layout in mbx_project.hip).
+constexpr int kRec = 16;
+constexpr int kFill = 4;  // 64-element groups loaded per round (forward staging)
+constexpr int kBackFill = 3;  // 64-row chunks loaded per round (back projection)
+constexpr int64_t kTableBudget = 512ll << 20;
+// Rows [first, last] packed in record word 15.
+__device__ __forceinline__ int rec_mlo(float w) { return __float_as_int(w) & 0xffff; }
+__device__ __forceinline__ int rec_mhi(float w) { return __float_as_int(w) >> 16; }
+
 int table_chunk(const DevGeom &g, int npix, int nviews);  // pixels per table
 int64_t table_bytes(int chunk, int nviews);                // records + n_c array
 // Slot s of the table is pixel pix[s * pstride].

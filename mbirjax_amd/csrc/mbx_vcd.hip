@@ -229,11 +229,11 @@ __device__ __forceinline__ void axpy_body(TE *__restrict__ e, const TD *__restri
         load4_nt(e + 4 * t, ev);
         load4_nt(d + 4 * t, dv);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) ev[u] = ev[u] - a * dv[u];
+        for (int u = 0; u < 4; ++u) ev[u] = fmaf(-a, dv[u], ev[u]);
         store4_nt(e + 4 * t, ev);
     }
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = 4 * n4 + t; i < n; i += stride) e[i] = from_float<TE>(to_float(e[i]) - a * to_float(d[i]));
+    for (int64_t i = 4 * n4 + t; i < n; i += stride) e[i] = from_float<TE>(fmaf(-a, to_float(d[i]), to_float(e[i])));
 }
 
 // one thread per four elements
