@@ -608,6 +608,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             for (int t = 0; t <= 2 * RAD; ++t) acc[r][t] = fmaf(Aall[2 - RAD + t], v[r], acc[r][t]);
     };
     double ss = 0.0;  // this lane's share of sum w * (A delta)^2
+    const bool hasw = w != nullptr;  // kernel arguments: wave-uniform tests
+    const bool want_ss = part_sumsq != nullptr;
     // store element (row m, channel n) of the output (optionally adding to what is there)
     auto put = [&](int m, int n, float v) {
         const size_t o = LAYOUT == MBX_LAYOUT_VCR ? ((size_t)vi * g.C + n) * (size_t)g.R + m
@@ -615,20 +617,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         if (accumulate) v += to_float(sino[o]);
         const OT st = from_float<OT>(v);
         sino[o] = st;
-        if (part_sumsq) {
+        if (want_ss) {
             const float vs = to_float(st);
-            ss += (double)vs * (double)vs * (w ? (double)w[o] : 1.0);
+            ss += (double)vs * (double)vs * (hasw ? (double)w[o] : 1.0);
         }
     };
     // move the window to the next bucket: channel n0 - 2*RAD + dc is complete
     // (buckets up to its centre + RAD are added) and leaves it
     int dc = 0;
     int nextb = SW ? __builtin_amdgcn_readlane(bsl, 1) : 0;  // start of bucket dc + 1
-    // (channel-major output) this lane's first row of channel n0 - 2*RAD + dc;
-    // rows 64 apart are immediate offsets from it
+    // (channel-major output) channel n0 - 2*RAD + dc, rows wm0.. of this wave:
+    // a wave-uniform base, lanes and rows 64 apart as offsets from it
     const bool all_rows = wm0 + kWave * RPL <= m1;  // wave-uniform
-    OT *pch = sino + ((size_t)vi * g.C + (n0 - 2 * RAD)) * (size_t)g.R + wm0 + lane;
-    const float *wch = w ? w + ((size_t)vi * g.C + (n0 - 2 * RAD)) * (size_t)g.R + wm0 + lane : nullptr;
+    const size_t ch0 = ((size_t)vi * g.C + (n0 - 2 * RAD)) * (size_t)g.R + wm0;
+    OT *pch = sino + ch0;
+    const float *wch = w + (hasw ? ch0 : 0);
     auto advance = [&]() {
         const int n = n0 - 2 * RAD + dc;
         if (n >= n0 && n < n1) {  // wave-uniform
@@ -638,19 +641,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                 for (int r = 0; r < RPL; ++r) v[r] = acc[r][0];
                 if (accumulate) {
 #pragma unroll
-                    for (int r = 0; r < RPL; ++r) v[r] += to_float(pch[kWave * r]);
+                    for (int r = 0; r < RPL; ++r) v[r] += to_float(pch[lane + kWave * r]);
                 }
                 OT st[RPL];
 #pragma unroll
                 for (int r = 0; r < RPL; ++r) {
                     st[r] = from_float<OT>(v[r]);
-                    pch[kWave * r] = st[r];
+                    pch[lane + kWave * r] = st[r];
                 }
-                if (part_sumsq) {
+                if (want_ss) {
+                    if (hasw) {
 #pragma unroll
-                    for (int r = 0; r < RPL; ++r) {
-                        const double vs = (double)to_float(st[r]);
-                        ss += vs * vs * (wch ? (double)wch[kWave * r] : 1.0);
+                        for (int r = 0; r < RPL; ++r) {
+                            const double vs = (double)to_float(st[r]);
+                            ss += vs * vs * (double)wch[lane + kWave * r];
+                        }
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < RPL; ++r) {
+                            const double vs = (double)to_float(st[r]);
+                            ss += vs * vs;
+                        }
                     }
                 }
             } else {
@@ -662,7 +673,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             }
         }
         pch += g.R;
-        if (wch) wch += g.R;
+        if (hasw) wch += g.R;
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
 #pragma unroll
