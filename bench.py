@@ -134,7 +134,7 @@ def run_precision(model, y, part_np, precision, steps, warmup):
                t_back_ms=t_back * 1e3, t_fwd_ms=t_fwd * 1e3, t_fwd_kernel_ms=t_fwd_kernel * 1e3,
                fwd_kernel_GBps=b_fwd / t_fwd_kernel / 1e9, back_GBps=b_back / t_back / 1e9,
                fwd_GBps=b_fwd / t_fwd / 1e9, proj_GBps=(b_back + b_fwd) / (t_back + t_fwd) / 1e9,
-               b_back=b_back, b_fwd=b_fwd, fm_rmse=state.forward_model_rmse())
+               b_back=b_back, b_fwd=b_fwd, b_e=b_e, b_d=b_d, K=int(K), fm_rmse=state.forward_model_rmse())
     del state
     torch.cuda.empty_cache()
     return res
@@ -199,10 +199,54 @@ def cpu_baseline(model, part_np, view_sample=72):
     op.forward(geom, vox, sub, views=views, real=np.float32)
     dt = time.time() - t0
     t_iter = dt * (V / view_sample) * part_np.shape[0]
-    return dict(value=1.0 / t_iter, unit="iter/s", cores=cores, kind="port",
+    return dict(value=1.0 / t_iter, unit="iter/s", cores=cores, kind="port", cpu_model=cpu_model(),
                 sample="float32 oracle port, back+forward of one %d-pixel subset over %d of %d views (%.2f s), "
                        "scaled x%d views x%d subsets" % (sub.size, view_sample, V, dt, V // view_sample,
                                                          part_np.shape[0]))
+
+
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo), or the platform string."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+def run_c3(reps=3):
+    """Full-volume forward and back projection at 256^3: cone beam, 256 views over
+    [0, 2 pi), 256 rows x 384 channels, source-detector 1536, source-iso 768
+    (magnification 2), x and y ~ N(0, 1); HIP events on the launch stream."""
+    import torch
+    import mbirjax_amd as mj
+    V, R, C = 256, 256, 384
+    m = mj.ConeBeamModel((V, R, C), np.linspace(0, 2 * np.pi, V, endpoint=False), source_detector_dist=1536.0,
+                         source_iso_dist=768.0)
+    m.set_params(recon_shape=(256, 256, 256), verbose=0)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    x = torch.randn((256, 256, 256), generator=g, device="cuda")
+    y = torch.randn((V, R, C), generator=g, device="cuda")
+    m.forward_project(x)
+    m.back_project(y)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    t_f = timed(lambda: m.forward_project(x), reps, stream)
+    t_b = timed(lambda: m.back_project(y), reps, stream)
+    from mbirjax_amd import vcd_utils
+    npix = int(np.count_nonzero(vcd_utils.get_2d_ror_mask((256, 256, 256))))
+    b_f = 4 * npix * 256 + 4 * V * R * C  # cylinders in, sinogram out
+    b_b = 4 * V * R * C + 4 * npix * 256  # sinogram in, cylinders out
+    return dict(workload="cone-beam 256 views x 256 rows x 384 channels, recon 256^3 (%d pixels in the region "
+                         "of reconstruction), full forward_project and back_project" % npix,
+                fwd_ms=t_f * 1e3, back_ms=t_b * 1e3, fwd_back_ms=(t_f + t_b) * 1e3,
+                fwd_GBps=b_f / t_f / 1e9, back_GBps=b_b / t_b / 1e9,
+                fwd_back_per_s=1.0 / (t_f + t_b))
 
 
 def copy_calibration(nbytes=1 << 30):
@@ -235,6 +279,18 @@ def profiled_traffic(kernel, workload, precision):
     return profiled_value("traffic_bytes_per_launch", kernel, workload, precision)
 
 
+def subset_update_block(r, model):
+    """Per subset update of the timed iteration: milliseconds, algorithmic HBM
+    bytes of the chain (back and forward projections as in algorithmic_bytes,
+    plus the error-sinogram update e -= alpha * (A delta): e read and written,
+    the delta sinogram read) and their rate against the HBM peak."""
+    V, R, C = model.get_params('sinogram_shape')
+    K = r["K"]
+    b = r["b_back"] + r["b_fwd"] + (2 * r["b_e"] + r["b_d"]) * V * R * C
+    ms = r["t_iter"] / K * 1e3
+    return dict(ms=ms, bytes=b, GBps=b / (ms / 1e3) / 1e9, frac=b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -244,6 +300,7 @@ def main():
     ap.add_argument("--precision", default="all", choices=["fp32", "mixed", "fp16", "bf16", "all"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 256^3 forward + back projection leg")
     ap.add_argument("--no-full-run", action="store_true",
                     help="skip the 20-iteration default-schedule run (K = 1, 4, 16, 64, 128, ...)")
     args = ap.parse_args()
@@ -303,12 +360,18 @@ def main():
                      "valu_issue_per_cu_cycle": profiled_value("valu_issue_per_cu_cycle", kname, args.workload,
                                                                main_p)[0]},
         "projector_GBps": r["proj_GBps"],
-        "precisions": {k: {kk: vv for kk, vv in v.items() if kk not in ("b_back", "b_fwd")}
+        # the whole subset-update chain (table, back projection, qGGMRF and step,
+        # forward projection, step size, recon and error-sinogram updates)
+        "subset_update": subset_update_block(r, model),
+        "precisions": {k: {kk: vv for kk, vv in v.items() if kk not in ("b_back", "b_fwd", "b_e", "b_d")}
                        for k, v in results.items()},
     }
     if not args.no_full_run and "fp32" in results:
         line["full_run"] = run_full(model, y)
         log("full_run", json.dumps(line["full_run"]))
+    if not args.no_c3:
+        line["c3_fwd_back"] = run_c3()
+        log("c3", json.dumps(line["c3_fwd_back"]))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(model, part)
     if args.copy_calibration:

@@ -5,8 +5,9 @@ ValueError (unless no_warning), numpy scalars are normalised to Python
 scalars, setting sigma_y/sigma_x/sigma_prox disables auto-regularisation and
 setting sharpness/snr_db re-enables it, and the return value says whether a
 geometry parameter changed (the model then re-plans its projectors).
-Models are saved as YAML ({name: {val, recompile_flag}}) with PyYAML's safe
-loader.
+Models are saved as YAML in the reference's format ({name: {val, recompile_flag}},
+arrays as ':ARRAY:' strings of %.7f values with a 'shape' entry) and read with
+PyYAML's safe loader.
 """
 import io
 import logging
@@ -140,68 +141,96 @@ class ParameterHandler:
             raise ValueError("sinogram_shape must be (views, rows, channels). \n"
                              "Got {} for sinogram shape.".format(sinogram_shape))
         geometry_type = self.get_params('geometry_type')
-        if geometry_type != str(type(self)):
+        if self._class_name(geometry_type) != self._class_name(str(type(self))):
             raise ValueError('Parameters are associated with {}, but the current model is {}'.format(
                 geometry_type, str(type(self))))
 
-    # ------------------------------------------------------------------ YAML
     @staticmethod
-    def _to_yaml_value(val):
-        if hasattr(val, 'detach'):
-            val = val.detach().cpu().numpy()
-        if isinstance(val, np.ndarray):
-            return ParameterHandler.array_prefix + yaml.safe_dump({'shape': list(val.shape), 'dtype': str(val.dtype),
-                                                                   'data': val.ravel().tolist()})
-        if isinstance(val, tuple):
-            return {'__tuple__': [ParameterHandler._to_yaml_value(v) for v in val]}
-        if isinstance(val, list):
-            return [ParameterHandler._to_yaml_value(v) for v in val]
-        if isinstance(val, float) and not np.isfinite(val):
-            return {'__float__': repr(val)}
-        return ParameterHandler.normalize_scalar(val)
+    def _class_name(type_str):
+        """'ConeBeamModel' from "<class 'mbirjax.cone_beam.ConeBeamModel'>" (or from the
+        same string naming this package): files written by either load here."""
+        return str(type_str).strip("<>'").split('.')[-1].strip("'")
+
+    # ------------------------------------------------------------------ YAML
+    # The on-disk format is the reference's (mbirjax/parameter_handler.py:195-249):
+    # {name: {val, recompile_flag}} with every array written as the string
+    # ':ARRAY:' followed by its values in %.7f and a 'shape' entry beside 'val',
+    # numpy scalars as Python scalars and lists as tuples (YAML sequences).
 
     @staticmethod
-    def _from_yaml_value(val):
-        if isinstance(val, str) and val.startswith(ParameterHandler.array_prefix):
-            d = yaml.safe_load(val[len(ParameterHandler.array_prefix):])
-            return np.array(d['data'], dtype=d['dtype']).reshape(d['shape'])
-        if isinstance(val, dict) and '__tuple__' in val:
-            return tuple(ParameterHandler._from_yaml_value(v) for v in val['__tuple__'])
-        if isinstance(val, dict) and '__float__' in val:
-            return float(val['__float__'])
-        if isinstance(val, list):
-            return [ParameterHandler._from_yaml_value(v) for v in val]
+    def _plain(val):
+        """numpy scalars as Python scalars, lists and tuples as tuples (recursively)."""
+        if isinstance(val, np.generic):
+            return val.item()
+        if isinstance(val, (list, tuple)):
+            return tuple(ParameterHandler._plain(v) for v in val)
         return val
 
     @staticmethod
+    def serialize_parameter(param):
+        """A parameter as the YAML entry the reference writes for it."""
+        val = param.val
+        if hasattr(val, 'detach'):
+            val = val.detach().cpu().numpy()
+        if isinstance(val, np.ndarray):
+            return {'val': ParameterHandler.array_prefix + " ".join("%.7f" % x for x in val.ravel()),
+                    'shape': tuple(int(n) for n in val.shape), 'recompile_flag': bool(param.recompile_flag)}
+        return {'val': ParameterHandler._plain(val), 'recompile_flag': bool(param.recompile_flag)}
+
+    @staticmethod
+    def deserialize_parameter(entry):
+        """A YAML entry (as the reference writes it) as a Param: ':ARRAY:' strings
+        become float64 arrays of the entry's shape, sequences become tuples."""
+        val = entry['val']
+        flag = entry.get('recompile_flag', True)
+        if isinstance(val, str) and val.startswith(ParameterHandler.array_prefix):
+            body = val[len(ParameterHandler.array_prefix):].replace('[', ' ').replace(']', ' ')
+            arr = np.array(body.split(), dtype=np.float64)
+            if 'shape' in entry:
+                arr = arr.reshape(tuple(entry['shape']))
+            return Param(arr, flag)
+        return Param(ParameterHandler._plain(val), flag)
+
+    @staticmethod
     def save_params(params, filename=None):
-        out = {k: {'val': ParameterHandler._to_yaml_value(p.val), 'recompile_flag': bool(p.recompile_flag)}
-               for k, p in params.items()}
-        text = yaml.safe_dump(out, sort_keys=False)
+        """YAML text of the parameters (returned), or written to filename (.yml or .yaml)."""
+        out = {k: ParameterHandler.serialize_parameter(p) for k, p in params.items()}
+        text = yaml.safe_dump(out, sort_keys=False, default_flow_style=False)
         if filename is None:
             return text
-        if not (filename.endswith('.yml') or filename.endswith('.yaml')):
+        if not (filename.lower().endswith('.yml') or filename.lower().endswith('.yaml')):
             raise ValueError('Filename must end in .yaml or .yml: ' + filename)
+        d = os.path.dirname(filename)
+        if d:
+            os.makedirs(d, exist_ok=True)
         with open(filename, 'w') as f:
             f.write(text)
         return None
 
     @staticmethod
     def load_param_dict(source, required_param_names=None, values_only=True):
+        """Parameters from a YAML file name (.yml or .yaml), YAML text or a file-like
+        object; with required_param_names, (required, others) as the reference's
+        load_param_dict returns them."""
         if hasattr(source, 'read'):
             text = source.read()
-        elif isinstance(source, str) and (source.endswith('.yml') or source.endswith('.yaml')) and \
-                os.path.exists(source):
+        elif isinstance(source, str) and os.path.exists(source):
+            if not source.lower().endswith(('.yml', '.yaml')):
+                raise ValueError('Filename must end in .yml or .yaml')
             with open(source) as f:
                 text = f.read()
-        else:
+        elif isinstance(source, str):
             text = source
+        else:
+            raise TypeError('Invalid source type for load_param_dict. Must be str or file-like object.')
         raw = yaml.safe_load(text)
-        params = {k: Param(ParameterHandler._from_yaml_value(v['val']), v['recompile_flag']) for k, v in raw.items()}
+        params = {k: ParameterHandler.deserialize_parameter(v) for k, v in raw.items()}
         if required_param_names is None:
             return params
         required = {}
         for name in required_param_names:
+            if name not in params:
+                raise ValueError("The parameters do not hold '{}', which this model requires".format(name))
             required[name] = params.pop(name)
         if values_only:
             required = {k: v.val for k, v in required.items()}

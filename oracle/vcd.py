@@ -212,7 +212,10 @@ class OracleModel:
             delta = np.maximum(-pos * flat[S], delta)
             ds = projectors.forward(self.geom, delta, S)
         flat = flat.copy()
-        flat[S] += alpha * delta
+        # every listed copy of a pixel adds its update, as .at[].add does
+        # (mbirjax/tomography_model.py:1820-1823): a padded subset can list a
+        # pixel twice (mbirjax/vcd_utils.py:212-217)
+        np.add.at(flat, S, alpha * delta)
         e = e + alpha * ds if self.planted == "error_update_sign" else e - alpha * ds
         if self.check_invariant:
             idx = np.flatnonzero(partitions.get_2d_ror_mask(self.geom.recon_shape))

@@ -183,6 +183,42 @@ def test_unrolled_gradient_fp64_finite_differences(gpu):
         assert abs(grad[i] - fd) <= 2e-3 * abs(fd) + 1e-9
 
 
+def test_c5_sigma_x_gradient_vs_fp64_fixture(gpu):
+    """d loss / d log(sigma_x[i]) for four per-iteration sigma_x values through
+    four unrolled full-volume VCD updates (cone beam, 128 views, 128 x 192
+    detector, recon 128^3), against FP64 central finite differences computed
+    with the oracle alone and committed as a fixture by
+    tests/golden/make_c5_golden.py; checkpointing the error sinogram every 2
+    updates gives the same gradient bit for bit."""
+    import importlib.util
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "make_c5_golden.py")
+    spec = importlib.util.spec_from_file_location("make_c5_golden", path)
+    gold = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gold)
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "c5_sigma_x_fd.npz"))
+    g, truth, y, S = gold.c5_inputs()
+    assert abs(np.sum(y) - fx["y_sum"]) <= 1e-9 * abs(fx["y_sum"])
+    m, _ = cone(gold.V, gold.R, gold.C, gold.SDD, gold.SID, recon_shape=gold.SHAPE)
+    assert abs(m.get_params('delta_voxel') - float(fx["delta_voxel"])) <= 1e-12
+    m.set_params(sigma_y=gold.SIGMA_Y, sigma_x=gold.SIGMA_X, no_warning=True)
+    tt = torch.as_tensor(truth.astype(np.float32), device='cuda')
+    yt = torch.as_tensor(y.astype(np.float32), device='cuda')
+    grads = []
+    for ck in (None, 2):
+        net = ag.UnrolledVCD(m, gold.UPDATES, checkpoint_every=ck)
+        x = net(yt, torch.zeros(gold.SHAPE, device='cuda'))
+        loss = 0.5 * torch.sum((x.double() - tt.double()) ** 2)
+        loss.backward()
+        grads.append(net.log_sigma_x.grad.detach().cpu().numpy().astype(np.float64))
+    assert np.array_equal(grads[0], grads[1])
+    print("loss %.9e (FP64 oracle %.9e)" % (loss.item(), float(fx["loss"])), flush=True)
+    assert abs(loss.item() - float(fx["loss"])) <= 1e-6 * float(fx["loss"])
+    for i in range(gold.UPDATES):
+        print("log sigma_x[%d]: autograd %.6e  FP64 FD %.6e" % (i, grads[0][i], fx["fd_grad"][i]), flush=True)
+    assert np.all(np.abs(grads[0] - fx["fd_grad"]) <= 1e-4 * np.abs(fx["fd_grad"]) + 1e-6 * np.max(np.abs(fx["fd_grad"])))
+
+
 def test_checkpointing_same_gradient_less_memory(gpu):
     m, _ = cone(64, 48, 64, 256.0, 128.0, recon_shape=(64, 64, 48))
     shape = m.get_params('recon_shape')

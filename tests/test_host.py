@@ -205,6 +205,143 @@ def test_yaml_round_trip(tmp_path):
     assert mj.ParameterHandler.compare_parameter_handlers(p, p2)
 
 
+# A parallel-beam model file as mbirjax's to_file writes it
+# (mbirjax/parameter_handler.py:195-249, 324-415): every parameter as
+# {val, recompile_flag}, arrays as ':ARRAY:' plus %.7f values with a 'shape'
+# entry, tuples as YAML sequences.  Written by hand from that format.
+REFERENCE_FORMAT_YAML = """\
+geometry_type:
+  val: <class 'mbirjax.parallel_beam.ParallelBeamModel'>
+  recompile_flag: false
+file_format:
+  val: 1.0
+  recompile_flag: false
+sinogram_shape:
+  val:
+  - 4
+  - 3
+  - 5
+  recompile_flag: true
+delta_det_channel:
+  val: 1.0
+  recompile_flag: true
+delta_det_row:
+  val: 1.0
+  recompile_flag: true
+det_row_offset:
+  val: 0.25
+  recompile_flag: true
+det_channel_offset:
+  val: 0.0
+  recompile_flag: true
+sigma_y:
+  val: 1.0
+  recompile_flag: false
+recon_shape:
+  val:
+  - 5
+  - 5
+  - 3
+  recompile_flag: true
+delta_voxel:
+  val: 1.0
+  recompile_flag: true
+sigma_x:
+  val: 1.0
+  recompile_flag: false
+sigma_prox:
+  val: 1.0
+  recompile_flag: false
+p:
+  val: 2.0
+  recompile_flag: false
+q:
+  val: 1.2
+  recompile_flag: false
+T:
+  val: 1.0
+  recompile_flag: false
+qggmrf_nbr_wts:
+  val:
+  - 1.0
+  - 0.5
+  - 1.0
+  recompile_flag: false
+auto_regularize_flag:
+  val: true
+  recompile_flag: false
+positivity_flag:
+  val: false
+  recompile_flag: false
+snr_db:
+  val: 30.0
+  recompile_flag: false
+sharpness:
+  val: 0.5
+  recompile_flag: false
+granularity:
+  val:
+  - 1
+  - 2
+  - 4
+  - 8
+  recompile_flag: false
+partition_sequence:
+  val:
+  - 0
+  - 1
+  - 3
+  recompile_flag: false
+verbose:
+  val: 0
+  recompile_flag: false
+use_gpu:
+  val: automatic
+  recompile_flag: true
+angles:
+  val: ':ARRAY:0.0000000 0.7853982 1.5707963 2.3561945'
+  shape:
+  - 4
+  recompile_flag: true
+view_params_name:
+  val: angles
+  recompile_flag: true
+"""
+
+
+def test_yaml_reference_format():
+    """A model file in mbirjax's format loads with from_file, and this
+    package's to_file writes the same format back (arrays as ':ARRAY:' %.7f
+    strings with their shape, lists as tuples)."""
+    import yaml
+    p = mj.ParallelBeamModel.from_file(REFERENCE_FORMAT_YAML)
+    assert p.get_params('sinogram_shape') == (4, 3, 5)
+    assert p.get_params('recon_shape') == (5, 5, 3)
+    assert p.get_params('det_row_offset') == 0.25 and p.get_params('sharpness') == 0.5
+    assert p.get_params('qggmrf_nbr_wts') == (1.0, 0.5, 1.0)
+    assert p.get_params('granularity') == (1, 2, 4, 8) and p.get_params('partition_sequence') == (0, 1, 3)
+    assert np.allclose(np.asarray(p.get_params('angles')), np.arange(4) * np.pi / 4, atol=1e-7)
+    text = p.to_file(None)
+    raw = yaml.safe_load(text)
+    assert raw['angles']['val'] == ':ARRAY:0.0000000 0.7853982 1.5707963 2.3561945'
+    assert raw['angles']['shape'] == [4] and raw['angles']['recompile_flag'] is True
+    assert raw['qggmrf_nbr_wts'] == {'val': [1.0, 0.5, 1.0], 'recompile_flag': False}
+    assert raw['geometry_type']['val'] == "<class 'mbirjax.parallel_beam.ParallelBeamModel'>"
+    ref = yaml.safe_load(REFERENCE_FORMAT_YAML)
+    assert set(raw) == set(ref)
+    for k in ref:
+        assert raw[k] == ref[k], k
+    p2 = mj.ParallelBeamModel.from_file(text)
+    assert mj.ParameterHandler.compare_parameter_handlers(p, p2)
+    # a model of one geometry does not load into another
+    with pytest.raises(ValueError):
+        mj.ConeBeamModel.from_file(REFERENCE_FORMAT_YAML.replace('ParallelBeamModel', 'ConeBeamModel'))
+    # 2-D arrays keep their shape
+    q = mj.ParameterHandler.deserialize_parameter(mj.ParameterHandler.serialize_parameter(
+        mj._utils.Param(np.arange(6.0).reshape(2, 3) / 7, True)))
+    assert q.val.shape == (2, 3) and np.allclose(q.val, np.arange(6.0).reshape(2, 3) / 7, atol=5e-8)
+
+
 @pytest.mark.parametrize("shape,sdd,sid", [((128, 128, 192), 768.0, 384.0), ((64, 40, 128), 512.0, 512.0),
                                            ((720, 512, 768), 3072.0, 1536.0), ((13, 70, 45), 200.0, 90.0)])
 def test_geometry_matches_oracle(shape, sdd, sid):

@@ -203,3 +203,26 @@ def test_planted_vcd_mistake_is_caught(planted):
             gaps["fixed_point"] = fixed_point_error(prob, positivity, planted, iterations=30)
         assert not gaps["fixed_point"] < 1e-3, (ovcd.PLANTED[planted], gaps)
     assert not gaps["replay"] < 1e-6, (ovcd.PLANTED[planted], gaps)
+
+
+def test_duplicate_pixels_add_every_copy():
+    """A padded subset can list a pixel twice (mbirjax/vcd_utils.py:212-217);
+    the update then adds every copy's step, as .at[].add does
+    (mbirjax/tomography_model.py:1820-1823).  The error sinogram is updated
+    with the projection of every listed copy, so the invariant e = y - A x
+    holds only if the recon takes every copy too."""
+    prob = Problem()
+    om = prob.oracle()
+    nz = prob.shape[2]
+    y = prob.y.reshape(prob.geom.sinogram_shape)
+    om.y = y
+    om.check_invariant = True
+    flat = np.zeros((prob.shape[0] * prob.shape[1], nz))
+    fm_hess = om.hessian_diagonal().reshape(-1, nz)
+    ror = prob.ror
+    S = np.array([ror[3], ror[10], ror[3], ror[17], ror[10], ror[3]])
+    flat, e, ell1, alpha = om.subset_update(flat, y.copy(), S, 1.0, True, fm_hess, 1.0 / SIGMA_Y ** 2,
+                                            om.qparams(), None, 1.0)
+    resid = y.reshape(-1) - prob.A @ flat.reshape(-1)
+    assert np.max(np.abs(resid - e.reshape(-1))) <= 1e-9 * np.max(np.abs(y))
+    assert np.any(flat[ror[3]] != 0)
