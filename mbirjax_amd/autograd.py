@@ -108,7 +108,7 @@ def forward_project(model, recon, pixel_indices=None, layout='vrc'):
         pix = _ror_indices(model, recon.device)
         vox = recon.reshape(rows * cols, nz)[pix.long()]
     else:
-        pix = model._indices(pixel_indices)
+        pix = model._indices(pixel_indices, device_check=False)
         vox = recon
     return ForwardProjector.apply(vox, model, pix, layout)
 
@@ -124,7 +124,7 @@ def back_project(model, sinogram, pixel_indices=None, layout='vrc'):
         full = torch.zeros((rows * cols, nz), dtype=cyl.dtype, device=cyl.device)
         full = full.index_copy(0, pix.long(), cyl)
         return full.reshape(rows, cols, nz)
-    return BackProjector.apply(sinogram, model, model._indices(pixel_indices), layout)
+    return BackProjector.apply(sinogram, model, model._indices(pixel_indices, device_check=False), layout)
 
 
 # ---------------------------------------------------------------------- qGGMRF in torch

@@ -142,7 +142,13 @@ __global__ __launch_bounds__(256) void pv_table_kernel(DevGeom g, const float *_
     if (s >= npix) return;
     const mbx_geom &h = g.h;
     const int va = views ? views[vi] : vi;
-    const int p = pix[(size_t)s * pstride];
+    // A pixel index outside the recon contributes nothing: its geometry is
+    // that of pixel 0, its channel weights are zero and its centre channel is
+    // far off the detector (the forward binning drops it), so no index the
+    // caller passes reaches memory through the table.
+    const int pin = pix[(size_t)s * pstride];
+    const bool valid = pin >= 0 && pin < g.rows * g.cols;
+    const int p = valid ? pin : 0;
     const int i = p / g.cols, j = p - (p / g.cols) * g.cols;
     const double dv = h.delta_voxel, dc = h.delta_det_channel;
     // Wc: voxel width on the detector in channels; sc: channel weight per unit
@@ -188,7 +194,7 @@ __global__ __launch_bounds__(256) void pv_table_kernel(DevGeom g, const float *_
         sc = dv / cosa;
     }
     const double hc = 0.5 * (Wc + 1.0), Lc = fmin(1.0, Wc);
-    const int nc = (int)rint(np);
+    const int nc = valid ? (int)rint(np) : -(1 << 28);
     float r[kRec];
     r[0] = __int_as_float(nc);
 #pragma unroll

@@ -143,17 +143,30 @@ class TomographyModel(ParameterHandler):
             return a.detach().to(device=dev, dtype=dtype).contiguous()
         return torch.as_tensor(np.asarray(a), device=dev).to(dtype).contiguous()
 
-    def _indices(self, idx, bound=None, what='pixel_indices'):
-        """int32 device copy of an index array; with bound, every entry must lie in [0, bound)."""
+    def _indices(self, idx, bound=None, what='pixel_indices', device_check=True):
+        """int32 device copy of an index array; with bound, every entry must lie in [0, bound).
+
+        Host arrays (NumPy, lists, CPU tensors) are checked on the host before
+        the upload.  A tensor already on the GPU is checked there only with
+        device_check, which waits for the device (two reductions read back);
+        the projectors pass device_check=False for pixel indices, which the
+        kernels use for geometry alone: an index outside the recon contributes
+        nothing (zero channel weights) and never addresses memory."""
         dev = self._device()
-        if isinstance(idx, torch.Tensor):
-            t = idx.detach().to(device=dev, dtype=torch.int32).reshape(-1).contiguous()
-        else:
-            t = torch.as_tensor(_lib.as_i32(idx), device=dev)
         if bound is None:
             rows, cols = self.get_params('recon_shape')[:2]
             bound = rows * cols
-        if t.numel() > 0:
+        on_device = isinstance(idx, torch.Tensor) and idx.device.type != 'cpu'
+        if on_device:
+            t = idx.detach().to(device=dev, dtype=torch.int32).reshape(-1).contiguous()
+        else:
+            h = np.asarray(idx.detach().numpy() if isinstance(idx, torch.Tensor) else idx).reshape(-1)
+            if h.size > 0:
+                lo, hi = int(h.min()), int(h.max())
+                if lo < 0 or hi >= bound:
+                    raise ValueError('{} must lie in [0, {}); got range [{}, {}]'.format(what, bound, lo, hi))
+            return torch.as_tensor(_lib.as_i32(h), device=dev)
+        if device_check and t.numel() > 0:
             lo, hi = int(t.min().item()), int(t.max().item())
             if lo < 0 or hi >= bound:
                 raise ValueError('{} must lie in [0, {}); got range [{}, {}]'.format(what, bound, lo, hi))
@@ -267,7 +280,7 @@ class TomographyModel(ParameterHandler):
         """(V', R, C) projection of the cylinders voxel_values (npix, Nz) at pixel_indices."""
         nz = self.get_params('recon_shape')[2]
         vox = self._tensor(voxel_values)
-        pix = self._indices(pixel_indices)
+        pix = self._indices(pixel_indices, device_check=False)
         if vox.dim() != 2 or vox.shape[0] != pix.numel() or vox.shape[1] != nz:
             raise ValueError('voxel_values must have shape[0:2] = (num_indices, num_slices)')
         V = self.get_params('sinogram_shape')[0]
@@ -277,7 +290,7 @@ class TomographyModel(ParameterHandler):
     def sparse_back_project(self, sinogram, pixel_indices, view_indices=None, coeff_power=1, output_device=None):
         """(npix, Nz) back projection; view_indices select rows of the passed sinogram
         (and their angles), as in mbirjax/tomography_model.py:661-663."""
-        pix = self._indices(pixel_indices)
+        pix = self._indices(pixel_indices, device_check=False)
         sino = self.to_channel_major(sinogram)
         V = self.get_params('sinogram_shape')[0]
         views = None if view_indices is None else self._indices(view_indices, min(V, sino.shape[0]),

@@ -379,3 +379,30 @@ def test_ramp_row_convolution_routes(gpu, C):
                                              torch.as_tensor(h.astype(np.float32), device='cuda'))
     ref = np.stack([[np.convolve(r.astype(np.float64), h, mode='valid') for r in v] for v in rows])
     assert rel(got, ref) <= TOL
+
+
+def test_out_of_range_device_pixel_indices_contribute_nothing(gpu):
+    """Pixel indices on the device are not read back for a range check (no
+    host sync): an index outside the recon contributes nothing to the forward
+    projection and back-projects to zeros, and the valid pixels' results are
+    unchanged.  Host arrays are still checked before the upload."""
+    m, g = CASES["cone_m2"]()
+    rows, cols, nz = m.get_params('recon_shape')
+    rng = np.random.default_rng(21)
+    valid = rng.choice(rows * cols, 37, replace=False).astype(np.int32)
+    bad = np.array([-5, rows * cols, rows * cols + 1000, -(1 << 30)], dtype=np.int32)
+    pix = np.concatenate([valid[:20], bad[:2], valid[20:], bad[2:]])
+    x = rng.standard_normal((pix.size, nz)).astype(np.float32)
+    keep = np.isin(pix, valid)
+    pd = torch.as_tensor(pix, device='cuda')
+    a_all = m.sparse_forward_project(torch.as_tensor(x, device='cuda'), pd)
+    a_ok = m.sparse_forward_project(torch.as_tensor(x[keep], device='cuda'), torch.as_tensor(pix[keep], device='cuda'))
+    assert torch.allclose(a_all, a_ok, rtol=1e-6, atol=1e-6 * float(a_ok.abs().max()))
+    y = torch.as_tensor(rng.standard_normal(g.sinogram_shape).astype(np.float32), device='cuda')
+    b_all = m.sparse_back_project(y, pd)
+    b_ok = m.sparse_back_project(y, torch.as_tensor(pix[keep], device='cuda'))
+    assert torch.allclose(b_all[torch.as_tensor(keep, device='cuda')], b_ok, rtol=1e-6,
+                          atol=1e-6 * float(b_ok.abs().max()))
+    assert torch.count_nonzero(b_all[torch.as_tensor(~keep, device='cuda')]) == 0
+    with pytest.raises(ValueError):
+        m.sparse_forward_project(x, pix)
