@@ -46,6 +46,9 @@
  *                           (mbirjax/tomography_model.py:1139-1152, 1633-1658)
  *   mbx_forward_plan / mbx_forward_apply: no counterpart (mbx_forward_project
  *                           split into its setup and its projection kernel)
+ *   mbx_vcd_subset_update_planned: vcd_subset_updater with the subset's
+ *                           projector setup (geometry table, bins, records) built
+ *                           once by mbx_forward_plan and reused
  *
  * Python binds these through ctypes (mbirjax_amd/_lib.py).
  */
@@ -160,7 +163,8 @@ int mbx_forward_project(const mbx_geom *g, const float *angles, const float *vox
  * plan was built with, else MBX_EINVAL (so an apply of more pixels or views
  * than planned cannot read past the plan).  The plan stays valid while the
  * geometry, angles, pix and views are unchanged; applies on one stream may
- * share it. */
+ * share it (an apply that asks for sumsq writes the per-block FP64 partials
+ * into the plan's scratch slots, so applies must not overlap in time). */
 int mbx_forward_plan(const mbx_geom *g, const float *angles, const int32_t *pix, int32_t npix, const int32_t *views,
                      int32_t nviews, void *plan, int64_t plan_bytes, void *stream);
 int mbx_forward_apply(const mbx_geom *g, const void *plan, int64_t plan_bytes, const float *vox, int32_t npix,
@@ -247,6 +251,24 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
                           double fm_constant, const mbx_qggmrf *qparams, const float *prox_input,
                           double sigma_prox, int32_t positivity, int32_t delta_dtype,
                           double *stats, void *workspace, int64_t workspace_bytes, void *stream);
+
+/* The same subset update with the subset's geometry table, centre-channel bins
+ * and forward records taken from plan: the output of
+ * mbx_forward_plan(g, angles, pix, npix, NULL, 0, plan, plan_bytes, stream)
+ * for the same pix (all views), which must fit one geometry-table chunk
+ * (else MBX_EUNSUPPORTED); plan_bytes must equal
+ * mbx_forward_workspace_bytes(g, npix, g->num_views) (else MBX_EINVAL).
+ * For callers that sweep a fixed partition many times (VCD iterations with
+ * one granularity): the setup is then paid once per subset instead of once
+ * per update.  Results are bitwise those of mbx_vcd_subset_update.  The call
+ * writes the forward projection's FP64 partials into the plan's scratch slots:
+ * calls sharing a plan must not overlap in time. */
+int mbx_vcd_subset_update_planned(const mbx_geom *g, const float *angles, float *flat_recon, void *e,
+                                  int32_t e_dtype, const float *weights, const float *fm_hessian,
+                                  const int32_t *pix, int32_t npix, double fm_constant, const mbx_qggmrf *qparams,
+                                  const float *prox_input, double sigma_prox, int32_t positivity,
+                                  int32_t delta_dtype, void *plan, int64_t plan_bytes, double *stats,
+                                  void *workspace, int64_t workspace_bytes, void *stream);
 
 /* One subset update of the qGGMRF proximal-map denoiser (vcd_subset_denoiser,
  * mbirjax/denoising.py:366-411): the identity forward model on the image,

@@ -35,7 +35,7 @@ EXPORTS = (
     "mbx_qggmrf_grad_hess",
     "mbx_dot_workspace_bytes", "mbx_dot", "mbx_sino_axpy", "mbx_vcd_workspace_bytes", "mbx_vcd_subset_update",
     "mbx_denoise_workspace_bytes", "mbx_denoise_subset_update", "mbx_median_filter3d",
-    "mbx_noalloc_scratch_bytes", "mbx_back_project_noalloc",
+    "mbx_noalloc_scratch_bytes", "mbx_back_project_noalloc", "mbx_vcd_subset_update_planned",
 )
 
 
@@ -101,6 +101,8 @@ def lib():
             "mbx_vcd_workspace_bytes": (i64, [G, i32, i32]),
             "mbx_vcd_subset_update": (i32, [G, vp, vp, vp, i32, vp, vp, vp, i32, f64, Q, vp, f64, i32, i32, vp, vp,
                                             i64, vp]),
+            "mbx_vcd_subset_update_planned": (i32, [G, vp, vp, vp, i32, vp, vp, vp, i32, f64, Q, vp, f64, i32, i32,
+                                                    vp, i64, vp, vp, i64, vp]),
             "mbx_denoise_workspace_bytes": (i64, [i32, i32]),
             "mbx_denoise_subset_update": (i32, [vp, vp, i32, i32, i32, vp, i32, f64, Q, vp, vp, i64, vp]),
             "mbx_median_filter3d": (i32, [vp, i32, i32, i32, i32, vp, vp]),

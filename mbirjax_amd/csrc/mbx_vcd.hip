@@ -497,11 +497,18 @@ int64_t mbx_vcd_workspace_bytes(const mbx_geom *g, int32_t npix, int32_t delta_d
     return (int64_t)vcd_layout(d, npix, delta_dtype).total;
 }
 
-int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_recon, void *e, int32_t e_dtype,
-                          const float *weights, const float *fm_hessian, const int32_t *pix, int32_t npix,
-                          double fm_constant, const mbx_qggmrf *qparams, const float *prox_input, double sigma_prox,
-                          int32_t positivity, int32_t delta_dtype, double *stats, void *workspace,
-                          int64_t workspace_bytes, void *stream) {
+}  // extern "C"
+
+namespace mbx {
+
+// The subset update; plan (nullable): the subset's forward plan over every
+// view (mbx_forward_plan), whose table, bins and records it uses instead of
+// building them.
+static int vcd_update(const mbx_geom *g, const float *angles, float *flat_recon, void *e, int32_t e_dtype,
+                      const float *weights, const float *fm_hessian, const int32_t *pix, int32_t npix,
+                      double fm_constant, const mbx_qggmrf *qparams, const float *prox_input, double sigma_prox,
+                      int32_t positivity, int32_t delta_dtype, void *plan, int64_t plan_bytes, double *stats,
+                      void *workspace, int64_t workspace_bytes, void *stream) {
     DevGeom d;
     int rc = make_dev_geom(g, &d);
     if (rc != MBX_OK) return rc;
@@ -532,16 +539,31 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
     const int nb = red_blocks(ncyl);
 
     float *T = (float *)(ws + L.table);
-    int *NC = table_nc(T, L.chunk, d.V);
     float *bpart = (float *)(ws + L.bpart);
     void *fpart = ws + L.fpart;
     const bool one_table = npix <= L.chunk;  // build the table once for all projections
+    if (plan) {
+        // the plan's layout is the one mbx_forward_plan writes for these npix
+        // pixels and all views: the table (records, then n_c) and the forward
+        // aux (FP64 partial slots, pixel list, bucket starts, records)
+        if (!one_table)
+            return fail(MBX_EUNSUPPORTED, "a planned subset update needs a subset of one geometry-table chunk (" +
+                                              std::to_string(L.chunk) + " pixels)");
+        const int64_t need = fwd_workspace_bytes(d, npix, d.V);
+        if (plan_bytes != need)
+            return fail(MBX_EINVAL, "plan_bytes " + std::to_string(plan_bytes) + " is not the size " +
+                                        std::to_string(need) + " of a forward plan of " + std::to_string(npix) +
+                                        " pixels over all views");
+        T = (float *)plan;
+        fpart = (char *)plan + table_bytes(L.chunk, d.V);
+    }
+    int *NC = table_nc(T, L.chunk, d.V);
 
     // (c) back projection of the weighted error sinogram (with one table, the
     // delta kernel sums its view-group partials, and the forward binning runs
     // on the side stream meanwhile when it is long enough to pay for the fork
     // and join: 2^18 (view, pixel) records or more)
-    const bool side_bin = one_table && (int64_t)npix * d.V >= (1 << 18);
+    const bool side_bin = !plan && one_table && (int64_t)npix * d.V >= (1 << 18);
     int bsplit = 0;
     SideStream *side = nullptr;
     std::unique_lock<std::mutex> side_lock;
@@ -560,8 +582,10 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
     } join{side, st};
     for (int c0 = 0; c0 < npix; c0 += L.chunk) {
         const int n = std::min(L.chunk, npix - c0);
-        rc = launch_table(d, angles, pix + c0, n, nullptr, d.V, T, NC, st);
-        if (rc != MBX_OK) return rc;
+        if (!plan) {
+            rc = launch_table(d, angles, pix + c0, n, nullptr, d.V, T, NC, st);
+            if (rc != MBX_OK) return rc;
+        }
         if (side_bin && (side = side_stream(st)) != nullptr) {
             side_lock = std::unique_lock<std::mutex>(side->mu);
             MBX_HIP_CHECK(hipEventRecord(side->fork, st));
@@ -583,7 +607,7 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
     // (a second projection of one table chunk reuses the bins and records of the
     // first; with partials, the last chunk leaves the FP64 partials of
     // sum w * (A delta)^2 at the start of fpart)
-    bool binned = false;
+    bool binned = plan != nullptr;
     auto project_delta = [&](bool partials, const float *w) -> int {
         const int nch = (npix + L.chunk - 1) / L.chunk;
         for (int c = 0; c < nch; ++c) {
@@ -646,6 +670,32 @@ int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_re
     }
     // (l) error sinogram update, alpha read from scal[1]
     return launch_axpy(e, e_dtype, dsino, delta_dtype, scal + 1, nsino, st);
+}
+
+}  // namespace mbx
+
+extern "C" {
+
+int mbx_vcd_subset_update(const mbx_geom *g, const float *angles, float *flat_recon, void *e, int32_t e_dtype,
+                          const float *weights, const float *fm_hessian, const int32_t *pix, int32_t npix,
+                          double fm_constant, const mbx_qggmrf *qparams, const float *prox_input, double sigma_prox,
+                          int32_t positivity, int32_t delta_dtype, double *stats, void *workspace,
+                          int64_t workspace_bytes, void *stream) {
+    return vcd_update(g, angles, flat_recon, e, e_dtype, weights, fm_hessian, pix, npix, fm_constant, qparams,
+                      prox_input, sigma_prox, positivity, delta_dtype, nullptr, 0, stats, workspace, workspace_bytes,
+                      stream);
+}
+
+int mbx_vcd_subset_update_planned(const mbx_geom *g, const float *angles, float *flat_recon, void *e,
+                                  int32_t e_dtype, const float *weights, const float *fm_hessian,
+                                  const int32_t *pix, int32_t npix, double fm_constant, const mbx_qggmrf *qparams,
+                                  const float *prox_input, double sigma_prox, int32_t positivity,
+                                  int32_t delta_dtype, void *plan, int64_t plan_bytes, double *stats,
+                                  void *workspace, int64_t workspace_bytes, void *stream) {
+    if (!plan) return fail(MBX_EINVAL, "NULL plan");
+    return vcd_update(g, angles, flat_recon, e, e_dtype, weights, fm_hessian, pix, npix, fm_constant, qparams,
+                      prox_input, sigma_prox, positivity, delta_dtype, plan, plan_bytes, stats, workspace,
+                      workspace_bytes, stream);
 }
 
 }  // extern "C"

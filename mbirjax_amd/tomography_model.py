@@ -598,10 +598,13 @@ class TomographyModel(ParameterHandler):
         nmae_update = np.zeros(max_iters)
         alpha_values = np.zeros(max_iters)
         num_iters = 0
+        # partitions swept more than once keep their subsets' projector setups
+        uses = np.bincount(np.asarray(partition_sequence, dtype=np.int64), minlength=len(parts_dev))
         for i in range(max_iters):
-            part = parts_dev[int(partition_sequence[i])]
+            k = int(partition_sequence[i])
+            part = parts_dev[k]
             order = np.random.permutation(part.shape[0])
-            ell1, alpha_sum = state.iterate(part, order)
+            ell1, alpha_sum = state.iterate(part, order, cache=bool(uses[k] > 1))
             fm_rmse[i] = state.forward_model_rmse()
             nmae_update[i] = ell1 / float(torch.sum(torch.abs(state.flat_recon), dtype=torch.float64).item())
             alpha_values[i] = alpha_sum / part.shape[0]
@@ -706,7 +709,11 @@ class VCDState:
     bfloat16 by precision), weights, Hessian diagonal and workspace.  With
     float16 storage the state holds the problem scaled by 1/scale (a power of
     two): recon() and error_sinogram() undo it.  iterate() runs one partition
-    through mbx_vcd_subset_update with no host syncs."""
+    through mbx_vcd_subset_update with no host syncs; with cache=True each
+    subset's projector setup (geometry table, bins, forward records:
+    mbx_forward_plan) is built on its first update and reused by
+    mbx_vcd_subset_update_planned, within half the device memory that was free
+    when the state was made."""
 
     def __init__(self, model, e, e_dtype, delta_dtype, w, flat_recon, fm_hessian, prox_input, precision,
                  max_subset_size, scale=1.0):
@@ -731,6 +738,10 @@ class VCDState:
         self.ws = torch.empty(self.wsb, dtype=torch.uint8, device=e.device)
         self.stats = torch.zeros(4, dtype=torch.float64, device=e.device)
         self.avg_w = 1.0 if w is None else float(torch.mean(w, dtype=torch.float64).item())
+        # forward plans of subsets: (partition data pointer, row) -> (partition, its version, plan or None)
+        self.plans = {}
+        self.plan_bytes = 0
+        self.plan_budget = torch.cuda.mem_get_info(e.device)[0] // 2
 
     def recon(self):
         """The flat recon (rows*cols, Nz) in the caller's units."""
@@ -741,21 +752,54 @@ class VCDState:
         e = self.e.float()
         return e if self.scale == 1.0 else e * self.scale
 
-    def update_subset(self, subset):
-        """One subset update (vcd_subset_updater); subset is an int32 device tensor."""
+    def update_subset(self, subset, plan=None):
+        """One subset update (vcd_subset_updater); subset is an int32 device tensor,
+        plan (optional) its ForwardPlan over all views."""
         m = self.model
-        _lib.check(_lib.lib().mbx_vcd_subset_update(
-            m._geom, _lib.ptr(m._angles()), _lib.ptr(self.flat_recon), _lib.ptr(self.e), self.e_dtype,
-            _lib.ptr(self.w), _lib.ptr(self.fm_hessian), _lib.ptr(subset), int(subset.numel()), self.fm_constant,
-            self.qstruct, _lib.ptr(self.prox_input), self.sigma_prox, int(self.positivity), self.delta_dtype,
-            _lib.ptr(self.stats), _lib.ptr(self.ws), self.wsb, _lib.stream()))
+        common = (m._geom, _lib.ptr(m._angles()), _lib.ptr(self.flat_recon), _lib.ptr(self.e), self.e_dtype,
+                  _lib.ptr(self.w), _lib.ptr(self.fm_hessian), _lib.ptr(subset), int(subset.numel()),
+                  self.fm_constant, self.qstruct, _lib.ptr(self.prox_input), self.sigma_prox, int(self.positivity),
+                  self.delta_dtype)
+        if plan is None:
+            _lib.check(_lib.lib().mbx_vcd_subset_update(*common, _lib.ptr(self.stats), _lib.ptr(self.ws), self.wsb,
+                                                        _lib.stream()))
+        else:
+            _lib.check(_lib.lib().mbx_vcd_subset_update_planned(*common, _lib.ptr(plan.buf), plan.nbytes,
+                                                                _lib.ptr(self.stats), _lib.ptr(self.ws), self.wsb,
+                                                                _lib.stream()))
 
-    def iterate(self, part, order):
+    def _plan(self, part, s):
+        """The forward plan of subset part[s] (built on its first use), or None when
+        the subset spans more than one geometry-table chunk or the cache is full."""
+        key = (part.data_ptr(), int(s))
+        hit = self.plans.get(key)
+        if hit is not None and hit[0] is part and hit[1] == part._version:
+            return hit[2]
+        if hit is not None and hit[2] is not None:
+            self.plan_bytes -= hit[2].nbytes
+        self.plans.pop(key, None)
+        subset = part[int(s)]
+        V = self.model.get_params('sinogram_shape')[0]
+        nb = int(_lib.lib().mbx_forward_workspace_bytes(self.model._geom, int(subset.numel()), V))
+        if self.plan_bytes + nb > self.plan_budget:
+            return None
+        try:
+            plan = self.model._fwd_plan(subset)
+        except RuntimeError:  # more pixels than one geometry-table chunk
+            plan = None
+        self.plans[key] = (part, part._version, plan)
+        if plan is not None:
+            self.plan_bytes += plan.nbytes
+        return plan
+
+    def iterate(self, part, order, cache=True):
         """All subsets of a partition (K, nS) in the given order (vcd_partition_iterator);
-        returns (sum |alpha delta|, sum alpha) -- one host sync."""
+        returns (sum |alpha delta|, sum alpha) -- one host sync.  cache: reuse
+        each subset's projector setup across calls with this partition tensor
+        (bitwise the same results)."""
         self.stats.zero_()
         for s in order:
-            self.update_subset(part[int(s)])
+            self.update_subset(part[int(s)], self._plan(part, s) if cache else None)
         ell1, alpha_sum, _, _ = self.stats.cpu().tolist()
         return ell1, alpha_sum
 

@@ -400,3 +400,33 @@ def test_padded_subset_duplicates(gpu):
     err = np.linalg.norm(x - flat) / np.linalg.norm(flat)
     print("padded subset update vs oracle %.2e" % err, flush=True)
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("weighted_positive", [False, True])
+def test_planned_subset_updates_bitwise_equal(gpu, weighted_positive):
+    """Sweeping a partition with each subset's projector setup cached
+    (mbx_vcd_subset_update_planned, the plan built on the first sweep) gives
+    bit for bit the recon, error sinogram and statistics of sweeps that build
+    the setup in every update; also with weights and positivity (whose second
+    projection of delta reuses the plan's bins)."""
+    m, g, phantom, sino = problem("cone")
+    m.set_params(sharpness=0.0, verbose=0, positivity_flag=weighted_positive)
+    w = (np.random.default_rng(5).uniform(0.5, 1.5, sino.shape).astype(np.float32)
+         if weighted_positive else None)
+    m.auto_set_regularization_params(sino, weights=w)
+    np.random.seed(4)
+    parts = mj.vcd_utils.gen_set_of_pixel_partitions(g.recon_shape, [4, 16])
+    nmax = max(int(p.shape[1]) for p in parts)
+    states = [m.vcd_setup(sino, weights=w, init_recon=None, precision='fp32', max_subset_size=nmax)
+              for _ in range(2)]
+    dev = states[0].e.device
+    parts = [torch.as_tensor(np.ascontiguousarray(p, dtype=np.int32), device=dev) for p in parts]
+    rng = np.random.RandomState(6)
+    for k in (0, 1, 1, 0, 1):
+        order = rng.permutation(parts[k].shape[0])
+        a = states[0].iterate(parts[k], order, cache=True)
+        b = states[1].iterate(parts[k], order, cache=False)
+        assert a == b
+        assert torch.equal(states[0].flat_recon, states[1].flat_recon)
+        assert torch.equal(states[0].e, states[1].e)
+    assert len(states[0].plans) == 4 + 16 and states[0].plan_bytes > 0 and not states[1].plans

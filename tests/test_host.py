@@ -113,6 +113,15 @@ def test_validation_before_any_hip_call():
     assert L.mbx_vcd_subset_update(geom(), fake, fake, fake, 3, None, fake, fake, 4, 1.0, q, None, 1.0, 0, 3,
                                    None, fake, 1 << 30, None) == _lib.MBX_EINVAL
     assert L.mbx_sino_transpose(fake, 3, 0, ctypes.c_void_p(32), 1, 0, 2, 3, 4, None) == _lib.MBX_EUNSUPPORTED
+    # the planned subset update: a plan is required, and its size must be that
+    # of a forward plan of these pixels over all views
+    assert L.mbx_vcd_subset_update_planned(geom(), fake, fake, fake, 0, None, fake, fake, 4, 1.0, q, None, 1.0, 0,
+                                           0, None, 0, None, fake, 1 << 30, None) == _lib.MBX_EINVAL
+    need = L.mbx_forward_workspace_bytes(geom(), 4, geom().num_views)
+    assert need > 0
+    assert L.mbx_vcd_subset_update_planned(geom(), fake, fake, fake, 0, None, fake, fake, 4, 1.0, q, None, 1.0, 0,
+                                           0, fake, need + 256, None, fake, 1 << 30, None) == _lib.MBX_EINVAL
+    assert b"plan_bytes" in L.mbx_last_error()
 
 
 @pytest.mark.parametrize("views,slices", [(64, 512), (32, 512), (16, 1024)])
