@@ -181,10 +181,11 @@ def run_full(model, y, iterations=20, precision="fp32"):
     return res
 
 
-def cpu_baseline(model, part_np, view_sample=72):
-    """float32 oracle port (oracle/liboracle_f32.so, OpenMP over the host cores)
-    on one K=128 subset, back + forward over a sample of views, scaled to one
-    iteration (K subsets x all views)."""
+def cpu_baseline(model, part_np, min_seconds=10.0, max_subsets=8):
+    """float32 oracle port (oracle/liboracle_f32.so, OpenMP over the host cores):
+    back + forward projection of whole K=128 subsets over all views (the work
+    of a subset update), subset after subset until at least min_seconds of host
+    time (at most max_subsets), scaled to one iteration (all K subsets)."""
     from oracle import projectors as op
     V, R, C = model.get_params('sinogram_shape')
     rshape = model.get_params('recon_shape')
@@ -193,21 +194,23 @@ def cpu_baseline(model, part_np, view_sample=72):
     cores = len(os.sched_getaffinity(0))
     if os.environ.get("OMP_NUM_THREADS", "").isdigit():
         cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
-    sub = part_np[0]
-    views = np.linspace(0, V - 1, view_sample).astype(np.int32)
     rng = np.random.default_rng(0)
-    vox = rng.standard_normal((sub.size, rshape[2])).astype(np.float32)
-    sino = np.zeros((V, R, C), np.float32)
-    sino[views] = rng.standard_normal((view_sample, R, C)).astype(np.float32)
-    t0 = time.time()
-    op.back(geom, sino, sub, views=views, real=np.float32)
-    op.forward(geom, vox, sub, views=views, real=np.float32)
-    dt = time.time() - t0
-    t_iter = dt * (V / view_sample) * part_np.shape[0]
+    sino = rng.standard_normal((V, R, C)).astype(np.float32)
+    done, dt = 0, 0.0
+    while done < min(max_subsets, part_np.shape[0]) and (done == 0 or dt < min_seconds):
+        sub = part_np[done]
+        vox = rng.standard_normal((sub.size, rshape[2])).astype(np.float32)
+        t0 = time.time()
+        op.back(geom, sino, sub, real=np.float32)
+        op.forward(geom, vox, sub, real=np.float32)
+        dt += time.time() - t0
+        done += 1
+    t_iter = dt / done * part_np.shape[0]
     return dict(value=1.0 / t_iter, unit="iter/s", cores=cores, kind="port", cpu_model=cpu_model(),
-                sample="float32 oracle port, back+forward of one %d-pixel subset over %d of %d views (%.2f s), "
-                       "scaled x%d views x%d subsets" % (sub.size, view_sample, V, dt, V // view_sample,
-                                                         part_np.shape[0]))
+                sample="float32 oracle port, back+forward projection of %d whole subset%s (%d pixels x %d views "
+                       "each) in %.1f s, scaled x%.0f to the %d subsets of one iteration" % (
+                           done, "" if done == 1 else "s", part_np.shape[1], V, dt, part_np.shape[0] / done,
+                           part_np.shape[0]))
 
 
 def cpu_model():
