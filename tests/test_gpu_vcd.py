@@ -402,23 +402,24 @@ def test_padded_subset_duplicates(gpu):
     assert err <= 1e-5
 
 
-@pytest.mark.parametrize("weighted_positive", [False, True])
-def test_planned_subset_updates_bitwise_equal(gpu, weighted_positive):
+@pytest.mark.parametrize("mode", ["plain", "weighted_positive", "prox"])
+def test_planned_subset_updates_bitwise_equal(gpu, mode):
     """Sweeping a partition with each subset's projector setup cached
     (mbx_vcd_subset_update_planned, the plan built on the first sweep) gives
     bit for bit the recon, error sinogram and statistics of sweeps that build
     the setup in every update; also with weights and positivity (whose second
-    projection of delta reuses the plan's bins)."""
+    projection of delta reuses the plan's bins) and for the proximal map."""
     m, g, phantom, sino = problem("cone")
-    m.set_params(sharpness=0.0, verbose=0, positivity_flag=weighted_positive)
+    m.set_params(sharpness=0.0, verbose=0, positivity_flag=(mode == "weighted_positive"))
     w = (np.random.default_rng(5).uniform(0.5, 1.5, sino.shape).astype(np.float32)
-         if weighted_positive else None)
+         if mode == "weighted_positive" else None)
     m.auto_set_regularization_params(sino, weights=w)
+    prox = phantom.astype(np.float32) if mode == "prox" else None
     np.random.seed(4)
     parts = mj.vcd_utils.gen_set_of_pixel_partitions(g.recon_shape, [4, 16])
     nmax = max(int(p.shape[1]) for p in parts)
-    states = [m.vcd_setup(sino, weights=w, init_recon=None, precision='fp32', max_subset_size=nmax)
-              for _ in range(2)]
+    states = [m.vcd_setup(sino, weights=w, init_recon=None, prox_input=prox, precision='fp32',
+                          max_subset_size=nmax) for _ in range(2)]
     dev = states[0].e.device
     parts = [torch.as_tensor(np.ascontiguousarray(p, dtype=np.int32), device=dev) for p in parts]
     rng = np.random.RandomState(6)
