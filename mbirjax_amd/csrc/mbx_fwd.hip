@@ -82,10 +82,11 @@ int64_t fwd_blocks(const DevGeom &g, int nviews, int layout) { return plan_fwd(g
 // (centre channel, then pixel): what the forward update reads per candidate
 // pixel, so that a block walks its candidates through contiguous scalar loads.
 //   [0..2] Ac(n_c-1 .. n_c+1)   [3] W   [4] W_lo
-//   [5] acr (acr - (1-W)/2 in the two-slice mode)   [6] 1/W   [7] -acr/W
+//   [5] acr (acr - (1-W)/2 in the two-slice form)   [6] 1/W   [7] -acr/W
 //   [8] ai (float)   [9], [10] clamp range of the resampling slice index,
 //   relative to kref   [11] kf | span << 16: the slices kf .. kf+span-1 reach
-//   a detector row   [12] bit g: a row of 128g .. 128g+127 is reached (all
+//   a detector row; bit 31 set: the two-slice form in a unit-sum launch
+//   [12] bit g: a row of 128g .. 128g+127 is reached (all
 //   bits when the detector has more than 4096 rows)   [13] (1-W)/2
 //   [14] v/sdd at kref   [15] its step per slice
 constexpr int kFRec = 16;
@@ -215,6 +216,7 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
         o[1] = R0.w;
         o[2] = R1.x;
         int rlo = 0, rhi = g.R - 1, kf = 0, span = 0;
+        bool two = false;
         constexpr int P = kFwdPad;
         float clo = 0.0f, chi = 0.0f;
         if (is_fan(g)) {
@@ -232,9 +234,13 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
                 rhi = 0;
             }
             const float q1 = hv - W;
+            // In the unit-sum mode a candidate with W >= 1 takes the two-slice
+            // form (its third tap always weighs 0: a window of width W + 1 holds
+            // at most two slices W apart), flagged in bit 31 of word 11.
+            two = vmode == kVertTwo || (vmode == kVertUnit && (W > 1.0f || (W == 1.0f && Wlo >= 0.0f)));
             o[3] = W;
             o[4] = Wlo;
-            o[5] = vmode == kVertTwo ? acr - q1 : acr;
+            o[5] = two ? acr - q1 : acr;
             o[6] = invW;
             o[7] = -(acr * invW);
             o[8] = (float)ai;
@@ -244,7 +250,7 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
             // staged slice i of the cylinder is slice kf - P + i; slices kf-P ..
             // kf-1 and kf+span .. kf+span+P-1 are zeros.  Clamping the nearest
             // (or lower) slice keeps every tap there; a clamped row only meets zeros.
-            if (vmode == kVertTwo) {
+            if (two) {
                 clo = (float)(kf - g.kref - 2);
                 chi = (float)(kf - g.kref + span);
             } else {
@@ -264,7 +270,7 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
         }
         o[9] = clo;
         o[10] = chi;
-        o[11] = __int_as_float(kf | (span << 16));
+        o[11] = __int_as_float(kf | (span << 16) | (two && vmode == kVertUnit ? (int)0x80000000u : 0));
         unsigned gm = 0;
         if (g.R > 32 * 128) {
             gm = rlo <= rhi ? 0xffffffffu : 0u;
@@ -431,7 +437,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         const int nb = min(nbatch, hi - lo - bi * nbatch);
         for (int slot = wave; slot < nb; slot += nw) {
             const int ks = __builtin_amdgcn_readlane(q.ks, slot);
-            const int kf = ks & 0xffff, span = ks >> 16;
+            const int kf = ks & 0xffff, span = (ks >> 16) & 0x7fff;
             if (span == 0) continue;
             const int sc = __builtin_amdgcn_readlane(q.sc, slot);
             // slices past the cylinder end read 0 (num_records bounds the buffer)
@@ -458,7 +464,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         const int nb = min(nbatch, hi - lo - bi * nbatch);
         for (int slot = wave; slot < nb; slot += nw) {
             const int ks = __builtin_amdgcn_readlane(q.ks, slot);
-            const int kf = ks & 0xffff, span = ks >> 16;
+            const int kf = ks & 0xffff, span = (ks >> 16) & 0x7fff;
             if (span == 0) continue;  // no row reaches it: never read
             float *cyl = buf + (size_t)slot * cstride;
             const int nch = (span + kWave - 1) / kWave;
@@ -536,6 +542,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                     v[r] = fmaf(f, c1 - c0, c0);
                 }
             } else if (VM == kVertUnit) {
+                if (__float_as_int(R2.w) < 0) {  // this candidate's W >= 1: the two-slice form
+#pragma unroll
+                    for (int r = 0; r < RPL; ++r) {
+                        const float mp = mrow[r] - aif;  // m - ai, exact
+                        const float jf = floorf(__builtin_amdgcn_fmed3f(fmaf(mp, invW, nai), clo, chi));
+                        const float dm = fmaf(W, jf, -mp) + fmaf(Wlo, jf, acr);
+                        const float f = __builtin_amdgcn_fmed3f(-dm, 0.0f, 1.0f);
+                        const float *cp = lds + (ib + (int)jf) + RB;  // slice jf (the base is one slice lower)
+                        const float c0 = cp[0], c1 = cp[1];
+                        v[r] = fmaf(f, c1 - c0, c0);
+                    }
+                    return;
+                }
 #pragma unroll
                 for (int r = 0; r < RPL; ++r) {
                     const float mp = mrow[r] - aif;  // m - ai, exact
