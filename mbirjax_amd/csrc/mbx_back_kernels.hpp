@@ -38,7 +38,13 @@ struct ColumnSource {
 // mc-1, mc, mc+1 sum to W (a box of width W convolved with a box of width 1,
 // sampled at unit spacing), and the side weights (W-1)/2 -+ d never exceed
 // W/2 <= min(1, W), so the centre row carries W minus the two sides.
-template <int KPL, int RAD, int POW, bool TAIL, bool FIT, bool UNIT>
+// TWO (UNIT with W <= 1): the window of half-width (W+1)/2 <= 1 reaches only
+// rows floor(t) and floor(t)+1, at distances f and 1-f (f = t - floor(t)).
+// Their weights clip((W+1)/2 - f, 0, W) and clip(f - (1-W)/2, 0, W) still sum
+// to W, so the upper row carries med3(f - (1-W)/2, 0, W) and the lower row the
+// rest.  The caller passes acd - 1/2, so that rint of the shifted position is
+// floor(t) and f = 1/2 + d.
+template <int KPL, int RAD, int POW, bool TAIL, bool FIT, bool UNIT, bool TWO = false>
 __device__ __forceinline__ void back_vertical(const float *col, int shift, int n_rows, double acd, double Wd,
                                               float Wf, float hv, float Lv, float c0, float c1, float kr0,
                                               float krlast, float (&acc)[KPL]) {
@@ -48,7 +54,7 @@ __device__ __forceinline__ void back_vertical(const float *col, int shift, int n
     // the rounded integer.
     constexpr double kMagic = 6755399441055744.0;
     const float *colp = col - shift;  // colp[m - RAD] is row ai + m (32-bit LDS address arithmetic)
-    const float hm1 = hv - 1.0f;
+    const float hm1 = hv - 1.0f, hW = 0.5f * Wf;
     const double W64 = Wd * (double)kWave;
     const float u0 = fmaf(c1, kr0, c0), c164 = c1 * (float)kWave;
     double t = fma(Wd, (double)kr0, acd);
@@ -67,7 +73,21 @@ __device__ __forceinline__ void back_vertical(const float *col, int shift, int n
         const int mi = (int)(uint32_t)__builtin_bit_cast(uint64_t, sh);
         const float d = (float)(t - (sh - kMagic));  // |d| <= 1/2
         float sum;
-        if (RAD == 1) {
+        if constexpr (TWO) {
+            static_assert(RAD == 1 && UNIT, "the two-row form is a unit-sum radius-1 form");
+            float clo, chi;  // rows floor(t), floor(t) + 1
+            if (FIT) {
+                const float *q = colp + mi;
+                clo = q[1];
+                chi = q[2];
+            } else {
+                const int base = mi - shift;
+                clo = col[min(max(base + 1, 0), n_rows - 1)];
+                chi = col[min(max(base + 2, 0), n_rows - 1)];
+            }
+            const float whi = __builtin_amdgcn_fmed3f(d + hW, 0.0f, Wf);
+            sum = fmaf(whi, chi - clo, Wf * clo);
+        } else if (RAD == 1) {
             float cm, cc, cp;
             if (FIT) {
                 const float *q = colp + mi;  // one address, three immediate offsets
@@ -219,7 +239,14 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *_
         const float kr0 = (float)(k0 + lane - g.kref), krl = (float)(kend - 1 - g.kref);
         const float Wf = rec[7];
         constexpr bool CAN_UNIT = RAD == 1 && POW == 1;
-        if (CAN_UNIT && (flags & kBackUnit)) {  // wave-uniform
+        if (CAN_UNIT && (flags & kBackUnit) && Wd <= 1.0) {  // wave-uniform
+            if (fit)
+                back_vertical<KPL, RAD, POW, TAIL, true, CAN_UNIT, CAN_UNIT>(col, shift, n_rows, acd - 0.5, Wd, Wf, hv,
+                                                                             Lv, c0, c1, kr0, krl, acc);
+            else
+                back_vertical<KPL, RAD, POW, TAIL, false, CAN_UNIT, CAN_UNIT>(col, shift, n_rows, acd - 0.5, Wd, Wf,
+                                                                              hv, Lv, c0, c1, kr0, krl, acc);
+        } else if (CAN_UNIT && (flags & kBackUnit)) {
             if (fit)
                 back_vertical<KPL, RAD, POW, TAIL, true, CAN_UNIT>(col, shift, n_rows, acd, Wd, Wf, hv, Lv, c0, c1,
                                                                    kr0, krl, acc);
