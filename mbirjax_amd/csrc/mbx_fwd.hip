@@ -721,20 +721,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         const int boff = (bi & 1) * nbatch * cstride;  // LDS float offset of this batch
         if constexpr (SW) {
             if (wm0 < m1) {  // wave-uniform
-                // the batch's candidates in list order; the window follows their buckets
+                // the batch's candidates in list order; the window follows their
+                // buckets: the candidates of the current bucket in an inner loop
+                // (nothing but their taps touches the window there), the window's
+                // advances between buckets
                 const float4 *rp = (const float4 *)(FRv + (size_t)b0 * kFRec);
                 int sb = boff + g.kref + P - (VM == kVertTwo ? 0 : RB);
-                for (int slot = 0; slot < nb; ++slot, rp += 4, sb += cstride) {
-#pragma clang loop unroll(disable)
-                    while (b0 + slot >= nextb) advance();
-                    const float4 R0 = rp[0], R1 = rp[1], R2 = rp[2], R3 = rp[3];
-                    asm volatile("" ::"s"(R0.x), "s"(R0.y), "s"(R0.z), "s"(R0.w), "s"(R1.x), "s"(R1.y), "s"(R1.z),
-                                 "s"(R1.w), "s"(R2.x), "s"(R2.y), "s"(R2.z), "s"(R2.w), "s"(R3.x), "s"(R3.y));
-                    if (GEOM != MBX_GEOM_CONE || ((__float_as_uint(R3.x) >> wgrp) & gbits)) {
-                        float v[RPL];
-                        vert(R0, R1, R2, R3, b0 + slot, slot, sb, v);
-                        add_window(R0, b0 + slot, v);
+                int slot = 0;
+                for (;;) {
+                    const int send = min(nextb - b0, nb);  // end of the current bucket in this batch
+                    for (; slot < send; ++slot, rp += 4, sb += cstride) {
+                        const float4 R0 = rp[0], R1 = rp[1], R2 = rp[2], R3 = rp[3];
+                        asm volatile("" ::"s"(R0.x), "s"(R0.y), "s"(R0.z), "s"(R0.w), "s"(R1.x), "s"(R1.y),
+                                     "s"(R1.z), "s"(R1.w), "s"(R2.x), "s"(R2.y), "s"(R2.z), "s"(R2.w), "s"(R3.x),
+                                     "s"(R3.y));
+                        if (GEOM != MBX_GEOM_CONE || ((__float_as_uint(R3.x) >> wgrp) & gbits)) {
+                            float v[RPL];
+                            vert(R0, R1, R2, R3, b0 + slot, slot, sb, v);
+                            add_window(R0, b0 + slot, v);
+                        }
                     }
+                    if (slot >= nb) break;  // the bucket goes on in the next batch
+                    advance();              // the next candidate starts a later bucket
                 }
             }
         } else if (wm0 < m1) {  // wave-uniform
