@@ -60,6 +60,13 @@ __device__ __forceinline__ f16 from_float<f16>(float x) {
     return r;
 }
 
+// Streaming store (nontemporal hint): for outputs written once and not read
+// again by the same kernel, so that they do not evict from L2 what the kernel
+// does read again (the forward projector's staged voxel cylinders).
+__device__ __forceinline__ void store_stream(float *p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void store_stream(bf16 *p, bf16 v) { __builtin_nontemporal_store(v.bits, &p->bits); }
+__device__ __forceinline__ void store_stream(f16 *p, f16 v) { __builtin_nontemporal_store(v.bits, &p->bits); }
+
 // Four consecutive elements through one 16-byte (float) or 8-byte (bf16) access.
 __device__ __forceinline__ void load4(const float *p, float v[4]) {
     const float4 t = *(const float4 *)p;

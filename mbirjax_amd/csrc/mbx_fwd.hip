@@ -635,7 +635,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                                                   : ((size_t)vi * g.R + m) * (size_t)g.C + n;
         if (accumulate) v += to_float(sino[o]);
         const OT st = from_float<OT>(v);
-        sino[o] = st;
+        store_stream(&sino[o], st);  // written once: kept out of L2
         if (want_ss) {
             const float vs = to_float(st);
             ss += (double)vs * (double)vs * (hasw ? (double)w[o] : 1.0);
@@ -666,7 +666,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
 #pragma unroll
                 for (int r = 0; r < RPL; ++r) {
                     st[r] = from_float<OT>(v[r]);
-                    pch[lane + kWave * r] = st[r];
+                    store_stream(&pch[lane + kWave * r], st[r]);  // written once: kept out of L2
                 }
                 if (want_ss) {
                     if (hasw) {
