@@ -32,6 +32,8 @@ constexpr int kFwdMaxBatch = 16;  // cylinders staged per batch
 constexpr size_t kFwdLds = 36 * 1024;   // both staging buffers of a block, (views, rows, channels) output
 constexpr size_t kFwdLdsW = 20 * 1024;  // the same, (views, channels, rows) output: small batches, more blocks per CU
 constexpr int kBinMax = 16384;    // channel buckets the binning kernel holds in LDS
+constexpr int kFRecLds = 16;      // floats per forward record staged in LDS (window kernel)
+constexpr size_t kFwdRecLdsBytes = 2 * kFwdMaxBatch * kFRecLds * sizeof(float);  // its static LDS array
 
 FwdPlan plan_fwd(const DevGeom &g, int nviews, int layout) {
     FwdPlan p;
@@ -62,7 +64,8 @@ FwdPlan plan_fwd(const DevGeom &g, int nviews, int layout) {
     const size_t slot_bytes = 2 * (size_t)p.cstride * sizeof(float);
     if (p.nblocks * p.nw <= 2 * kFwdMinWaves) {
         const int64_t per_cu = (p.nblocks + 255) / 256;
-        const size_t fit = (size_t)(160 * 1024 / per_cu) - 1024 - 64 * sizeof(int);
+        const size_t fit = (size_t)(160 * 1024 / per_cu) - 1024 - 64 * sizeof(int) -
+                           (layout == MBX_LAYOUT_VCR ? kFwdRecLdsBytes : 0);
         if (fit < budget && fit >= 4 * slot_bytes) budget = fit;
     }
     p.nb = (int)std::max<size_t>(1, std::min<size_t>(kFwdMaxBatch, budget / slot_bytes));
@@ -371,6 +374,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     const int nw = blockDim.x >> 6;
     float *bufs = lds;                                                  // [2][nbatch][cstride]
     double *dscratch = (double *)(lds + 2 * (size_t)nbatch * cstride);  // [8]
+    // (window kernel) the batches' forward records: an LDS array of its own,
+    // apart from the cylinders' asynchronous copies
+    float *recs = nullptr;
+    if constexpr (SW) {
+        __shared__ float4 recs_s[kFwdRecLdsBytes / sizeof(float4)];  // [2][kFwdMaxBatch][16] floats
+        recs = (float *)recs_s;
+    }
 
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
@@ -420,6 +430,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     struct Meta {
         int ks, sc;
         float u0, du;
+        float4 rq;  // (window kernel) words 4 (l % 4) .. + 3 of the batch's record l / 4
     };
     auto load_meta = [&](int bi) {
         Meta q;
@@ -429,6 +440,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         q.sc = lv[i];
         q.u0 = rf[14];
         q.du = rf[15];
+        if constexpr (SW) {
+            const int ri = min(lo + bi * nbatch + (lane >> 2), hi - 1);
+            q.rq = *(const float4 *)(FRv + (size_t)ri * kFRec + 4 * (lane & 3));
+        }
         return q;
     };
     // asynchronous global -> LDS copy of the raw slices of this wave's slots
@@ -462,6 +477,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     auto finish = [&](const Meta &q, int bi) {
         float *buf = bufs + (size_t)(bi & 1) * nbatch * cstride;
         const int nb = min(nbatch, hi - lo - bi * nbatch);
+        if (SW && wave == nw - 1 && lane < 4 * nb)  // the batch's forward records, read by every wave
+            *(float4 *)(recs + (size_t)(bi & 1) * kFwdMaxBatch * kFRecLds + 4 * lane) = q.rq;
         for (int slot = wave; slot < nb; slot += nw) {
             const int ks = __builtin_amdgcn_readlane(q.ks, slot);
             const int kf = ks & 0xffff, span = (ks >> 16) & 0x7fff;
@@ -527,8 +544,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             const float W = R0.w, Wlo = R1.x, acr = R1.y, invW = R1.z, nai = R1.w, aif = R2.x;
             const float clo = R2.y, chi = R2.z, q1 = R3.y;
             // base at the lowest tap (LDS offsets are unsigned)
-            int ib = sb - (__float_as_int(R2.w) & 0xffff);
-            asm("" : "+s"(ib));  // one scalar base: the address is one shift-add per row
+            // one scalar base: the address is one shift-add per row
+            const int ib = __builtin_amdgcn_readfirstlane(sb - (__float_as_int(R2.w) & 0xffff));
             if (VM == kVertTwo) {
 #pragma unroll
                 for (int r = 0; r < RPL; ++r) {
@@ -725,16 +742,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                 // buckets: the candidates of the current bucket in an inner loop
                 // (nothing but their taps touches the window there), the window's
                 // advances between buckets
-                const float4 *rp = (const float4 *)(FRv + (size_t)b0 * kFRec);
+                // records staged in LDS with the batch: vector operands, read
+                // as 16-byte broadcasts
+                const float4 *rp = (const float4 *)(recs + (size_t)(bi & 1) * kFwdMaxBatch * kFRecLds);
                 int sb = boff + g.kref + P - (VM == kVertTwo ? 0 : RB);
                 int slot = 0;
                 for (;;) {
                     const int send = min(nextb - b0, nb);  // end of the current bucket in this batch
                     for (; slot < send; ++slot, rp += 4, sb += cstride) {
                         const float4 R0 = rp[0], R1 = rp[1], R2 = rp[2], R3 = rp[3];
-                        asm volatile("" ::"s"(R0.x), "s"(R0.y), "s"(R0.z), "s"(R0.w), "s"(R1.x), "s"(R1.y),
-                                     "s"(R1.z), "s"(R1.w), "s"(R2.x), "s"(R2.y), "s"(R2.z), "s"(R2.w), "s"(R3.x),
-                                     "s"(R3.y));
+                        asm volatile("" ::"v"(R0.x), "v"(R0.y), "v"(R0.z), "v"(R0.w), "v"(R1.x), "v"(R1.y),
+                                     "v"(R1.z), "v"(R1.w), "v"(R2.x), "v"(R2.y), "v"(R2.z), "v"(R2.w), "v"(R3.x),
+                                     "v"(R3.y));  // the record in one LDS round trip
                         if (GEOM != MBX_GEOM_CONE || ((__float_as_uint(R3.x) >> wgrp) & gbits)) {
                             float v[RPL];
                             vert(R0, R1, R2, R3, b0 + slot, slot, sb, v);
