@@ -197,6 +197,13 @@ __device__ __forceinline__ i32x4_t make_rsrc(const void *p) {
     return r;
 }
 
+// The same with num_records = nbytes: loads at byte offsets past it return 0.
+__device__ __forceinline__ i32x4_t make_rsrc_n(const void *p, int nbytes) {
+    i32x4_t r = make_rsrc(p);
+    r.z = nbytes;
+    return r;
+}
+
 template <typename T>
 __device__ __forceinline__ float buf_load(i32x4_t rsrc, int voffset, int soffset);
 template <>

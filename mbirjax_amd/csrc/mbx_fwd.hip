@@ -74,6 +74,12 @@ FwdPlan plan_fwd(const DevGeom &g, int nviews, int layout) {
 }
 
 static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// The window kernel stages each batch through registers (two 256-slice chunks
+// per slot, at most two slots per wave) when the batch and the spans fit.
+static bool fwd_reg_staging(const FwdPlan &p) {
+    return p.rpl == kFwdRPLW && p.nb <= 2 * p.nw && p.cstride <= 2 * 4 * kWave + 2 * kFwdPad + 8;
+}
 // FP64 partial slots of the squared norm: one per block of either output layout
 static int64_t fwd_part_blocks(const DevGeom &g, int nviews) {
     return std::max(plan_fwd(g, nviews, MBX_LAYOUT_VCR).nblocks, plan_fwd(g, nviews, MBX_LAYOUT_VRC).nblocks);
@@ -354,7 +360,7 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
 // puts every tap in a fixed register) and the tile written once.
 // Each output element has one writer and a fixed summation order (the list
 // order of its candidates, the same in both schemes).
-template <int GEOM, int RAD, int RB, int VM, typename OT, int LAYOUT, int RPLW>
+template <int GEOM, int RAD, int RB, int VM, typename OT, int LAYOUT, int RPLW, bool RS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void fwd_kernel(
     DevGeom g, const float *__restrict__ T, const float *__restrict__ FR, const int *__restrict__ list,
     const int *__restrict__ cstart, const float *__restrict__ vox, int64_t vstride, int npix, int ncb, int nrb,
@@ -446,10 +452,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         }
         return q;
     };
-    // asynchronous global -> LDS copy of the raw slices of this wave's slots
+    // (window kernel, fan geometries) register staging: each wave loads the
+    // raw slices of its (at most two) slots of the next batch into registers,
+    // 16 bytes per lane, while the current batch is in use, and scales and
+    // stores them to LDS when that batch starts: no asynchronous LDS copies
+    // and no LDS read-back.  Spans of at most 512 slices (two 256-slice chunks).
+    // RS: the launch satisfies fwd_reg_staging() (at most two slots per wave,
+    // spans of at most 512 slices); the kernel then has no asynchronous copies.
+    static_assert(!RS || (SW && GEOM == MBX_GEOM_CONE), "register staging: window kernel, fan geometries");
+    mbx_f32x4 sreg[2][2];  // [slot s of this wave][chunk]
+    // global -> registers (register staging) or asynchronous global -> LDS
+    // copy of the raw slices of this wave's slots
     auto issue = [&](const Meta &q, int bi) {
         float *buf = bufs + (size_t)(bi & 1) * nbatch * cstride;
         const int nb = min(nbatch, hi - lo - bi * nbatch);
+        if constexpr (RS) {
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int slot = wave + t * nw;  // < 8: a valid lane to read
+                const int ks = slot < nb ? __builtin_amdgcn_readlane(q.ks, slot) : 0;
+                const int kf = ks & 0xffff, span = (ks >> 16) & 0x7fff;
+                const int sc = __builtin_amdgcn_readlane(q.sc, slot);
+                // slices past the cylinder end read 0 (num_records bounds the buffer)
+                const i32x4_t rsrc = make_rsrc_n(vox + sc * vstride + kf, (g.Nz - kf) * 4);
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+                    if (c * 4 * kWave < span) sreg[t][c] = raw_load_f32x4(rsrc, (c * kWave + lane) * 16, 0, 0);
+            }
+            return;
+        }
         for (int slot = wave; slot < nb; slot += nw) {
             const int ks = __builtin_amdgcn_readlane(q.ks, slot);
             const int kf = ks & 0xffff, span = (ks >> 16) & 0x7fff;
@@ -479,6 +510,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         const int nb = min(nbatch, hi - lo - bi * nbatch);
         if (SW && wave == nw - 1 && lane < 4 * nb)  // the batch's forward records, read by every wave
             *(float4 *)(recs + (size_t)(bi & 1) * kFwdMaxBatch * kFRecLds + 4 * lane) = q.rq;
+        if constexpr (RS) {
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int slot = wave + t * nw;
+                if (slot >= nb) break;
+                const int ks = __builtin_amdgcn_readlane(q.ks, slot);
+                const int kf = ks & 0xffff, span = (ks >> 16) & 0x7fff;
+                if (span == 0) continue;  // no row reaches it: never read
+                float *cyl = buf + (size_t)slot * cstride;
+                const float c0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, q.u0), slot));
+                const float c1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, q.du), slot));
+                const float du = c1 * (float)kWave;
+                // lane l holds slices 256 c + 4 l + j: v/sdd as in the copy path,
+                // from the slice's position within its 64-slice chunk and the chunk
+                float w4[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) w4[j] = fmaf(c1, (float)(kf + 4 * (lane & 15) + j - g.kref), c0);
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    if (c * 4 * kWave >= span) break;
+                    const float cf = (float)(4 * c + (lane >> 4));
+                    mbx_f32x4 y;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const float uu = fmaf(du, cf, w4[j]);
+                        const float x = sreg[t][c][j] * __builtin_amdgcn_sqrtf(fmaf(uu, uu, 1.0f));
+                        y[j] = c * 4 * kWave + 4 * lane + j < span ? x : 0.0f;  // zero past the span
+                    }
+                    *(mbx_f32x4 *)(cyl + P + c * 4 * kWave + 4 * lane) = y;  // 16-byte aligned
+                }
+                if (lane < P) {
+                    cyl[lane] = 0.0f;
+                    cyl[P + span + lane] = 0.0f;
+                }
+            }
+            return;
+        }
         for (int slot = wave; slot < nb; slot += nw) {
             const int ks = __builtin_amdgcn_readlane(q.ks, slot);
             const int kf = ks & 0xffff, span = (ks >> 16) & 0x7fff;
@@ -727,7 +795,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         mB = load_meta(min(1, nbat - 1));
     }
     for (int bi = 0; bi < nbat; ++bi) {
-        __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of batch bi (and the next batch's data) have landed
+        if constexpr (!RS)
+            __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of batch bi (and the next batch's data) have landed
         finish(mA, bi);
         __syncthreads();  // batch bi staged by every wave; batch bi-1 no longer read
         issue(mB, bi + 1);  // no slots past the last batch
@@ -817,13 +886,22 @@ static void launch_fwd_t(const DevGeom &g, const FwdPlan &pl, const float *T, co
     // rows per lane of the window kernel: the plan's (the row-major kernel has its own)
     if constexpr (LAYOUT == MBX_LAYOUT_VCR) {
         if (pl.rpl == 2) {
-            hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, VM, OT, LAYOUT, 2>), dim3((unsigned)pl.nblocks),
+            hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, VM, OT, LAYOUT, 2, false>), dim3((unsigned)pl.nblocks),
                                dim3(64 * pl.nw), pl.lds, st, g, T, FR, list, cstart, vox, vstride, npix, pl.ncb,
                                pl.nrb, pl.cb, pl.nb, pl.cstride, sino, accumulate, w, part);
             return;
         }
+        if constexpr (GEOM == MBX_GEOM_CONE) {
+            if (fwd_reg_staging(pl)) {
+                hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, VM, OT, LAYOUT, kFwdRPLW, true>),
+                                   dim3((unsigned)pl.nblocks), dim3(64 * pl.nw), pl.lds, st, g, T, FR, list, cstart,
+                                   vox, vstride, npix, pl.ncb, pl.nrb, pl.cb, pl.nb, pl.cstride, sino, accumulate, w,
+                                   part);
+                return;
+            }
+        }
     }
-    hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, VM, OT, LAYOUT, kFwdRPLW>), dim3((unsigned)pl.nblocks),
+    hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, VM, OT, LAYOUT, kFwdRPLW, false>), dim3((unsigned)pl.nblocks),
                        dim3(64 * pl.nw), pl.lds, st, g, T, FR, list, cstart, vox, vstride, npix, pl.ncb, pl.nrb, pl.cb,
                        pl.nb, pl.cstride, sino, accumulate, w, part);
 }

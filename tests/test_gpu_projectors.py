@@ -83,6 +83,12 @@ CASES = {
     "cone_bp2": lambda: make_cone(12, 24, 24, 96.0, 48.0, delta_voxel=0.2, recon_shape=(20, 20, 40)),
     "parallel_psf2": lambda: make_par(10, 8, 20, delta_voxel=2.5, recon_shape=(8, 8, 8)),
     "translation_psf2": lambda: make_trans(12, 20, 36, 144.0, 72.0, (6, 30, 24), 1.5, delta_voxel=1.2),
+    # more than 256 detector rows and 449..512 slices: the forward stages its
+    # batches through registers; cylinders partly off the detector (spans cut
+    # at its edges) and offsets
+    "cone_regstage": lambda: make_cone(10, 300, 40, 240.0, 120.0, recon_shape=(12, 12, 480), det_row_offset=2.7,
+                                       recon_slice_offset=-1.3),
+    "translation_regstage": lambda: make_trans(8, 300, 40, 240.0, 120.0, (6, 12, 470), 1.5),
 }
 
 
