@@ -185,6 +185,24 @@ int mbx_back_project(const mbx_geom *g, const float *angles, const void *sino, i
                      const int32_t *views, int32_t nviews, int32_t coeff_power, float scale, int32_t accumulate,
                      float *out, void *workspace, int64_t workspace_bytes, void *stream);
 
+/* The back projection of a planned pixel set (no counterpart in mbirjax, whose
+ * projector recomputes its geometry on every call; for callers that
+ * back-project onto the same pixel set many times, such as the backward pass
+ * of an unrolled or autograd loop over a fixed subset, and for timing the
+ * back-projection kernel alone).  mbx_back_project of the npix pixels and the
+ * views a plan of mbx_forward_plan was built with, reading the plan's
+ * geometry table instead of building one: same arguments and result, with
+ * plan and plan_bytes as for mbx_forward_apply (plan_bytes must equal
+ * mbx_forward_workspace_bytes(g, npix, nviews), else MBX_EINVAL) and a
+ * workspace of mbx_back_apply_workspace_bytes(g, npix, nviews) bytes (the
+ * view-group partials; 0 bytes needs no workspace).  The plan is only read,
+ * so back and forward applies of one plan may follow each other on a stream. */
+int64_t mbx_back_apply_workspace_bytes(const mbx_geom *g, int32_t npix, int32_t nviews);
+int mbx_back_apply(const mbx_geom *g, const void *plan, int64_t plan_bytes, const void *sino, int32_t sino_dtype,
+                   int32_t num_sino_views, const float *weights, int32_t npix, const int32_t *views, int32_t nviews,
+                   int32_t coeff_power, float scale, int32_t accumulate, float *out, void *workspace,
+                   int64_t workspace_bytes, void *stream);
+
 /* mbx_back_project without a workspace (no counterpart in mbirjax: for callers
  * that cannot allocate device memory -- code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  Same arguments and result

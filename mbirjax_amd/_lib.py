@@ -36,6 +36,7 @@ EXPORTS = (
     "mbx_dot_workspace_bytes", "mbx_dot", "mbx_sino_axpy", "mbx_vcd_workspace_bytes", "mbx_vcd_subset_update",
     "mbx_denoise_workspace_bytes", "mbx_denoise_subset_update", "mbx_median_filter3d",
     "mbx_noalloc_scratch_bytes", "mbx_back_project_noalloc", "mbx_vcd_subset_update_planned",
+    "mbx_back_apply_workspace_bytes", "mbx_back_apply",
 )
 
 
@@ -108,6 +109,8 @@ def lib():
             "mbx_median_filter3d": (i32, [vp, i32, i32, i32, i32, vp, vp]),
             "mbx_noalloc_scratch_bytes": (i64, []),
             "mbx_back_project_noalloc": (i32, [G, vp, vp, i32, i32, vp, vp, i32, vp, i32, i32, f32, i32, vp, vp]),
+            "mbx_back_apply_workspace_bytes": (i64, [G, i32, i32]),
+            "mbx_back_apply": (i32, [G, vp, i64, vp, i32, i32, vp, i32, vp, i32, i32, f32, i32, vp, vp, i64, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

@@ -452,6 +452,53 @@ int mbx_back_project(const mbx_geom *g, const float *angles, const void *sino, i
                        out, workspace, (hipStream_t)stream);
 }
 
+int64_t mbx_back_apply_workspace_bytes(const mbx_geom *g, int32_t npix, int32_t nviews) {
+    DevGeom d;
+    if (make_dev_geom(g, &d) != MBX_OK || npix < 0 || nviews < 0) return -1;
+    if (npix == 0 || nviews == 0) return 0;
+    return back_part_bytes(d, table_chunk(d, npix, nviews), nviews);
+}
+
+// mbx_back_project of a planned pixel set: the geometry table is the plan's.
+int mbx_back_apply(const mbx_geom *g, const void *plan, int64_t plan_bytes, const void *sino, int32_t sino_dtype,
+                   int32_t num_sino_views, const float *weights, int32_t npix, const int32_t *views, int32_t nviews,
+                   int32_t coeff_power, float scale, int32_t accumulate, float *out, void *workspace,
+                   int64_t workspace_bytes, void *stream) {
+    DevGeom d;
+    int rc = make_dev_geom(g, &d);
+    if (rc != MBX_OK) return rc;
+    if (npix < 0) return fail(MBX_EINVAL, "npix must be >= 0");
+    if (!dtype_ok(sino_dtype)) return fail(MBX_EINVAL, "unknown sinogram dtype");
+    if (coeff_power != 1 && coeff_power != 2) return fail(MBX_EINVAL, "coeff_power must be 1 or 2");
+    if (!views) {
+        if (num_sino_views != d.V)
+            return fail(MBX_EINVAL, "without view indices the sinogram must hold all " + std::to_string(d.V) +
+                                        " views (got " + std::to_string(num_sino_views) + ")");
+        nviews = d.V;
+    }
+    if (nviews < 0) return fail(MBX_EINVAL, "nviews must be >= 0");
+    const int chunk = table_chunk(d, std::max(npix, 1), std::max(nviews, 1));
+    if (npix > chunk) return fail(MBX_EUNSUPPORTED, "back apply: the pixel set exceeds one table chunk");
+    if (npix > 0 && (!plan || !sino || !out)) return fail(MBX_EINVAL, "NULL device pointer (plan, sino or out)");
+    // as in mbx_forward_apply: the plan must be the one of npix pixels and nviews views
+    const int64_t plan_need = fwd_workspace_bytes(d, npix, nviews);
+    if (plan_bytes != plan_need)
+        return fail(MBX_EINVAL, "back apply: plan_bytes " + std::to_string(plan_bytes) + " is not the size " +
+                                    std::to_string(plan_need) + " of a plan of " + std::to_string(npix) +
+                                    " pixels and " + std::to_string(nviews) + " views");
+    const int64_t need = mbx_back_apply_workspace_bytes(g, npix, nviews);
+    if (workspace_bytes < need || (need > 0 && !workspace))
+        return fail(MBX_EWORKSPACE, "back apply workspace too small: need " + std::to_string(need) + " bytes");
+    hipStream_t st = (hipStream_t)stream;
+    if (npix == 0) return MBX_OK;
+    if (nviews == 0) {
+        if (!accumulate) MBX_HIP_CHECK(hipMemsetAsync(out, 0, (size_t)npix * d.Nz * sizeof(float), st));
+        return MBX_OK;
+    }
+    return launch_back_tab(d, (const float *)plan, npix, sino, sino_dtype, weights, views, nviews, coeff_power, scale,
+                           accumulate, out, (float *)workspace, st, nullptr, table_chunk(d, npix, nviews));
+}
+
 int mbx_sino_transpose(const void *src, int32_t src_dtype, int32_t src_layout, void *dst, int32_t dst_dtype,
                        int32_t dst_layout, int32_t num_views, int32_t num_rows, int32_t num_channels, void *stream) {
     if (!dtype_ok(src_dtype) || !dtype_ok(dst_dtype)) return fail(MBX_EINVAL, "unknown dtype");

@@ -244,6 +244,25 @@ class TomographyModel(ParameterHandler):
                                                 layout, int(accumulate), None, None, _lib.stream()))
         return out
 
+    def _back_apply(self, plan, sino_vcr, views=None, coeff_power=1, weights=None, scale=1.0, out=None,
+                    accumulate=False):
+        """Back projection of a channel-major sinogram onto the pixel set of a
+        plan from _fwd_plan (its geometry table reused): _back without the table."""
+        nz = self.get_params('recon_shape')[2]
+        V = self.get_params('sinogram_shape')[0]
+        nv = V if views is None else int(views.numel())
+        if nv != plan.nviews:
+            raise ValueError('plan of {} views applied to {} views'.format(plan.nviews, nv))
+        if out is None:
+            out = torch.empty((plan.npix, nz), dtype=torch.float32, device=sino_vcr.device)
+        wsb = int(_lib.lib().mbx_back_apply_workspace_bytes(self._geom, plan.npix, nv))
+        ws = torch.empty(wsb, dtype=torch.uint8, device=sino_vcr.device) if wsb > 0 else None
+        _lib.check(_lib.lib().mbx_back_apply(
+            self._geom, _lib.ptr(plan.buf), plan.nbytes, _lib.ptr(sino_vcr), _dtype_code(sino_vcr),
+            int(sino_vcr.shape[0]), _lib.ptr(weights), plan.npix, _lib.ptr(views), nv, int(coeff_power),
+            float(scale), int(accumulate), _lib.ptr(out), _lib.ptr(ws), wsb, _lib.stream()))
+        return out
+
     def _fwd_rows(self, vox, pix, views=None):
         """(V', R, C) float32 projection: the channel-major forward, then one
         transpose (bitwise the same sums; the channel-major kernel's sliding

@@ -91,6 +91,27 @@ def test_validation_before_any_hip_call():
     assert L.mbx_forward_apply(geom(), fake, nb4, fake, 40, None, 0, fake, 0, 0, 0, None, None,
                                None) == _lib.MBX_EINVAL
     assert b"plan_bytes" in L.mbx_last_error()
+    # back apply of a plan: the same checks as mbx_back_project, the plan size
+    # as mbx_forward_apply, a partials workspace
+    wb4 = L.mbx_back_apply_workspace_bytes(geom(), 4, 8)
+    assert wb4 >= 0 and L.mbx_back_apply_workspace_bytes(geom(), -1, 8) == -1
+    assert L.mbx_back_apply(geom(), fake, nb4, fake, 5, 8, None, 4, None, 0, 1, 1.0, 0, fake, fake, wb4,
+                            None) == _lib.MBX_EINVAL  # dtype
+    assert L.mbx_back_apply(geom(), fake, nb4, fake, 0, 8, None, 4, None, 0, 3, 1.0, 0, fake, fake, wb4,
+                            None) == _lib.MBX_EINVAL  # coeff_power
+    assert L.mbx_back_apply(geom(), fake, nb4, fake, 0, 7, None, 4, None, 0, 1, 1.0, 0, fake, fake, wb4,
+                            None) == _lib.MBX_EINVAL  # sinogram views without view indices
+    assert L.mbx_back_apply(geom(), None, nb4, fake, 0, 8, None, 4, None, 0, 1, 1.0, 0, fake, fake, wb4,
+                            None) == _lib.MBX_EINVAL  # NULL plan
+    assert L.mbx_back_apply(geom(), fake, nb4, fake, 0, 8, None, 40, None, 0, 1, 1.0, 0, fake, fake, 1 << 30,
+                            None) == _lib.MBX_EINVAL  # more pixels than planned
+    assert b"plan_bytes" in L.mbx_last_error()
+    assert L.mbx_back_apply(big, fake, 1 << 40, fake, 0, 720, None, chunk_px, None, 0, 1, 1.0, 0, fake, fake,
+                            1 << 40, None) == _lib.MBX_EUNSUPPORTED
+    wb40 = L.mbx_back_apply_workspace_bytes(geom(), 40, 8)
+    if wb40 > 0:
+        assert L.mbx_back_apply(geom(), fake, nb40, fake, 0, 8, None, 40, None, 0, 1, 1.0, 0, fake, fake, wb40 - 1,
+                                None) == _lib.MBX_EWORKSPACE
     # transpose aliasing, dot and axpy arguments
     assert L.mbx_sino_transpose(fake, 0, 0, fake, 0, 1, 2, 3, 4, None) == _lib.MBX_EINVAL
     assert L.mbx_dot(fake, 0, None, 0, None, -1, fake, fake, 8192, None) == _lib.MBX_EINVAL
