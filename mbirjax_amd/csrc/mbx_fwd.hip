@@ -69,6 +69,11 @@ FwdPlan plan_fwd(const DevGeom &g, int nviews, int layout) {
         if (fit < budget && fit >= 4 * slot_bytes) budget = fit;
     }
     p.nb = (int)std::max<size_t>(1, std::min<size_t>(kFwdMaxBatch, budget / slot_bytes));
+    // fan geometries with spans of at most 512 slices and blocks of two or
+    // more waves stage through registers (fwd_reg_staging), two slots per wave
+    // at most (single-wave blocks keep their larger batches)
+    if (layout == MBX_LAYOUT_VCR && is_fan(g) && p.nw >= 2 && p.cstride <= 2 * 4 * kWave + 2 * kFwdPad + 8)
+        p.nb = std::min(p.nb, 2 * p.nw);
     p.lds = 2 * (size_t)p.nb * p.cstride * sizeof(float) + 64 * sizeof(int);
     return p;
 }
@@ -76,9 +81,10 @@ FwdPlan plan_fwd(const DevGeom &g, int nviews, int layout) {
 static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // The window kernel stages each batch through registers (two 256-slice chunks
-// per slot, at most two slots per wave) when the batch and the spans fit.
+// per slot, at most two slots per wave) when the batch and the spans fit
+// (plan_fwd caps the batch at two slots per wave for fan geometries).
 static bool fwd_reg_staging(const FwdPlan &p) {
-    return p.rpl == kFwdRPLW && p.nb <= 2 * p.nw && p.cstride <= 2 * 4 * kWave + 2 * kFwdPad + 8;
+    return p.nw >= 2 && p.nb <= 2 * p.nw && p.cstride <= 2 * 4 * kWave + 2 * kFwdPad + 8;
 }
 // FP64 partial slots of the squared norm: one per block of either output layout
 static int64_t fwd_part_blocks(const DevGeom &g, int nviews) {
@@ -538,7 +544,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                         const float x = sreg[t][c][j] * __builtin_amdgcn_sqrtf(fmaf(uu, uu, 1.0f));
                         y[j] = c * 4 * kWave + 4 * lane + j < span ? x : 0.0f;  // zero past the span
                     }
-                    *(mbx_f32x4 *)(cyl + P + c * 4 * kWave + 4 * lane) = y;  // 16-byte aligned
+                    // only lanes holding slices of the span store (the zero pad past it
+                    // follows; no tap reads beyond the pad), so the slot's stride is
+                    // never exceeded however short the span
+                    if (c * 4 * kWave + 4 * lane < span)
+                        *(mbx_f32x4 *)(cyl + P + c * 4 * kWave + 4 * lane) = y;  // 16-byte aligned
                 }
                 if (lane < P) {
                     cyl[lane] = 0.0f;
@@ -886,6 +896,14 @@ static void launch_fwd_t(const DevGeom &g, const FwdPlan &pl, const float *T, co
     // rows per lane of the window kernel: the plan's (the row-major kernel has its own)
     if constexpr (LAYOUT == MBX_LAYOUT_VCR) {
         if (pl.rpl == 2) {
+            if constexpr (GEOM == MBX_GEOM_CONE) {
+                if (fwd_reg_staging(pl)) {
+                    hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, VM, OT, LAYOUT, 2, true>), dim3((unsigned)pl.nblocks),
+                                       dim3(64 * pl.nw), pl.lds, st, g, T, FR, list, cstart, vox, vstride, npix, pl.ncb,
+                                       pl.nrb, pl.cb, pl.nb, pl.cstride, sino, accumulate, w, part);
+                    return;
+                }
+            }
             hipLaunchKernelGGL((fwd_kernel<GEOM, RAD, RB, VM, OT, LAYOUT, 2, false>), dim3((unsigned)pl.nblocks),
                                dim3(64 * pl.nw), pl.lds, st, g, T, FR, list, cstart, vox, vstride, npix, pl.ncb,
                                pl.nrb, pl.cb, pl.nb, pl.cstride, sino, accumulate, w, part);
