@@ -138,8 +138,20 @@ __device__ __forceinline__ void back_vertical(const float *col, int shift, int n
 constexpr int kBackUnit = 1;  // the unit-sum row weights apply (every W <= 2, radius-1 row taps)
 constexpr int kBackVec4 = 2;  // R % 4 == 0, sinogram aligned to 4 elements: 4-row column loads
 
+// One (view slot, pixel) record of the geometry table, read whole with one
+// scalar load so that the next one can be in flight while this one is used.
+typedef float BackRec __attribute__((ext_vector_type(kRec)));
+
+// Marks every word of a record as read at this point.  A scalar load writes
+// all 16 registers when it lands; words the kernel never reads would
+// otherwise be reused while the load is in flight, and the write-after-write
+// hazard costs a full wait right after the load is issued.
+__device__ __forceinline__ void rec_touch(const BackRec &r) {
+    asm volatile("" ::"s"(r[0]), "s"(r[1]), "s"(r[5]), "s"(r[11]), "s"(r[15]));
+}
+
 template <int GEOM, int KPL, int RAD, typename ET, int POW, bool WGT, bool TAIL>
-__device__ __forceinline__ void back_pixel_view(const DevGeom &g, const float *__restrict__ rec,
+__device__ __forceinline__ void back_pixel_view(const DevGeom &g, const BackRec &rec,
                                                 const ET *__restrict__ sino, const float *__restrict__ w, int vd,
                                                 int k0, int kend, int cap, int flags, float *col, int lane,
                                                 float (&acc)[KPL]) {
@@ -306,15 +318,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
     float accA[KPL], accB[KPL];
 #pragma unroll
     for (int i = 0; i < KPL; ++i) accA[i] = accB[i] = 0.0f;
+    // Records are loaded one pixel-view ahead: B's while A is projected, the
+    // next view's A while B is (vb < nviews, so the first load is in range).
+    const size_t vstride = (size_t)npix * kRec;
+    const float *TA = T + (size_t)sA * kRec, *TB = T + (size_t)sB * kRec;
+    BackRec recA = *(const BackRec *)(TA + vb * vstride);
+    rec_touch(recA);  // landed before the loop
     for (int vi = vb; vi < ve; ++vi) {
         const int vd = views ? views[vi] : vi;
-        const float *recA = T + ((size_t)vi * npix + sA) * kRec;
-        const float *recB = T + ((size_t)vi * npix + sB) * kRec;
+        const BackRec recB = *(const BackRec *)(TB + vi * vstride);
         back_pixel_view<GEOM, KPL, RAD, ET, POW, WGT, TAIL>(g, recA, sino, w, vd, k0, kend, cap, flags, colA, lane,
                                                             accA);
+        rec_touch(recB);
+        const BackRec recA1 = *(const BackRec *)(TA + min(vi + 1, ve - 1) * vstride);
         if (hasB)
             back_pixel_view<GEOM, KPL, RAD, ET, POW, WGT, TAIL>(g, recB, sino, w, vd, k0, kend, cap, flags, colB, lane,
                                                                 accB);
+        // The next A record has landed long since (B's projection waited on
+        // the shared counter); waiting for it here, on every path, keeps the
+        // use of the record at the loop top from also waiting for B's next
+        // load (scalar loads return out of order: any wait is a full wait).
+        rec_touch(recA1);
+        recA = recA1;
     }
 #pragma unroll
     for (int i = 0; i < KPL; ++i) {
