@@ -150,6 +150,60 @@ __device__ __forceinline__ void rec_touch(const BackRec &r) {
     asm volatile("" ::"s"(r[0]), "s"(r[1]), "s"(r[5]), "s"(r[11]), "s"(r[15]));
 }
 
+// Detector column col[i] = sum_j wc[j] * sinogram(column j, row mstart + i)
+// for the NC channel columns at byte offsets soff[j] of the view (rs; rw the
+// weights' view): whole 256-row groups 4 rows per lane (v4), the rest in
+// 64-row chunks (loads of up to kBackFill chunks in flight).  Rows are clamped
+// to the detector for the loads; the caller zeroes rows off it.
+template <int NC, typename ET, bool WGT>
+__device__ __forceinline__ void gather_column(const DevGeom &g, i32x4_t rs, i32x4_t rw, const int (&soff)[NC],
+                                              const float (&wc)[NC], bool v4, int mstart, int n_rows, float *col,
+                                              int lane) {
+    const int ng = v4 ? n_rows >> 8 : 0;
+    for (int gq = 0; gq < ng; ++gq) {
+        const int r = min(max(mstart + gq * 4 * kWave + 4 * lane, 0), g.R - 4);
+        mbx_f32x4 ev[NC];
+#pragma unroll
+        for (int j = 0; j < NC; ++j) ev[j] = buf_load4<ET>(rs, r * (int)sizeof(ET), soff[j]);
+        mbx_f32x4 c;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float cq = 0.0f;
+#pragma unroll
+            for (int j = 0; j < NC; ++j) cq = fmaf(wc[j], ev[j][q], cq);
+            c[q] = cq;
+        }
+        *(mbx_f32x4 *)(col + gq * 4 * kWave + 4 * lane) = c;  // 16-byte aligned
+    }
+    constexpr int FILL = kBackFill;
+    const int nch = (n_rows + kWave - 1) / kWave;
+    for (int c0 = 4 * ng; c0 < nch; c0 += FILL) {
+        float ev[FILL][NC];
+#pragma unroll
+        for (int u = 0; u < FILL; ++u) {
+            if (c0 + u < nch) {  // wave-uniform
+                const int r = min(max(mstart + (c0 + u) * kWave + lane, 0), g.R - 1);
+                const int vo = r * (int)sizeof(ET);
+#pragma unroll
+                for (int j = 0; j < NC; ++j) {
+                    float e = buf_load<ET>(rs, vo, soff[j]);
+                    if (WGT) e *= buf_load<float>(rw, r * 4, soff[j] * (int)(4 / sizeof(ET)));
+                    ev[u][j] = e;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < FILL; ++u) {
+            if (c0 + u < nch) {
+                float c = 0.0f;
+#pragma unroll
+                for (int j = 0; j < NC; ++j) c = fmaf(wc[j], ev[u][j], c);
+                col[(c0 + u) * kWave + lane] = c;  // cap holds whole chunks
+            }
+        }
+    }
+}
+
 template <int GEOM, int KPL, int RAD, typename ET, int POW, bool WGT, bool TAIL>
 __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const BackRec &rec,
                                                 const ET *__restrict__ sino, const float *__restrict__ w, int vd,
@@ -176,8 +230,13 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const BackRec 
         const float hv = rec[9], Lv = rec[10], c0 = rec[12], c1 = rec[13];
         const int ai = __float_as_int(rec[14]);
         const double Wd = (double)rec[7] + (double)rec[8], acd = (double)rec[6];
-        const int mfirst = ai + (int)rint(fma(Wd, (double)(k0 - g.kref), acd)) - RAD;
-        const int mlast = ai + (int)rint(fma(Wd, (double)(kend - 1 - g.kref), acd)) + RAD;
+        // First and last rows the chunk's slices reach (a record of radius 1
+        // holds them for the whole slice range)
+        const bool whole = RAD == 1 && k0 == 0 && kend == g.Nz;  // wave-uniform
+        const int mfirst = whole ? __float_as_int(rec[1])
+                                 : ai + (int)rint(fma(Wd, (double)(k0 - g.kref), acd)) - RAD;
+        const int mlast = whole ? __float_as_int(rec[5])
+                                : ai + (int)rint(fma(Wd, (double)(kend - 1 - g.kref), acd)) + RAD;
         const int mlo = max(mfirst, 0), mhi = min(mlast, g.R - 1);
         if (mlo > mhi) return;  // wave-uniform: this view misses the detector
         // One zero row of margin on each side: a lane's slice position may
@@ -195,52 +254,20 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const BackRec 
             n_rows += mstart & 3;
             mstart &= ~3;
         }
-        // ---- detector column col[i] = row mstart + i: whole 256-row groups
-        // 4 rows per lane (v4), the rest in 64-row chunks (loads of up to
-        // kFill chunks in flight).  Rows are clamped to the detector for the
-        // loads; rows off it are zeroed afterwards.
-        const int ng = v4 ? n_rows >> 8 : 0;
-        for (int gq = 0; gq < ng; ++gq) {
-            const int r = min(max(mstart + gq * 4 * kWave + 4 * lane, 0), g.R - 4);
-            mbx_f32x4 ev[2 * RAD + 1];
+        // ---- detector column col[i] = row mstart + i.  With radius-1 taps a
+        // pixel rarely covers three channels: when an edge column has zero
+        // weight (or lies off the detector) only the other two are gathered
+        // (the same sums: the dropped term adds 0 * e).
+        float wc[2 * RAD + 1];
 #pragma unroll
-            for (int j = 0; j <= 2 * RAD; ++j) ev[j] = buf_load4<ET>(src.rs, r * (int)sizeof(ET), src.soff[j]);
-            mbx_f32x4 c;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                float cq = 0.0f;
-#pragma unroll
-                for (int j = 0; j <= 2 * RAD; ++j) cq = fmaf(src.cv[j] ? Aw[j] : 0.0f, ev[j][q], cq);
-                c[q] = cq;
-            }
-            *(mbx_f32x4 *)(col + gq * 4 * kWave + 4 * lane) = c;  // 16-byte aligned
-        }
-        constexpr int FILL = kBackFill;
-        const int nch = (n_rows + kWave - 1) / kWave;
-        for (int c0 = 4 * ng; c0 < nch; c0 += FILL) {
-            float ev[FILL][2 * RAD + 1];
-#pragma unroll
-            for (int u = 0; u < FILL; ++u) {
-                if (c0 + u < nch) {  // wave-uniform
-                    const int r = min(max(mstart + (c0 + u) * kWave + lane, 0), g.R - 1);
-                    const int vo = r * (int)sizeof(ET);
-#pragma unroll
-                    for (int j = 0; j <= 2 * RAD; ++j) {
-                        float e = buf_load<ET>(src.rs, vo, src.soff[j]);
-                        if (WGT) e *= buf_load<float>(src.rw, r * 4, src.soff[j] * (int)(4 / sizeof(ET)));
-                        ev[u][j] = e;
-                    }
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < FILL; ++u) {
-                if (c0 + u < nch) {
-                    float c = 0.0f;
-#pragma unroll
-                    for (int j = 0; j <= 2 * RAD; ++j) c = fmaf(src.cv[j] ? Aw[j] : 0.0f, ev[u][j], c);
-                    col[(c0 + u) * kWave + lane] = c;  // cap holds whole chunks
-                }
-            }
+        for (int j = 0; j <= 2 * RAD; ++j) wc[j] = src.cv[j] ? Aw[j] : 0.0f;
+        if (RAD == 1 && (wc[0] == 0.0f || wc[2 * RAD] == 0.0f)) {  // wave-uniform
+            const int o = wc[0] == 0.0f ? 1 : 0;
+            const int so2[2] = {src.soff[o], src.soff[o + 1]};
+            const float wc2[2] = {wc[o], wc[o + 1]};
+            gather_column<2, ET, WGT>(g, src.rs, src.rw, so2, wc2, v4, mstart, n_rows, col, lane);
+        } else {
+            gather_column<2 * RAD + 1, ET, WGT>(g, src.rs, src.rw, src.soff, wc, v4, mstart, n_rows, col, lane);
         }
         if (mstart < 0)  // rows above the detector
             for (int i = lane; i < -mstart; i += kWave) col[i] = 0.0f;

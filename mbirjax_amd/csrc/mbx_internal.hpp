@@ -44,7 +44,7 @@ inline bool dtype_fwd_ok(int dt) { return dt == MBX_DTYPE_F32 || dt == MBX_DTYPE
 // kRec floats per (view slot, pixel) (layout in mbx_project.hip).
 constexpr int kRec = 16;
 constexpr int kFill = 4;  // 64-element groups loaded per round (forward staging)
-constexpr int kBackFill = 3;  // 64-row chunks loaded per round (back projection)
+constexpr int kBackFill = 2;  // 64-row chunks loaded per round (back projection)
 constexpr int64_t kTableBudget = 512ll << 20;
 // Rows [first, last] packed in record word 15.
 __device__ __forceinline__ int rec_mlo(float w) { return __float_as_int(w) & 0xffff; }

@@ -225,6 +225,14 @@ __global__ __launch_bounds__(256) void pv_table_kernel(DevGeom g, const float *_
         const double t0 = tref - W * g.kref, t1 = tref + W * (g.Nz - 1 - g.kref);
         mlo = max(0, (int)rint(t0) - RAD);
         mhi = min(g.R - 1, (int)rint(t1) + RAD);
+        if (RAD == 1) {
+            // The rows the back projection of all Nz slices reaches, formed
+            // exactly as the back projector forms them from this record (the
+            // +-2 tap words are free at radius 1).
+            const double Wd = (double)r[7] + (double)r[8], acd = (double)r[6];
+            r[1] = __int_as_float(ai + (int)rint(fma(Wd, (double)(-g.kref), acd)) - RAD);
+            r[5] = __int_as_float(ai + (int)rint(fma(Wd, (double)(g.Nz - 1 - g.kref), acd)) + RAD);
+        }
     } else {
         r[6] = 0.0f;
         r[7] = 1.0f;
