@@ -21,12 +21,11 @@ namespace mbx {
 // views together and each view is fetched from HBM once.
 // The 2r+1 channel columns of one pixel at one view: a buffer resource on the
 // view (and on its weights), the byte offset of each column (channels off the
-// detector are clamped to a valid column and get weight 0).
+// detector are clamped to a valid column; their table weights are 0).
 template <int RAD>
 struct ColumnSource {
     i32x4_t rs, rw;
     int soff[2 * RAD + 1];
-    bool cv[2 * RAD + 1];
 };
 
 // Vertical fan of one pixel at one view (mbirjax/cone_beam.py:480-541): every
@@ -222,9 +221,8 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const BackRec 
     if (WGT) src.rw = make_rsrc(w + vofs);
 #pragma unroll
     for (int j = 0; j <= 2 * RAD; ++j) {
-        const int n = nc - RAD + j;
-        src.cv[j] = (n >= 0) && (n < g.C);
-        src.soff[j] = (src.cv[j] ? n : 0) * g.R * (int)sizeof(ET);
+        const int n = min(max(nc - RAD + j, 0), g.C - 1);
+        src.soff[j] = n * g.R * (int)sizeof(ET);
     }
     if (GEOM == MBX_GEOM_CONE) {
         const float hv = rec[9], Lv = rec[10], c0 = rec[12], c1 = rec[13];
@@ -258,16 +256,13 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const BackRec 
         // pixel rarely covers three channels: when an edge column has zero
         // weight (or lies off the detector) only the other two are gathered
         // (the same sums: the dropped term adds 0 * e).
-        float wc[2 * RAD + 1];
-#pragma unroll
-        for (int j = 0; j <= 2 * RAD; ++j) wc[j] = src.cv[j] ? Aw[j] : 0.0f;
-        if (RAD == 1 && (wc[0] == 0.0f || wc[2 * RAD] == 0.0f)) {  // wave-uniform
-            const int o = wc[0] == 0.0f ? 1 : 0;
+        if (RAD == 1 && (Aw[0] == 0.0f || Aw[2 * RAD] == 0.0f)) {  // wave-uniform
+            const int o = Aw[0] == 0.0f ? 1 : 0;
             const int so2[2] = {src.soff[o], src.soff[o + 1]};
-            const float wc2[2] = {wc[o], wc[o + 1]};
+            const float wc2[2] = {Aw[o], Aw[o + 1]};
             gather_column<2, ET, WGT>(g, src.rs, src.rw, so2, wc2, v4, mstart, n_rows, col, lane);
         } else {
-            gather_column<2 * RAD + 1, ET, WGT>(g, src.rs, src.rw, src.soff, wc, v4, mstart, n_rows, col, lane);
+            gather_column<2 * RAD + 1, ET, WGT>(g, src.rs, src.rw, src.soff, Aw, v4, mstart, n_rows, col, lane);
         }
         if (mstart < 0)  // rows above the detector
             for (int i = lane; i < -mstart; i += kWave) col[i] = 0.0f;
@@ -309,7 +304,7 @@ __device__ __forceinline__ void back_pixel_view(const DevGeom &g, const BackRec 
             for (int j = 0; j <= 2 * RAD; ++j) {
                 float e = buf_load<ET>(src.rs, k * (int)sizeof(ET), src.soff[j]);
                 if (WGT) e *= buf_load<float>(src.rw, k * 4, src.soff[j] * (int)(4 / sizeof(ET)));
-                c = fmaf(src.cv[j] ? Aw[j] : 0.0f, e, c);
+                c = fmaf(Aw[j], e, c);
             }
             acc[i] += c;
         }
