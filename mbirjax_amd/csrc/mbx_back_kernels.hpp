@@ -13,10 +13,11 @@
 namespace mbx {
 
 // One wavefront per (pixel pair, slice chunk, view group).  For every view the
-// wave reads the two pixels' records with scalar loads, gathers each pixel's
-// 2r+1 channel columns row-by-row into its LDS detector column (clamped,
-// always-valid addresses; rows off the detector are zeroed by a select), then
-// every lane resamples both columns onto its slices.  Two pixels per wave keep
+// wave reads the two pixels' records with scalar loads (one pixel-view
+// ahead), gathers each pixel's weighted channel columns (2r+1, or the two with
+// non-zero weight at radius 1) row-by-row into its LDS detector column
+// (clamped, always-valid addresses; rows off the detector are zeroed
+// afterwards), then every lane resamples both columns onto its slices.  Two pixels per wave keep
 // all waves of a subset resident at once, so the waves of an XCD sweep the
 // views together and each view is fetched from HBM once.
 // The 2r+1 channel columns of one pixel at one view: a buffer resource on the

@@ -109,7 +109,9 @@ int make_dev_geom(const mbx_geom *g, DevGeom *d) {
 // and read by both projectors with scalar loads, so both directions use
 // bit-identical weights:
 //   [0] n_c = rint(n_p) (int bits)    [1..5] Ac(n_c-2 .. n_c+2), zero outside
-//   the tap radius and outside [0, C)
+//   the tap radius and outside [0, C); at radius 1 (cone and translation)
+//   [1] and [5] hold instead the first and last rows (int bits) that the
+//   back projection of all Nz slices reaches, taps included
 //   vertical fan, relative to the reference slice kref = Nz/2 and the integer
 //   row ai = rint(t_kref):  t_k - ai = acr + (W + W_lo) * (k - kref)
 //   [6] acr   [7] W (rounded)   [8] W_lo = W - float(W)   [9] (W+1)/2
