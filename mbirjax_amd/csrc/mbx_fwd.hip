@@ -541,12 +541,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const float uu = fmaf(du, cf, w4[j]);
-                        const float x = sreg[t][c][j] * __builtin_amdgcn_sqrtf(fmaf(uu, uu, 1.0f));
-                        y[j] = c * 4 * kWave + 4 * lane + j < span ? x : 0.0f;  // zero past the span
+                        y[j] = sreg[t][c][j] * __builtin_amdgcn_sqrtf(fmaf(uu, uu, 1.0f));
                     }
-                    // only lanes holding slices of the span store (the zero pad past it
-                    // follows; no tap reads beyond the pad), so the slot's stride is
-                    // never exceeded however short the span
+                    // only lanes holding slices of the span store, so the slot's stride
+                    // is never exceeded however short the span; the up to 3 slices past
+                    // the span in the last lane's group are overwritten by the zero pad
+                    // stored below (P >= 3, the same wave's later LDS store), and no
+                    // tap reads beyond the pad
+                    static_assert(kFwdPad >= 3, "the pad must cover a 4-slice group's overshoot");
                     if (c * 4 * kWave + 4 * lane < span)
                         *(mbx_f32x4 *)(cyl + P + c * 4 * kWave + 4 * lane) = y;  // 16-byte aligned
                 }
