@@ -79,6 +79,7 @@ constexpr int kRedBlocks = 1024;
 struct QP {
     float b[6];
     float tsig;      // T*sigma_x
+    float inv_tsig;  // 1 / (T*sigma_x)
     float scale;     // T^(p-2) / sigma_x^2
     float qm2;       // q - 2
     float qmp;       // q - p
@@ -86,15 +87,19 @@ struct QP {
 };
 
 // get_2_b_tilde (mbirjax/qggmrf.py:250-279): b * rho'(delta)/delta.
+// sd >= eps32 is a normal number, so the hardware log2 needs no denormal
+// scaling, and sd^(q-2), sd^(q-p) (1 <= q <= p <= 2) are normal unless
+// |delta| / (T sigma_x) exceeds 2^126, so neither does the hardware exp2; the
+// quotients use the hardware reciprocal (1 ulp).
 __device__ __forceinline__ float two_b_tilde(float delta, float bval, const QP &q) {
-    const float sd = fabsf(delta) / q.tsig + kEps32;
-    const float l2 = log2f(sd);
-    const float d_q2 = exp2f(q.qm2 * l2);  // sd^(q-2)
-    const float d_qp = exp2f(q.qmp * l2);  // sd^(q-p)
+    const float sd = fabsf(delta) * q.inv_tsig + kEps32;
+    const float l2 = __builtin_amdgcn_logf(sd);
+    const float d_q2 = __builtin_amdgcn_exp2f(q.qm2 * l2);  // sd^(q-2)
+    const float d_qp = __builtin_amdgcn_exp2f(q.qmp * l2);  // sd^(q-p)
     const float num = d_q2 * (q.q_over_p + d_qp);
     const float den1 = 1.0f + d_qp;
     const float den = den1 * den1;
-    return bval * (q.scale * num / den);
+    return bval * (q.scale * num * __builtin_amdgcn_rcpf(den));
 }
 
 // Surrogate gradient and Hessian of voxel (pixel p, slice k) over its six
@@ -144,6 +149,7 @@ static QP make_qp(const mbx_qggmrf *P) {
     QP q;
     for (int t = 0; t < 6; ++t) q.b[t] = (float)P->b[t];
     q.tsig = (float)(P->T * P->sigma_x);
+    q.inv_tsig = (float)(1.0 / (P->T * P->sigma_x));
     q.scale = (float)(std::pow(P->T, P->p - 2.0) / (P->sigma_x * P->sigma_x));
     q.qm2 = (float)(P->q - 2.0);
     q.qmp = (float)(P->q - P->p);
@@ -330,7 +336,7 @@ __global__ __launch_bounds__(256) void delta_kernel(const float *__restrict__ bp
 __global__ __launch_bounds__(1024) void alpha_kernel(const double *__restrict__ part, int nblk,
                                                     const double *__restrict__ fqpart, int64_t nfq, double fmc,
                                                     double *__restrict__ scal, double *__restrict__ stats) {
-    __shared__ double scratch[16];  // one slot per wave of the 1024-thread block
+    __shared__ double scratch[16][4];  // one row per wave of the 1024-thread block
     double a = 0.0, b = 0.0, c = 0.0, f = 0.0;
     for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
         a += part[i];
@@ -339,11 +345,31 @@ __global__ __launch_bounds__(1024) void alpha_kernel(const double *__restrict__ 
     }
 #pragma unroll 8
     for (int64_t i = threadIdx.x; i < nfq; i += blockDim.x) f += fqpart[i];
-    const double pl = block_reduce<double>(a, scratch);
-    const double pq = 0.5 * block_reduce<double>(b, scratch);
-    const double fl = fmc * block_reduce<double>(c, scratch);
-    const double fq_raw = block_reduce<double>(f, scratch);
+    // the four sums reduced together (each in the order block_reduce uses:
+    // a shuffle tree per wave, then the waves in order)
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+        c += __shfl_down(c, off, 64);
+        f += __shfl_down(f, off, 64);
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        scratch[wave][0] = a;
+        scratch[wave][1] = b;
+        scratch[wave][2] = c;
+        scratch[wave][3] = f;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
+        double sa = 0.0, sb = 0.0, sc = 0.0, sf = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+            sa += scratch[i][0];
+            sb += scratch[i][1];
+            sc += scratch[i][2];
+            sf += scratch[i][3];
+        }
+        const double pl = sa, pq = 0.5 * sb, fl = fmc * sc, fq_raw = sf;
         scal[0] = fq_raw;
         const double fq = fmc * fq_raw;
         const double eps = (double)kEps32;
