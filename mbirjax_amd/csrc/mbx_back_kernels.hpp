@@ -318,7 +318,7 @@ template <int GEOM, int KPL, int RAD, typename ET, int POW, bool WGT, bool TAIL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void back_kernel(
     DevGeom g, const float *__restrict__ T, const ET *__restrict__ sino, const float *__restrict__ w, int npix,
     const int32_t *__restrict__ views, int nviews, int nsplit, int chunk0, int nchunk, int cap, int flags, float scale,
-    int accumulate, float *__restrict__ out, float *__restrict__ part) {
+    int accumulate, float *__restrict__ out, float *__restrict__ part, int npair) {
     constexpr int PPW = 2;  // pixels per wave
     extern __shared__ float lds[];
     const int wave = threadIdx.x >> 6;
@@ -327,9 +327,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
     const int grp = blockIdx.x / nsplit;
     const int chunk = chunk0 + grp % nchunk;
     const int q = __builtin_amdgcn_readfirstlane((grp / nchunk) * 4 + wave);
-    const int sA = PPW * q;
+    // waves below npair take two pixels, the rest one (see back_pairs)
+    const int sA = q < npair ? PPW * q : PPW * npair + (q - npair);
     if (sA >= npix) return;  // whole wavefront leaves; no block barriers below
-    const bool hasB = sA + 1 < npix;
+    const bool hasB = q < npair && sA + 1 < npix;
     const int sB = hasB ? sA + 1 : sA;
     const int k0 = chunk * kWave * KPL;
     const int kend = min(g.Nz, k0 + kWave * KPL);
@@ -385,19 +386,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
     }
 }
 
+// Waves of a view group that take two pixels (the others take one).  Two
+// pixels per wave, unless the subset has more pixels than one XCD holds waves
+// at once but fewer pairs, and the pairs leave at most half of the XCD's 128
+// SIMDs one wave more than the rest (so those SIMDs set the time): then every
+// wave slot of the XCD gets a wave (npix - slots waves take two pixels, the
+// rest one).  One view group per XCD; 7 waves per SIMD by registers, fewer
+// if the blocks' LDS columns do not fit 7 to a CU.  A one-pixel wave costs
+// more per pixel than a pair, so balanced pairs (a multiple of 128) or
+// pairs that load most SIMDs alike are kept.
+inline int back_pairs(const BackPlan &pl, int npix, size_t lds_block) {
+    const int pairs = (npix + 1) / 2;
+    if (pl.nsplit != kXcds || lds_block == 0) return pairs;
+    const int per_cu = (int)std::min<size_t>(7, kLdsPerCu / lds_block);  // 4-wave blocks per CU at once
+    const int simds = kCusPerXcd * 4, slots = simds * per_cu;
+    const int extra = pairs % simds;  // SIMDs holding one pair wave more than the others
+    return (npix > slots && pairs < slots && extra > 0 && 2 * extra <= simds) ? npix - slots : pairs;
+}
+
 template <int GEOM, int KPL, int RAD, typename ET, int POW, bool WGT, bool TAIL>
 static void launch_back_range(const DevGeom &g, const BackPlan &pl, const float *T, const ET *sino, const float *w,
                               int npix, const int32_t *views, int nviews, int chunk0, int nch, float scale,
                               int accumulate, float *out, float *part, hipStream_t st) {
     constexpr int PPW = 2;  // pixels per wave, as in back_kernel
-    const int64_t groups = ((int64_t)npix + 4 * PPW - 1) / (4 * PPW);  // 4 waves per block
-    const int64_t blocks = groups * nch * pl.nsplit;
     const size_t lds = GEOM == MBX_GEOM_CONE ? 4 * PPW * (size_t)pl.cap * sizeof(float) : 0;
+    const int npair = back_pairs(pl, npix, lds);
+    const int64_t nwaves = npair + std::max(0, npix - PPW * npair);
+    const int64_t groups = (nwaves + 3) / 4;  // 4 waves per block
+    const int64_t blocks = groups * nch * pl.nsplit;
     const bool vec4 = g.R % 4 == 0 && ((uintptr_t)sino & (4 * sizeof(ET) - 1)) == 0;
     const int flags = (pl.unit ? kBackUnit : 0) | (vec4 ? kBackVec4 : 0);
     hipLaunchKernelGGL((back_kernel<GEOM, KPL, RAD, ET, POW, WGT, TAIL>), dim3((unsigned)blocks), dim3(256), lds, st,
                        g, T, sino, w, npix, views, nviews, pl.nsplit, chunk0, nch, pl.cap, flags, scale, accumulate,
-                       out, part);
+                       out, part, npair);
 }
 
 // Full slice chunks in one launch, a partial last chunk in a second one.

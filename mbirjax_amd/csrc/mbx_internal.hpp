@@ -45,6 +45,11 @@ inline bool dtype_fwd_ok(int dt) { return dt == MBX_DTYPE_F32 || dt == MBX_DTYPE
 constexpr int kRec = 16;
 constexpr int kFill = 4;  // 64-element groups loaded per round (forward staging)
 constexpr int kBackFill = 2;  // 64-row chunks loaded per round (back projection)
+// MI355X (gfx950): 8 XCDs of 32 CUs, 160 KB of LDS per CU; consecutive
+// workgroups of a launch go to consecutive XCDs (blockIdx % 8)
+constexpr int kXcds = 8;
+constexpr int kCusPerXcd = 32;
+constexpr size_t kLdsPerCu = 160 * 1024;
 constexpr int64_t kTableBudget = 512ll << 20;
 // Rows [first, last] packed in record word 15.
 __device__ __forceinline__ int rec_mlo(float w) { return __float_as_int(w) & 0xffff; }
