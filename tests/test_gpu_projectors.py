@@ -171,6 +171,26 @@ def test_benchmark_c4_subset_parity(gpu):
     assert rb <= TOL
 
 
+def test_back_projection_wave_layouts(gpu):
+    """Subsets of 1024-1791 pixels over 64 or more views back-project in 8 view
+    groups with every wave slot of an XCD filled: waves of two pixels and of
+    one when the pairs would load some SIMDs one wave more than the rest
+    (1100, 1152 and 1400 pixels), pairs only when they are balanced or load
+    most SIMDs alike (1280, 1700).  Each against the FP64 oracle."""
+    m, g = make_cone(64, 64, 96, 400.0, 200.0, recon_shape=(48, 48, 64))
+    from mbirjax_amd import vcd_utils
+    ror = np.flatnonzero(vcd_utils.get_2d_ror_mask((48, 48, 64))).astype(np.int32)
+    assert ror.size >= 1700
+    rng = np.random.default_rng(5)
+    pix_all = np.sort(rng.permutation(ror)[:1700]).astype(np.int32)
+    y = rng.random((64, 64, 96)).astype(np.float32)
+    for n in (1100, 1152, 1280, 1400, 1700):
+        pix = pix_all[:n]
+        r = rel(m.sparse_back_project(y, pix), op.back(g, y, pix))
+        print("back projection of %d pixels rel-L2 %.3e" % (n, r), flush=True)
+        assert r <= TOL
+
+
 @pytest.mark.parametrize("case", ["cone_m2", "parallel", "translation"])
 def test_adjoint_fp32(gpu, case):
     m, g = CASES[case]()
