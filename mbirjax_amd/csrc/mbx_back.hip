@@ -1,6 +1,7 @@
 // Sparse back projector on gfx950 (Projectors.sparse_back_project,
 // mbirjax/projectors.py:131-209): one wavefront per (pixel pair, slice chunk,
-// view group); for every view the wave gathers each pixel's 2r+1 channel
+// view group), some waves with one pixel where that fills an XCD's wave slots
+// evenly; for every view the wave gathers each pixel's weighted channel
 // columns (contiguous rows in the channel-major sinogram) into an LDS detector
 // column, then every lane resamples it onto its slices.  View groups are
 // mapped to XCDs (blockIdx % 8) so each XCD's L2 streams only its share of the

@@ -12,14 +12,15 @@
 
 namespace mbx {
 
-// One wavefront per (pixel pair, slice chunk, view group).  For every view the
-// wave reads the two pixels' records with scalar loads (one pixel-view
-// ahead), gathers each pixel's weighted channel columns (2r+1, or the two with
-// non-zero weight at radius 1) row-by-row into its LDS detector column
-// (clamped, always-valid addresses; rows off the detector are zeroed
-// afterwards), then every lane resamples both columns onto its slices.  Two pixels per wave keep
-// all waves of a subset resident at once, so the waves of an XCD sweep the
-// views together and each view is fetched from HBM once.
+// One wavefront per (pixel pair, slice chunk, view group), or per pixel for
+// some waves (back_pairs).  For every view the wave reads the two pixels'
+// records with scalar loads (one pixel-view ahead), gathers each pixel's
+// weighted channel columns (2r+1, or the two with non-zero weight at radius
+// 1) row-by-row into its LDS detector column (clamped, always-valid
+// addresses; rows off the detector are zeroed afterwards), then every lane
+// resamples both columns onto its slices.  Two pixels per wave keep all waves
+// of a subset resident at once, so the waves of an XCD sweep the views
+// together and each view is fetched from HBM about once.
 // The 2r+1 channel columns of one pixel at one view: a buffer resource on the
 // view (and on its weights), the byte offset of each column (channels off the
 // detector are clamped to a valid column; their table weights are 0).
