@@ -86,12 +86,16 @@ static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 static bool fwd_reg_staging(const FwdPlan &p) {
     return p.nw >= 2 && p.nb <= 2 * p.nw && p.cstride <= 2 * 4 * kWave + 2 * kFwdPad + 8;
 }
-// FP64 partial slots of the squared norm: one per block of either output layout
+// FP64 partial slots of the squared norm: one per wave of either output layout
 static int64_t fwd_part_blocks(const DevGeom &g, int nviews) {
-    return std::max(plan_fwd(g, nviews, MBX_LAYOUT_VCR).nblocks, plan_fwd(g, nviews, MBX_LAYOUT_VRC).nblocks);
+    const FwdPlan a = plan_fwd(g, nviews, MBX_LAYOUT_VCR), b = plan_fwd(g, nviews, MBX_LAYOUT_VRC);
+    return std::max(a.nblocks * a.nw, b.nblocks * b.nw);
 }
 static int fwd_nbk(const DevGeom &g) { return g.C + 2 * g.rh; }  // centre-channel buckets n_c + r
-int64_t fwd_blocks(const DevGeom &g, int nviews, int layout) { return plan_fwd(g, nviews, layout).nblocks; }
+int64_t fwd_partials(const DevGeom &g, int nviews, int layout) {
+    const FwdPlan p = plan_fwd(g, nviews, layout);
+    return p.nblocks * p.nw;
+}
 
 // Forward record of one (view, pixel) pair, stored in the view's binned order
 // (centre channel, then pixel): what the forward update reads per candidate
@@ -385,7 +389,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     extern __shared__ float lds[];
     const int nw = blockDim.x >> 6;
     float *bufs = lds;                                                  // [2][nbatch][cstride]
-    double *dscratch = (double *)(lds + 2 * (size_t)nbatch * cstride);  // [8]
     // (window kernel) the batches' forward records: an LDS array of its own,
     // apart from the cylinders' asynchronous copies
     float *recs = nullptr;
@@ -885,9 +888,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             }
         }
     }
-    if (part_sumsq) {
-        const double r = block_sum<double>(ss, dscratch);
-        if (tid == 0) part_sumsq[b] = r;
+    if (part_sumsq) {  // one partial per wave (no block-wide reduction)
+        double r = ss;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) r += __shfl_down(r, off, 64);
+        if (lane == 0) part_sumsq[(size_t)b * nw + wave] = r;
     }
 }
 
@@ -1006,7 +1011,7 @@ int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float 
     }
 #undef MBX_FWD
     MBX_LAUNCH_CHECK();
-    if (sumsq) return launch_sum_partials(part, pl.nblocks, sumsq, 0, st);
+    if (sumsq) return launch_sum_partials(part, pl.nblocks * pl.nw, sumsq, 0, st);
     return MBX_OK;
 }
 

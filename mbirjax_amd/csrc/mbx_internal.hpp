@@ -121,10 +121,10 @@ int launch_back_tab(const DevGeom &g, const float *T, int npix, const void *sino
 // The cylinder of table slot s is vox + s * pstride * Nz.  steps: kFwdBin
 // (bins and forward records from the table), kFwdProject (the projection
 // from bins and records already in aux), or both; kFwdPartials: the
-// fwd_blocks() FP64 partials of the weighted squared norm are written to the
-// start of aux (and not reduced unless sumsq is given).
+// fwd_partials() FP64 partials (one per wave) of the weighted squared norm
+// are written to the start of aux (and not reduced unless sumsq is given).
 constexpr int kFwdBin = 1, kFwdProject = 2, kFwdPartials = 4;
-int64_t fwd_blocks(const DevGeom &g, int nviews, int layout);
+int64_t fwd_partials(const DevGeom &g, int nviews, int layout);
 int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float *vox, int npix, int nviews, void *sino,
                    int sino_dtype, int layout, int accumulate, const float *weights, void *aux, int chunk,
                    double *sumsq, hipStream_t st, int pstride = 1, int steps = kFwdBin | kFwdProject);

@@ -677,7 +677,7 @@ static int vcd_update(const mbx_geom *g, const float *angles, float *flat_recon,
     if (rc != MBX_OK) return rc;
     // (i) step size on the device
     hipLaunchKernelGGL(alpha_kernel, dim3(1), dim3(1024), 0, st, dpart, nb, (const double *)fpart,
-                       fwd_blocks(d, d.V, MBX_LAYOUT_VCR), fm_constant, scal, stats);
+                       fwd_partials(d, d.V, MBX_LAYOUT_VCR), fm_constant, scal, stats);
     MBX_LAUNCH_CHECK();
     // (j) positivity
     if (positivity) {
