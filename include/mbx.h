@@ -163,7 +163,7 @@ int mbx_forward_project(const mbx_geom *g, const float *angles, const float *vox
  * plan was built with, else MBX_EINVAL (so an apply of more pixels or views
  * than planned cannot read past the plan).  The plan stays valid while the
  * geometry, angles, pix and views are unchanged; applies on one stream may
- * share it (an apply that asks for sumsq writes the per-block FP64 partials
+ * share it (an apply that asks for sumsq writes the per-wave FP64 partials
  * into the plan's scratch slots, so applies must not overlap in time). */
 int mbx_forward_plan(const mbx_geom *g, const float *angles, const int32_t *pix, int32_t npix, const int32_t *views,
                      int32_t nviews, void *plan, int64_t plan_bytes, void *stream);
