@@ -22,7 +22,8 @@ namespace mbx {
 constexpr int kFwdCB = 16;        // channels per block, (views, rows, channels) output: the per-lane register tile
 constexpr int kFwdCBW = 32;       // channels per block, (views, channels, rows) output: the sliding window's sweep
 constexpr int kFwdCBWMin = 8;     // ... narrowed down to this on small launches
-constexpr int64_t kFwdMinWaves = 4096;  // four waves per SIMD
+constexpr int64_t kFwdMinWaves = 16384;  // channel blocks narrow below this many waves (16 per SIMD)
+constexpr int64_t kFwdFewWaves = 4096;   // four waves per SIMD: a "few-wave" launch (its staging batch)
 constexpr int kFwdRPL = 2;        // detector rows per lane (64 apart), (views, rows, channels) output
 constexpr int kFwdRPLW = 4;       // detector rows per lane, (views, channels, rows) output (2 on
                                   // detectors of at most 256 rows)
@@ -58,11 +59,11 @@ FwdPlan plan_fwd(const DevGeom &g, int nviews, int layout) {
     p.cstride = ((span_max + kWave - 1) / kWave) * kWave + 2 * kFwdPad + 8;  // a multiple of 4
     p.nblocks = (int64_t)nviews * p.ncb * p.nrb;
     size_t budget = layout == MBX_LAYOUT_VCR ? kFwdLdsW : kFwdLds;
-    // Launches of few waves (at most two rounds of kFwdMinWaves): a smaller
+    // Launches of few waves (at most two rounds of kFwdFewWaves): a smaller
     // staging batch if that lets every block be resident at once (160 KB of
     // LDS per CU, 1 KB of allocation slack per block), down to 4 slots.
     const size_t slot_bytes = 2 * (size_t)p.cstride * sizeof(float);
-    if (p.nblocks * p.nw <= 2 * kFwdMinWaves) {
+    if (p.nblocks * p.nw <= 2 * kFwdFewWaves) {
         const int64_t per_cu = (p.nblocks + 255) / 256;
         const size_t fit = (size_t)(160 * 1024 / per_cu) - 1024 - 64 * sizeof(int) -
                            (layout == MBX_LAYOUT_VCR ? kFwdRecLdsBytes : 0);
