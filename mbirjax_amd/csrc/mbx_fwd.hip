@@ -111,14 +111,32 @@ int64_t fwd_partials(const DevGeom &g, int nviews, int layout) {
 //   [14] v/sdd at kref   [15] its step per slice
 constexpr int kFRec = 16;
 
-// Forward scratch after the geometry table: per-block FP64 partials of the
+// Binning blocks per view: a launch aims for kBinBlocks blocks (one block per
+// view leaves most CUs idle when a table chunk holds tens of thousands of
+// pixels over a few hundred views), each part holding kBinMinSlots slots or
+// more.  Non-decreasing in npix, so a workspace sized for a chunk holds the
+// part counts of every smaller pixel set.
+constexpr int kBinBlocks = 2048;
+constexpr int kBinMinSlots = 1024;
+static int bin_parts(int npix, int nviews) {
+    if (nviews <= 0) return 1;
+    return std::max(1, std::min((kBinBlocks + nviews - 1) / nviews, npix / kBinMinSlots));
+}
+// per (view, part, bucket) slot counts of a binning split into parts
+static size_t bin_count_bytes(const DevGeom &g, int chunk, int nviews) {
+    const int np = bin_parts(chunk, nviews);
+    return np > 1 ? al256((size_t)nviews * np * fwd_nbk(g) * sizeof(int)) : 0;
+}
+
+// Forward scratch after the geometry table: per-wave FP64 partials of the
 // squared norm, then per view the pixel list sorted by centre channel, the
-// bucket starts and the forward records in that order.
+// bucket starts and the forward records in that order, then the binning's
+// per-part bucket counts.
 int64_t fwd_aux_bytes(const DevGeom &g, int chunk, int nviews) {
     return (int64_t)(al256((size_t)fwd_part_blocks(g, nviews) * sizeof(double)) +
                      al256((size_t)nviews * chunk * sizeof(int)) +
                      al256((size_t)nviews * (fwd_nbk(g) + 1) * sizeof(int)) +
-                     al256((size_t)nviews * chunk * kFRec * sizeof(float)));
+                     al256((size_t)nviews * chunk * kFRec * sizeof(float)) + bin_count_bytes(g, chunk, nviews));
 }
 
 int64_t fwd_workspace_bytes(const DevGeom &g, int npix, int nviews) {
@@ -166,11 +184,39 @@ __device__ __forceinline__ T block_sum(T v, T *scratch) {
     return r;  // valid in thread 0
 }
 
+constexpr int kBinThreads = 1024;
+constexpr int kBinWaves = kBinThreads / 64;
+constexpr size_t kBinLds = 64 * 1024;  // counters [nseg][nbk]
+
+// Binning split into parts, first pass: block view * nparts + part counts the
+// slots of its part per centre-channel bucket into gtot[view][part][bucket]
+// (counts, so the atomics' order does not matter).
+__global__ __launch_bounds__(kBinThreads) void fwd_bin_count_kernel(DevGeom g, const int *__restrict__ NC, int npix, int nbk,
+                                                             int nparts, int *__restrict__ gtot) {
+    extern __shared__ int cnt[];  // [nbk]
+    const int v = blockIdx.x / nparts, part = blockIdx.x - v * nparts;
+    const int *ncv = NC + (size_t)v * npix;
+    for (int i = threadIdx.x; i < nbk; i += blockDim.x) cnt[i] = 0;
+    __syncthreads();
+    const int pb = (int)((int64_t)npix * part / nparts), pe = (int)((int64_t)npix * (part + 1) / nparts);
+    for (int s = pb + (int)threadIdx.x; s < pe; s += blockDim.x) {
+        const int b = ncv[s] + g.rh;
+        if (b >= 0 && b < nbk) atomicAdd(&cnt[b], 1);
+    }
+    __syncthreads();
+    int *dst = gtot + (size_t)blockIdx.x * nbk;
+    for (int i = threadIdx.x; i < nbk; i += blockDim.x) dst[i] = cnt[i];
+}
+
 // Per view: the subset pixels sorted by centre channel n_c (bucket n_c + r),
 // ascending pixel slot within a bucket, the bucket starts, and the forward
 // records in that order.  Pixels whose taps miss the detector are dropped.
-// Stable counting sort: wave s (s < nseg) owns the contiguous slot range
-// [s*npix/nseg, (s+1)*npix/nseg); pass 1 counts its pixels per bucket, a scan
+// Stable counting sort.  A view's slots are split into nparts contiguous
+// parts, one block each (blockIdx = view * nparts + part); with more than one
+// part, fwd_bin_count_kernel has counted every part's slots per bucket first
+// (gtot), so each block knows how many slots of a bucket the parts before it
+// hold.  Within a block, wave s (s < nseg) owns the s-th of nseg contiguous
+// ranges of the part's slots; pass 1 counts its pixels per bucket, a scan
 // over (bucket, segment) gives every segment its cursor in each bucket, and
 // pass 2 walks the range 64 slots at a time in order, sorting the round's
 // (bucket, lane) keys across the wave and ranking each slot within its bucket,
@@ -178,23 +224,23 @@ __device__ __forceinline__ T block_sum(T v, T *scratch) {
 // each bucket whatever the atomics' timing; the position goes back to the
 // slot's own lane, which loaded the slot's table record while the round was
 // sorted and writes its forward record there.
-constexpr int kBinThreads = 1024;
-constexpr int kBinWaves = kBinThreads / 64;
-constexpr size_t kBinLds = 64 * 1024;  // counters [nseg][nbk]
 __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const float *__restrict__ T,
                                                               const int *__restrict__ NC, int npix, int nbk,
-                                                              int vmode, int nseg, int *__restrict__ list,
+                                                              int vmode, int nseg, int nparts,
+                                                              const int *__restrict__ gtot, int *__restrict__ list,
                                                               int *__restrict__ cstart, float *__restrict__ FR) {
     extern __shared__ int cnt[];  // [nseg][nbk]
     const int rad = g.rh;
-    const int v = blockIdx.x, tid = threadIdx.x;
+    const int v = blockIdx.x / nparts, part = blockIdx.x - v * nparts, tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int *ncv = NC + (size_t)v * npix;
     int *lg = list + (size_t)v * npix;
     int *cs = cstart + (size_t)v * (nbk + 1);
     for (int i = tid; i < nseg * nbk; i += blockDim.x) cnt[i] = 0;
     __syncthreads();
-    const int sb = (int)((int64_t)npix * wave / nseg), se = (int)((int64_t)npix * (wave + 1) / nseg);
+    const int pb = (int)((int64_t)npix * part / nparts);
+    const int pn = (int)((int64_t)npix * (part + 1) / nparts) - pb;
+    const int sb = pb + (int)((int64_t)pn * wave / nseg), se = pb + (int)((int64_t)pn * (wave + 1) / nseg);
     int *cw = cnt + (size_t)wave * nbk;  // this wave's counters (wave < nseg)
     if (wave < nseg) {  // pass 1
         for (int s = sb + lane; s < se; s += 64) {
@@ -207,8 +253,12 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
     const int per = (nbk + blockDim.x - 1) / blockDim.x;
     const int b0 = tid * per, b1 = min(nbk, b0 + per);
     int run = 0;
-    for (int b = b0; b < b1; ++b)
-        for (int q = 0; q < nseg; ++q) run += cnt[q * nbk + b];
+    for (int b = b0; b < b1; ++b) {  // the view's slots in bucket b
+        if (nparts == 1)
+            for (int q = 0; q < nseg; ++q) run += cnt[q * nbk + b];
+        else
+            for (int p = 0; p < nparts; ++p) run += gtot[((size_t)v * nparts + p) * nbk + b];
+    }
     __shared__ int wsum[kBinWaves];
     int incl = run;
     for (int o = 1; o < 64; o <<= 1) {
@@ -220,14 +270,23 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
     int base = incl - run;
     for (int w = 0; w < wave; ++w) base += wsum[w];
     for (int b = b0; b < b1; ++b) {
-        cs[b] = base;
+        if (part == 0) cs[b] = base;
+        int cur = base, tot = 0;
+        if (nparts > 1) {  // the slots of the parts before this one come first
+            for (int p = 0; p < nparts; ++p) {
+                const int c = gtot[((size_t)v * nparts + p) * nbk + b];
+                if (p < part) cur += c;
+                tot += c;
+            }
+        }
         for (int q = 0; q < nseg; ++q) {  // segment q's cursor in bucket b
             const int c = cnt[q * nbk + b];
-            cnt[q * nbk + b] = base;
-            base += c;
+            cnt[q * nbk + b] = cur;
+            cur += c;
         }
+        base += nparts > 1 ? tot : cur - base;
     }
-    if (tid == blockDim.x - 1) cs[nbk] = base;
+    if (part == 0 && tid == blockDim.x - 1) cs[nbk] = base;
     __syncthreads();
     // forward record of slot s at list position i
     auto write_record = [&](int i, const float4 &R0, const float4 &R1, const float4 &R2, const float4 &R3) {
@@ -976,8 +1035,18 @@ int launch_fwd_tab(const DevGeom &g, const float *T, const int *NC, const float 
         // as many pass segments as the counters fit, up to one per wave
         const int nseg = std::min<int>(kBinWaves, (int)(kBinLds / sizeof(int)) / nbk);
         const size_t lds = (size_t)nseg * nbk * sizeof(int);
-        hipLaunchKernelGGL(fwd_bin_kernel, dim3(nviews), dim3(kBinThreads), lds, st, g, T, NC, npix, nbk, vm, nseg,
-                           list, cstart, FR);
+        // parts per view; their counts follow the records (bin_count_bytes)
+        const int np = bin_parts(npix, nviews);
+        if (np > bin_parts(chunk, nviews)) return fail(MBX_EINVAL, "forward binning split beyond its workspace");
+        int *gtot = nullptr;
+        if (np > 1) {
+            gtot = (int *)((char *)FR + al256((size_t)nviews * chunk * kFRec * sizeof(float)));
+            hipLaunchKernelGGL(fwd_bin_count_kernel, dim3(nviews * np), dim3(kBinThreads), (size_t)nbk * sizeof(int),
+                               st, g, NC, npix, nbk, np, gtot);
+            MBX_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(fwd_bin_kernel, dim3(nviews * np), dim3(kBinThreads), lds, st, g, T, NC, npix, nbk, vm, nseg,
+                           np, gtot, list, cstart, FR);
         MBX_LAUNCH_CHECK();
     }
     if (!(steps & kFwdProject)) return MBX_OK;

@@ -191,6 +191,21 @@ def test_back_projection_wave_layouts(gpu):
         assert r <= TOL
 
 
+@pytest.mark.parametrize("n", [2047, 3071, 4096])
+def test_forward_binning_in_parts(gpu, n):
+    """Pixel sets of 2047-4096 over 12 views are binned by several blocks per
+    view (1, 2 and 4 parts of at least 1024 slots); a 20-channel detector that
+    many of the 64 x 64 pixels miss in some views, and part bounds that split
+    channel buckets.  Forward projection against the FP64 oracle."""
+    m, g = make_cone(12, 16, 20, 96.0, 48.0, recon_shape=(64, 64, 8))
+    rng = np.random.default_rng(11)
+    pix = np.sort(rng.permutation(64 * 64)[:n]).astype(np.int32)
+    x = rng.standard_normal((n, 8)).astype(np.float32)
+    r = rel(m.sparse_forward_project(x, pix), op.forward(g, x, pix))
+    print("forward projection of %d pixels binned in parts: rel-L2 %.3e" % (n, r), flush=True)
+    assert r <= TOL
+
+
 @pytest.mark.parametrize("case", ["cone_m2", "parallel", "translation"])
 def test_adjoint_fp32(gpu, case):
     m, g = CASES[case]()
