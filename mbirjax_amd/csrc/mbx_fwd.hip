@@ -361,14 +361,27 @@ __global__ __launch_bounds__(kBinThreads) void fwd_bin_kernel(DevGeom g, const f
 #pragma unroll
         for (int q = 0; q < 4; ++q) dst[q] = make_float4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
     };
-    if (wave < nseg) {  // pass 2: place in slot order; each lane writes one slot's record
+    if (wave < nseg && sb < se) {  // pass 2: place in slot order; each lane writes one slot's record
+        // This lane's slot of a round: its bucket and table record are loaded
+        // a round ahead (lanes past the range load a valid slot, unused).
+        const float *Tv = T + (size_t)v * npix * kRec;
+        auto load_round = [&](int r0, float4 &A, float4 &B, float4 &C, float4 &D, int &nc) {
+            const int sc = min(r0 + lane, se - 1);
+            const float4 *r4 = (const float4 *)(Tv + (size_t)sc * kRec);
+            A = r4[0];
+            B = r4[1];
+            C = r4[2];
+            D = r4[3];
+            nc = ncv[sc];
+        };
+        float4 N0, N1, N2, N3;
+        int nnc;
+        load_round(sb, N0, N1, N2, N3, nnc);
         for (int s0 = sb; s0 < se; s0 += 64) {
             const int s = s0 + lane;
-            const int sc = min(s, se - 1);  // (lanes past the range load a valid slot, unused)
-            // this lane's slot: its table record is loaded now, while the round is sorted
-            const float4 *r4 = (const float4 *)(T + ((size_t)v * npix + sc) * kRec);
-            const float4 R0 = r4[0], R1 = r4[1], R2 = r4[2], R3 = r4[3];
-            const int b = s < se ? ncv[sc] + rad : -1;
+            const float4 R0 = N0, R1 = N1, R2 = N2, R3 = N3;
+            const int b = s < se ? nnc + rad : -1;
+            if (s0 + 64 < se) load_round(s0 + 64, N0, N1, N2, N3, nnc);
             const bool ok = b >= 0 && b < nbk;
             // sort the round's (bucket, lane) keys across the wave (bitonic
             // network over lane shuffles): equal buckets become adjacent, in lane
